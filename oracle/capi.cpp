@@ -1,0 +1,337 @@
+// Oracle (test infrastructure only): C entry points for ctypes.  Loaded by tests/ and by
+// bench.py's cpu_baseline leg; never by the product.
+#include "oracle.hpp"
+
+#include <cmath>
+#include <cstring>
+#include <exception>
+#include <stdexcept>
+#include <string>
+
+using namespace orc;
+
+namespace
+{
+    thread_local std::string g_err;
+
+    template <class F>
+    int guard(F &&f)
+    {
+        try
+        {
+            f();
+            return 0;
+        }
+        catch (const std::exception &e)
+        {
+            g_err = e.what();
+            return -1;
+        }
+    }
+
+    std::vector<std::vector<BasisShell>> unpack(int natoms, int nsh, const int *atom, const int *l,
+                                                const int *nprim, const double *exps,
+                                                const double *coeffs)
+    {
+        std::vector<std::vector<BasisShell>> atoms(natoms);
+        size_t o = 0;
+        for (int i = 0; i < nsh; i++)
+        {
+            if (atom[i] < 0 || atom[i] >= natoms)
+                throw std::runtime_error("shell atom index out of range");
+            if (i > 0 && atom[i] < atom[i - 1])
+                throw std::runtime_error("shells must be ordered by atom");
+            BasisShell b;
+            b.l = l[i];
+            b.exps.assign(exps + o, exps + o + nprim[i]);
+            b.coeffs.assign(coeffs + o, coeffs + o + nprim[i]);
+            o += nprim[i];
+            atoms[atom[i]].push_back(b);
+        }
+        return atoms;
+    }
+
+    double absSum(const std::vector<double> &v)
+    {
+        double s = 0;
+        for (double x : v)
+            s += std::fabs(x);
+        return s;
+    }
+}
+
+extern "C"
+{
+    const char *orc_last_error() { return g_err.c_str(); }
+
+    void orc_boys(int n, double x, double *out)
+    {
+        BoysGrid g(n);
+        calcBoysF(n, x, g, out);
+    }
+
+    // Boys values for many x at one n (tests of the GPU kernel).
+    void orc_boys_many(int n, long cnt, const double *x, double *out)
+    {
+        BoysGrid g(n);
+#pragma omp parallel for
+        for (long i = 0; i < cnt; i++)
+            calcBoysF(n, x[i], g, out + i * (n + 1));
+    }
+
+    void orc_rints3d(int l, double p, const double *xyz, const double *fnx, double *out)
+    {
+        auto r = calcRInts3D(l, p, xyz, fnx);
+        std::memcpy(out, r.data(), r.size() * sizeof(double));
+    }
+
+    int orc_sph_trafo(int l, int cap, int *sph, int *cart, double *val)
+    {
+        int n = -1;
+        guard([&]
+              {
+                  auto t = sphericalTrafo(l);
+                  n = static_cast<int>(t.size());
+                  for (int i = 0; i < n && i < cap; i++)
+                  {
+                      sph[i] = std::get<0>(t[i]);
+                      cart[i] = std::get<1>(t[i]);
+                      val[i] = std::get<2>(t[i]);
+                  }
+              });
+        return n;
+    }
+
+    double orc_pure_primitive_norm(int l, double e) { return purePrimitiveNorm(l, e); }
+
+    void orc_ecoeffs_recurrence1(double one_o_2a, int l, double *out)
+    {
+        auto e = ecoeffsRecurrence1(one_o_2a, l);
+        std::memcpy(out, e.data(), e.size() * sizeof(double));
+    }
+
+    void orc_ecoeffs_recurrence2(double a, double b, int la, int lb, double PA, double PB,
+                                 double Kab, double *out)
+    {
+        auto e = ecoeffsRecurrence2(a, b, la, lb, PA, PB, Kab);
+        std::memcpy(out, e.data(), e.size() * sizeof(double));
+    }
+
+    void orc_ecoeffs_recurrence2_n1(double a, double b, int la, int lb, double PA, double PB,
+                                    double Kab, double A, double B, double *out)
+    {
+        auto e0 = ecoeffsRecurrence2(a, b, la, lb, PA, PB, Kab);
+        auto e1 = ecoeffsRecurrence2_n1(a, b, la, lb, A, B, e0);
+        std::memcpy(out, e1.data(), e1.size() * sizeof(double));
+    }
+
+    // 3 consecutive blocks (x, y, z), each (la+1)(lb+1)(la+lb+1)
+    void orc_ecoeffs_primitive_pair(double a, double b, int la, int lb, const double *xa,
+                                    const double *xb, const double *A, const double *B,
+                                    int deriv, double *out)
+    {
+        auto e0 = ecoeffsPrimitivePair(a, b, la, lb, xa, xb);
+        auto e = deriv ? ecoeffsPrimitivePair_n1(a, b, la, lb, A, B, e0) : e0;
+        size_t n = e[0].size();
+        for (int k = 0; k < 3; k++)
+            std::memcpy(out + k * n, e[k].data(), n * sizeof(double));
+    }
+
+    double orc_ecoeffs_shell_abs_sum(int l, int n, const double *exps)
+    {
+        auto e = ecoeffsShell(l, std::vector<double>(exps, exps + n));
+        double s = 0;
+        for (auto &v : e)
+            s += absSum(v);
+        return s;
+    }
+
+    void *orc_structure_create(int natoms, const double *coords_bohr, int nsh, const int *atom,
+                               const int *l, const int *nprim, const double *exps,
+                               const double *coeffs, int nsh_aux, const int *atom_aux,
+                               const int *l_aux, const int *nprim_aux, const double *exps_aux,
+                               const double *coeffs_aux)
+    {
+        Structure *s = nullptr;
+        guard([&]
+              {
+                  std::vector<std::array<double, 3>> xyz(natoms);
+                  for (int i = 0; i < natoms; i++)
+                      xyz[i] = {coords_bohr[3 * i], coords_bohr[3 * i + 1], coords_bohr[3 * i + 2]};
+                  auto main = unpack(natoms, nsh, atom, l, nprim, exps, coeffs);
+                  if (nsh_aux > 0)
+                  {
+                      auto aux = unpack(natoms, nsh_aux, atom_aux, l_aux, nprim_aux, exps_aux,
+                                        coeffs_aux);
+                      s = new Structure(makeStructure(xyz, main, &aux));
+                  }
+                  else
+                      s = new Structure(makeStructure(xyz, main, nullptr));
+              });
+        return s;
+    }
+
+    void orc_structure_destroy(void *h) { delete static_cast<Structure *>(h); }
+
+    void orc_structure_dims(void *h, long *nbf, long *nbf_aux, long *natoms)
+    {
+        auto *s = static_cast<Structure *>(h);
+        *nbf = s->nbf;
+        *nbf_aux = s->nbf_aux;
+        *natoms = s->natoms;
+    }
+
+    // shell normalisation constants, concatenated in shell order (main basis)
+    void orc_structure_norms(void *h, double *out)
+    {
+        auto *s = static_cast<Structure *>(h);
+        size_t o = 0;
+        for (auto &sh : s->shells)
+            for (double v : sh.norms)
+                out[o++] = v;
+    }
+
+    int orc_eri4(void *h, double *out)
+    {
+        return guard([&]
+                     {
+                         auto v = eri4(*static_cast<Structure *>(h));
+                         std::memcpy(out, v.data(), v.size() * sizeof(double));
+                     });
+    }
+
+    int orc_eri4_diag(void *h, double *out)
+    {
+        return guard([&]
+                     {
+                         auto v = eri4Diagonal(*static_cast<Structure *>(h));
+                         std::memcpy(out, v.data(), v.size() * sizeof(double));
+                     });
+    }
+
+    int orc_eri3(void *h, double *out)
+    {
+        return guard([&]
+                     {
+                         auto v = eri3(*static_cast<Structure *>(h));
+                         std::memcpy(out, v.data(), v.size() * sizeof(double));
+                     });
+    }
+
+    int orc_eri2(void *h, double *out)
+    {
+        return guard([&]
+                     {
+                         auto v = eri2(*static_cast<Structure *>(h));
+                         std::memcpy(out, v.data(), v.size() * sizeof(double));
+                     });
+    }
+
+    int orc_eri2_diag(void *h, double *out)
+    {
+        return guard([&]
+                     {
+                         auto v = eri2Diagonal(*static_cast<Structure *>(h));
+                         std::memcpy(out, v.data(), v.size() * sizeof(double));
+                     });
+    }
+
+    // Sum of |values| over every kernel call of the reference's "deploy" tests
+    // (tests_ints.cpp:873-1053).  which: 0 ERI4, 1 ERI3, 2 ERI2, 3 ERI4D1, 4 ERI3D1, 5 ERI2D1.
+    int orc_deploy_sum(void *h, int which, int use_symm, double *sum)
+    {
+        return guard([&]
+                     {
+                         const Structure &s = *static_cast<Structure *>(h);
+                         double tot = 0;
+                         if (which == 0 || which == 3)
+                         {
+                             auto sp = shellPairData(use_symm != 0, s);
+                             for (auto &A : sp)
+                                 for (auto &C : sp)
+                                 {
+                                     if (which == 0)
+                                     {
+                                         ERI4Kernel k(A, C);
+                                         for (size_t p = 0; p < A.n_pairs; p++)
+                                             for (size_t q = 0; q < C.n_pairs; q++)
+                                                 tot += absSum(k(p, q, A, C));
+                                     }
+                                     else
+                                     {
+                                         ERI4D1Kernel k(A, C);
+                                         for (size_t p = 0; p < A.n_pairs; p++)
+                                             for (size_t q = 0; q < C.n_pairs; q++)
+                                                 tot += absSum(k(p, q, A, C));
+                                     }
+                                 }
+                         }
+                         else if (which == 1 || which == 4)
+                         {
+                             auto sp = shellPairData(use_symm != 0, s);
+                             auto sh = shellDataAux(s);
+                             for (auto &A : sp)
+                                 for (auto &C : sh)
+                                 {
+                                     if (which == 1)
+                                     {
+                                         ERI3Kernel k(A, C);
+                                         for (size_t p = 0; p < A.n_pairs; p++)
+                                             for (size_t q = 0; q < C.n_shells; q++)
+                                                 tot += absSum(k(p, q, A, C));
+                                     }
+                                     else
+                                     {
+                                         ERI3D1Kernel k(A, C);
+                                         for (size_t p = 0; p < A.n_pairs; p++)
+                                             for (size_t q = 0; q < C.n_shells; q++)
+                                                 tot += absSum(k(p, q, A, C));
+                                     }
+                                 }
+                         }
+                         else
+                         {
+                             auto sh = shellDataAux(s);
+                             for (auto &A : sh)
+                                 for (auto &B : sh)
+                                 {
+                                     if (which == 2)
+                                     {
+                                         ERI2Kernel k(A, B);
+                                         for (size_t p = 0; p < A.n_shells; p++)
+                                             for (size_t q = 0; q < B.n_shells; q++)
+                                                 tot += absSum(k(p, q, A, B));
+                                     }
+                                     else
+                                     {
+                                         ERI2D1Kernel k(A, B);
+                                         for (size_t p = 0; p < A.n_shells; p++)
+                                             for (size_t q = 0; q < B.n_shells; q++)
+                                                 tot += absSum(k(p, q, A, B));
+                                     }
+                                 }
+                         }
+                         *sum = tot;
+                     });
+    }
+
+    // stats[0] = screened unique quartets processed, stats[1] = algorithmic FLOPs
+    int orc_jk(void *h, const double *D, double *J, double *K, double thresh, int stride,
+               double *stats)
+    {
+        return guard([&]
+                     {
+                         auto st = jk(*static_cast<Structure *>(h), D, J, K, thresh, stride);
+                         if (stats)
+                         {
+                             stats[0] = st.n_quartets;
+                             stats[1] = st.flops;
+                         }
+                     });
+    }
+
+    int orc_jk_grad(void *h, const double *D, double *grad, double thresh)
+    {
+        return guard([&] { jkGradient(*static_cast<Structure *>(h), D, grad, thresh); });
+    }
+}

@@ -1,0 +1,109 @@
+"""The oracle against the reference's own known-answer tests (test/tests_ints.cpp).
+
+Scalar KATs are printed in the reference with 12 decimals and checked there at absolute 1e-12
+(tests_ints.cpp:15); they are checked the same way here.  The ERI checksums are sums of |x| over
+10^4-10^7 values, which move at ~1e-13 relative with summation order alone (the reference's own
+eri4 and deployERI4Kernel sums of the same tensor differ by 1.4e-13 relative), so they are checked
+at 1e-12 relative: a dropped term, a wrong sign or a transposed operand changes them at >=1e-6.
+"""
+import numpy as np
+import pytest
+
+from lible_amd import basis as B
+from tests.conftest import H2O, O3, basis_path
+
+ABS = 1e-12
+REL = 1e-12
+
+
+def mol(main, geom, aux=None):
+    nrs, xyz = geom
+    return B.molecule_from_basis_files(basis_path(main), nrs, xyz,
+                                       basis_path(aux) if aux else None)
+
+
+def test_calc_boys_f(oracle):                                   # tests_ints.cpp:1296-1313
+    assert abs(oracle.boys(5, 2.5).sum() - 0.728561124180) < ABS
+
+
+def test_calc_rints3d(oracle):                                  # tests_ints.cpp:1315-1335
+    r = oracle.rints3d(5, 0.342412347, [0.5, -1.2, 3.4], oracle.boys(5, 2.5))
+    assert abs(np.abs(r).sum() - 2.814391945942) < ABS
+
+
+def test_pure_primitive_norm(oracle):                           # tests_ints.cpp:1359-1371
+    assert abs(oracle.pure_primitive_norm(4, 1.70325241) - 4.813408368164) < ABS
+
+
+def test_ecoeffs_recurrence1(oracle):                           # tests_ints.cpp:1150-1165
+    s = np.abs(oracle.ecoeffs_recurrence1(19.0034512342, 4)).sum()
+    assert abs(s - 181021.807437296084) <= REL * 181021.8
+
+
+def test_ecoeffs_recurrence2(oracle):                           # tests_ints.cpp:1167-1181
+    e = oracle.ecoeffs_recurrence2(0.0023445, 1.123621, 2, 3, 0.5113241, 1.2345123, 0.003451234)
+    assert abs(np.abs(e).sum() - 0.218834692700) < ABS
+
+
+def test_ecoeffs_recurrence2_n1(oracle):                        # tests_ints.cpp:1183-1208
+    e = oracle.ecoeffs_recurrence2_n1(0.0023445, 1.123621, 2, 3, 0.5113241, 1.2345123,
+                                      0.003451234, -0.571235891, 1.612357121)
+    assert abs(np.abs(e).sum() - 0.053908505303) < ABS
+
+
+def test_ecoeffs_primitive_pair(oracle):                        # tests_ints.cpp:1225-1242
+    e = oracle.ecoeffs_primitive_pair(1230.02342162, 0.0023445, 4, 3, [0, 1.5, 0], [1, 0, -1])
+    assert abs(np.abs(e).sum() - 16.100457827896) < ABS
+
+
+def test_ecoeffs_primitive_pair_n1(oracle):                     # tests_ints.cpp:1244-1275
+    e = oracle.ecoeffs_primitive_pair(1230.02342162, 0.0023445, 4, 3, [0, 1.5, 0], [1, 0, -1],
+                                      [0.571235891, 1.612357121, -0.234512341],
+                                      [1.2345123, -0.5113241, 0.003451234], deriv=1)
+    assert abs(np.abs(e).sum() - 21.712096988812) < ABS
+
+
+def test_ecoeffs_shell(oracle):                                 # tests_ints.cpp:1277-1294
+    s = oracle.ecoeffs_shell_abs_sum(5, [0.95556865, 0.36628794, 0.14651517])
+    assert abs(s - 112231.357645556767) <= REL * 112231.36
+
+
+def test_spherical_trafo(oracle):                               # tests_ints.cpp:856-871
+    t = oracle.sph_trafo(8)
+    s = sum(abs(v) + i + j for i, j, v in t)
+    assert abs(s - 7005.923248093791) < ABS
+
+
+@pytest.mark.parametrize("name,main,geom,aux,ref", [
+    ("eri4", "def2-svp", O3, None, 8246.711763196334),                     # :748-764
+    ("eri4_diag", "def2-tzvp", O3, None, 306.806419250122),                # :712-728
+    ("eri3", "def2-tzvp", O3, "def2-qzvp-rifit", 25854.105851585416531),   # :730-746
+    ("eri2", "def2-svp", H2O, "def2-qzvp-rifit", 8801.334703460838),       # :694-710
+    ("eri2_diag", "def2-svp", H2O, "def2-universal-jkfit", 830.788976485982),  # :149-165
+])
+def test_eri_checksums(oracle, name, main, geom, aux, ref):
+    s = oracle.Structure(mol(main, geom, aux))
+    val = np.abs(getattr(s, name)()).sum()
+    assert abs(val - ref) <= REL * ref, (val, ref)
+
+
+@pytest.mark.parametrize("which,symm,ref", [
+    ("eri4", False, 8246.711763197583),        # deployERI4Kernel :873-900
+    ("eri4d1", True, 30546.780073140635),      # deployERI4D1Kernel :961-990
+])
+def test_deploy_eri4(oracle, which, symm, ref):
+    s = oracle.Structure(mol("def2-svp", O3))
+    val = s.deploy_sum(which, symm)
+    assert abs(val - ref) <= REL * ref, (val, ref)
+
+
+@pytest.mark.parametrize("which,ref", [
+    ("eri3", 12350.130354189232),     # deployERI3Kernel :902-930
+    ("eri2", 21014.512841176023),     # deployERI2Kernel :932-959
+    ("eri3d1", 78466.360848033844),   # deployERI3D1Kernel :992-1022
+    ("eri2d1", 39949.218421684251),   # deployERI2D1Kernel :1024-1053
+])
+def test_deploy_ri(oracle, which, ref):
+    s = oracle.Structure(mol("def2-svp", O3, "def2-universal-jkfit"))
+    val = s.deploy_sum(which, False)
+    assert abs(val - ref) <= REL * ref, (val, ref)

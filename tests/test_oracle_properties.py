@@ -1,0 +1,149 @@
+"""Pins of the oracle beyond the reference's checksums: closed forms, invariants and finite
+differences, chosen so that a dropped term, a wrong sign or a transposed index fails one of them.
+J, K, E and the gradient have no reference counterpart; they are pinned here against the dense
+eri4 tensor (itself pinned by the reference's eri4 checksum) and by central differences."""
+import math
+
+import numpy as np
+import pytest
+from scipy import integrate, special
+
+from lible_amd import basis as B
+
+
+def small_mol(seed=7, natoms=3, box=4.0, recipe=B.RECIPE_SMALL):
+    return B.synthetic_molecule(seed, natoms, box, recipe)
+
+
+def test_boys_closed_forms(oracle):
+    # F_m(0) = 1/(2m+1); F_0(T) = sqrt(pi/T)/2 erf(sqrt T); F_m by quadrature;
+    # downward identity F_m = (2T F_{m+1} + e^-T)/(2m+1)
+    f0 = oracle.boys(12, 0.0)
+    assert np.allclose(f0, 1.0 / (2 * np.arange(13) + 1), rtol=0, atol=0)
+    for T in [1e-3, 0.37, 2.5, 11.1, 29.99, 30.5, 57.0]:
+        f = oracle.boys(12, T)
+        ref0 = 0.5 * math.sqrt(math.pi / T) * special.erf(math.sqrt(T))
+        # 7-term Taylor + downward recursion: ~1e-13 relative (boys_function.cpp:95-115)
+        assert abs(f[0] - ref0) <= 1e-12 * ref0
+        for m in (0, 3, 7, 12):
+            q = integrate.quad(lambda t: t ** (2 * m) * math.exp(-T * t * t), 0, 1,
+                               epsabs=0, epsrel=1e-13)[0]
+            # above T = 30 the reference drops the e^-T terms of the asymptotic form
+            # (boys_function.cpp:88-94): the absolute error is then below e^-T
+            tol = 1e-12 * q if T <= 30 else math.exp(-T) + 1e-14 * q
+            assert abs(f[m] - q) <= tol, (T, m, f[m], q)
+        if T <= 30:
+            for m in range(12):
+                assert abs(f[m] - (2 * T * f[m + 1] + math.exp(-T)) / (2 * m + 1)) <= 1e-14 * f[m]
+
+
+def test_rints_finite_difference(oracle):
+    # R_{tuv}(X) = d^t/dX^t d^u/dY^u d^v/dZ^v R_000 with R_000 = F_0(p |X|^2)
+    p, X = 0.7, np.array([0.3, -0.5, 0.8])
+
+    def r000(x):
+        return oracle.boys(0, p * float(x @ x))[0]
+
+    R = oracle.rints3d(3, p, X, oracle.boys(3, p * float(X @ X)))
+    h = 1e-3
+    e = np.eye(3)
+    for ax, (t, u, v) in enumerate([(1, 0, 0), (0, 1, 0), (0, 0, 1)]):
+        fd = (r000(X + h * e[ax]) - r000(X - h * e[ax])) / (2 * h)
+        assert abs(R[t, u, v] - fd) < 1e-7
+    fd = (r000(X + h * e[0] + h * e[1]) - r000(X + h * e[0] - h * e[1]) -
+          r000(X - h * e[0] + h * e[1]) + r000(X - h * e[0] - h * e[1])) / (4 * h * h)
+    assert abs(R[1, 1, 0] - fd) < 1e-6
+
+
+def test_normalised_self_overlap(oracle):
+    # the shell norms make every spherical AO unit-normalised: eri-independent check through
+    # the (s|s) overlap closed form for an s shell
+    exps = np.array([3.0, 0.8, 0.2])
+    coefs = np.array([0.3, 0.5, 0.4])
+    mol = B.Molecule(np.zeros((1, 3)), B.Shells.from_lists([[(0, list(exps), list(coefs))]]))
+    s = oracle.Structure(mol)
+    N = s.norms()[0]
+    pn = (2 * exps / np.pi) ** 0.75
+    d = coefs * pn
+    S = sum(d[i] * d[j] * (np.pi / (exps[i] + exps[j])) ** 1.5 for i in range(3) for j in range(3))
+    assert abs(N * N * S - 1.0) < 1e-14
+
+
+def test_ssss_closed_form(oracle):
+    # (ss|ss) for single primitives: 2 pi^{5/2}/(pq sqrt(p+q)) K_ab K_cd F_0(alpha R_PQ^2) x norms
+    a, b, c, d = 1.3, 0.4, 0.9, 2.2
+    A, Bc, C, D = (np.array(v) for v in ([0, 0, 0], [0.5, 0.2, -0.1], [1.0, -0.4, 0.7],
+                                         [-0.3, 0.8, 1.1]))
+    shells = B.Shells.from_lists([[(0, [a], [1.0])], [(0, [b], [1.0])], [(0, [c], [1.0])],
+                                  [(0, [d], [1.0])]])
+    s = oracle.Structure(B.Molecule(np.array([A, Bc, C, D]), shells))
+    eri = s.eri4()
+    p, q = a + b, c + d
+    P, Q = (a * A + b * Bc) / p, (c * C + d * D) / q
+    Kab = math.exp(-a * b / p * float((A - Bc) @ (A - Bc)))
+    Kcd = math.exp(-c * d / q * float((C - D) @ (C - D)))
+    al = p * q / (p + q)
+    T = al * float((P - Q) @ (P - Q))
+    F0 = 0.5 * math.sqrt(math.pi / T) * special.erf(math.sqrt(T))
+    N = [(2 * e / math.pi) ** 0.75 for e in (a, b, c, d)]
+    ref = 2 * math.pi ** 2.5 / (p * q * math.sqrt(p + q)) * Kab * Kcd * F0 * np.prod(N)
+    assert abs(eri[0, 1, 2, 3] - ref) < 1e-14 * abs(ref)
+
+
+@pytest.fixture(scope="module")
+def small(oracle):
+    m = small_mol()
+    s = oracle.Structure(m)
+    return m, s, s.eri4()
+
+
+def test_eri4_symmetry_and_schwarz(small):
+    _, s, eri = small
+    for perm in [(1, 0, 2, 3), (0, 1, 3, 2), (2, 3, 0, 1)]:
+        assert np.array_equal(eri, eri.transpose(perm))
+    n = s.nbf
+    M = eri.reshape(n * n, n * n)
+    d = np.sqrt(np.abs(np.diag(M)))
+    assert np.all(np.abs(M) <= np.outer(d, d) * (1 + 1e-12) + 1e-15)
+    w = np.linalg.eigvalsh(M)
+    assert w.min() > -1e-10 * w.max()
+
+
+def test_jk_against_dense(small):
+    _, s, eri = small
+    n = s.nbf
+    D = B.density_from_orbitals(3, n, 5, 0.3)
+    J, K, st = s.jk(D, thresh=0.0)
+    Jr = np.einsum("mnkl,kl->mn", eri, D)
+    Kr = np.einsum("mnkl,nl->mk", eri, D)
+    assert np.allclose(J, Jr, rtol=0, atol=1e-12 * np.abs(Jr).max())
+    assert np.allclose(K, Kr, rtol=0, atol=1e-12 * np.abs(Kr).max())
+    assert np.array_equal(J, J.T) and np.array_equal(K, K.T)
+    # Schwarz screening at 1e-12 changes nothing visible here
+    J2, K2, _ = s.jk(D, thresh=1e-12)
+    assert np.abs(J2 - J).max() < 1e-9 and np.abs(K2 - K).max() < 1e-9
+
+
+def energy(s, D):
+    J, K, _ = s.jk(D, thresh=0.0)
+    return 0.5 * np.sum(D * J) - 0.25 * np.sum(D * K)
+
+
+def test_gradient_finite_difference(oracle):
+    m = small_mol(seed=11, natoms=3, box=3.0)
+    s = oracle.Structure(m)
+    D = B.density_from_orbitals(5, s.nbf, 4, 0.3)
+    g = s.jk_grad(D, thresh=0.0)
+    h = 1e-4
+    fd = np.zeros_like(g)
+    for ia in range(m.natoms):
+        for x in range(3):
+            cp, cm = m.coords.copy(), m.coords.copy()
+            cp[ia, x] += h
+            cm[ia, x] -= h
+            ep = energy(oracle.Structure(B.Molecule(cp, m.shells)), D)
+            em = energy(oracle.Structure(B.Molecule(cm, m.shells)), D)
+            fd[ia, x] = (ep - em) / (2 * h)
+    assert np.abs(g - fd).max() < 1e-7, np.abs(g - fd).max()
+    # translational invariance: forces sum to zero
+    assert np.abs(g.sum(axis=0)).max() < 1e-10
