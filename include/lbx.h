@@ -1,0 +1,95 @@
+/* lbx — MI355X-native SHARK electron-repulsion integrals: the C-ABI boundary.
+ *
+ * Plain C types only.  Every array argument marked "device" is a pointer into HBM on the
+ * calling process's current HIP device (hipMalloc / torch.cuda memory); "host" arrays are
+ * ordinary process memory.  Functions return 0 on success and a negative status otherwise;
+ * lbx_last_error() then describes the failure.  No exception crosses this boundary.
+ *
+ * Each entry point names the reference interface (MihkuU/Lible, src/lible/ints/...) whose
+ * behaviour it replaces.  The reference's eri4/eri3/eri2 return dense row-major tensors with AO
+ * order = atoms in input order, shells in basis order, spherical m order 0,+1,-1,+2,-2,...;
+ * the dense entry points here write exactly that layout.
+ */
+#ifndef LBX_H
+#define LBX_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LBX_OK 0
+#define LBX_ERR_INVALID (-1)   /* bad argument: sizes, null pointers, l out of range */
+#define LBX_ERR_HIP (-2)       /* a HIP runtime call failed */
+#define LBX_ERR_NOMEM (-3)     /* device allocation failed */
+#define LBX_ERR_STATE (-4)     /* call not valid for this context (e.g. no auxiliary basis) */
+
+/* One basis on a set of atoms, flattened.  Shells are ordered by atom (atoms in input order)
+ * and in basis order within an atom, as constructShells builds them (shell.cpp:89-135).
+ * Coordinates in bohr.  coeffs are the raw contraction coefficients; primitive and contracted
+ * normalisation are applied inside (shell.cpp:11-46, utils.cpp:6-10). */
+typedef struct lbx_basis {
+    int natoms;
+    const double *coords;      /* host, natoms*3, bohr */
+    int nshells;
+    const int *shell_atom;     /* host, nshells */
+    const int *shell_l;        /* host, nshells; main basis l <= 3, auxiliary l <= 5 */
+    const int *shell_nprim;    /* host, nshells */
+    const double *exps;        /* host, sum(nprim) */
+    const double *coeffs;      /* host, sum(nprim) */
+} lbx_basis;
+
+typedef struct lbx_context lbx_context;
+
+/* Replaces Structure(basis_atoms_t, [aux,] atomic_nrs, coords) (structure.cpp:66-92) and the
+ * per-class ShellPairData/ShellData + E-matrix set-up of the kernel functors
+ * (shell_pair_data.cpp:73-202, ecoeffs.cpp:253-403, eri_kernels.cpp:1101-1162).  Builds the
+ * normalised shells, screened primitive pairs (threshold 1e-15, shell_pair_data.hpp:55-57) and
+ * the SHARK E matrices on the current device.  aux may be NULL. */
+int lbx_create(lbx_context **ctx, const lbx_basis *main_basis, const lbx_basis *aux_basis);
+int lbx_destroy(lbx_context *ctx);
+const char *lbx_last_error(void);
+
+/* Structure::getDimAO / getDimAOAux / getNAtoms (structure.hpp:79-91). */
+int lbx_dims(const lbx_context *ctx, long *nbf, long *nbf_aux, long *natoms);
+
+/* All work of a context is queued on this stream (default: the null stream).  hipStream_t. */
+int lbx_set_stream(lbx_context *ctx, void *stream);
+int lbx_synchronize(lbx_context *ctx);
+
+/* Boys function F_0..F_n at count points (device x[count] -> device out[count*(n+1)]).
+ * Replaces calcBoysF (boys_function.cpp:71-117); n <= 16. */
+int lbx_boys(int n, long count, const double *x, double *out, void *stream);
+
+/* Dense spherical ERIs, device outputs, row-major:
+ *   eri4      (nbf,nbf,nbf,nbf)   replaces eri4()         (eri4.cpp:38-95)
+ *   eri4_diag (nbf,nbf) (mn|mn)   replaces eri4Diagonal() (eri4.cpp:152-174)
+ *   eri3      (nbf,nbf,naux)      replaces eri3()         (eri3.cpp:7-46)
+ *   eri2      (naux,naux)         replaces eri2()         (eri2.cpp:27-72)
+ *   eri2_diag (naux)              replaces eri2Diagonal() (eri2.cpp:74-96) */
+int lbx_eri4(lbx_context *ctx, double *out);
+int lbx_eri4_diag(lbx_context *ctx, double *out);
+int lbx_eri3(lbx_context *ctx, double *out);
+int lbx_eri2(lbx_context *ctx, double *out);
+int lbx_eri2_diag(lbx_context *ctx, double *out);
+
+/* Coulomb and exchange matrices, no reference counterpart (the reference stops at the dense
+ * tensor).  D, J, K: device (nbf,nbf) row-major, D symmetric.
+ *   J_mn = sum_kl (mn|kl) D_kl,   K_mk = sum_nl (mn|kl) D_nl
+ * over unique shell quartets whose Schwarz bound Q_ab*Q_cd >= schwarz_thresh
+ * (Q_ab = sqrt(max |(ab|ab)|)).  precision: LBX_FP64, or LBX_MIXED (quartets with bound below
+ * mixed_thresh evaluated in FP32 with FP64 accumulation). */
+#define LBX_FP64 0
+#define LBX_MIXED 1
+typedef struct lbx_jk_stats {
+    double quartets;       /* shell quartets evaluated */
+    double flops;          /* algorithmic FLOPs of those quartets */
+    double quartets_fp32;  /* of which evaluated in FP32 (mixed precision) */
+} lbx_jk_stats;
+int lbx_jk(lbx_context *ctx, const double *D, double *J, double *K, double schwarz_thresh,
+           int precision, double mixed_thresh, lbx_jk_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,88 @@
+"""Builds lible_amd/liblbx.so (HIP, gfx950) in-tree with hipcc.
+
+Usage: python -m lible_amd.build [-j N]
+The quartet kernels are instantiated by one source (csrc/instances.hip) compiled once per
+instantiation unit, so the units build in parallel.
+"""
+from __future__ import annotations
+
+import argparse
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+OUT = os.path.join(HERE, "liblbx.so")
+
+N_UNITS = 12
+
+FLAGS = [
+    "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-munsafe-fp-atomics", "-ffp-contract=fast",
+    "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
+]
+
+
+def hipcc() -> str:
+    for c in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if c and os.path.exists(c):
+            return c
+    raise RuntimeError("hipcc not found")
+
+
+def _compile(src: str, obj: str, defines: list[str]) -> tuple[str, float]:
+    t = time.time()
+    cmd = [hipcc(), "-x", "hip", "-c", src, "-o", obj] + FLAGS + defines
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return obj, time.time() - t
+
+
+def build(jobs: int | None = None, verbose: bool = True) -> str:
+    os.makedirs(BUILD, exist_ok=True)
+    jobs = jobs or max(1, min(os.cpu_count() or 4, 8))
+    work = [
+        (os.path.join(CSRC, "host_setup.cpp"), os.path.join(BUILD, "host_setup.o"), []),
+        (os.path.join(CSRC, "ecoef.hip"), os.path.join(BUILD, "ecoef.o"), []),
+        (os.path.join(CSRC, "capi.hip"), os.path.join(BUILD, "capi.o"), []),
+    ]
+    for u in range(N_UNITS):
+        work.append((os.path.join(CSRC, "instances.hip"), os.path.join(BUILD, f"inst_{u}.o"),
+                     [f"-DLBX_UNIT={u}"]))
+    # heaviest units first
+    work.sort(key=lambda w: -int(w[2][0].split("=")[1]) if w[2] else 1)
+    t0 = time.time()
+    with cf.ThreadPoolExecutor(jobs) as ex:
+        futs = [ex.submit(_compile, *w) for w in work]
+        for f in cf.as_completed(futs):
+            obj, dt = f.result()
+            if verbose:
+                print(f"  {os.path.basename(obj)}  {dt:.1f}s", flush=True)
+    objs = [w[1] for w in work]
+    tmp = OUT + ".tmp"
+    cmd = [hipcc(), "-shared", "-o", tmp, "--offload-arch=gfx950", "-fPIC"] + objs
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, OUT)
+    if verbose:
+        print(f"built {OUT} in {time.time() - t0:.1f}s", flush=True)
+    return OUT
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("-j", type=int, default=None)
+    a = ap.parse_args(argv)
+    build(a.j)
+
+
+if __name__ == "__main__":
+    sys.exit(main())

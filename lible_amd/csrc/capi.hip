@@ -1,0 +1,611 @@
+// The C-ABI of lbx (include/lbx.h): context set-up, Schwarz screening, task tables and the
+// drivers that launch the quartet kernels class pair by class pair.
+#include "quartet.hpp"
+#include "registry.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <tuple>
+
+namespace lbx
+{
+    namespace
+    {
+        thread_local std::string g_err;
+
+        // scratch device buffer owned by one call
+        struct DevBuf
+        {
+            double *p = nullptr;
+            explicit DevBuf(size_t n) { LBX_HIP(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(double))); }
+            ~DevBuf()
+            {
+                if (p)
+                    (void)hipFree(p);
+            }
+            DevBuf(const DevBuf &) = delete;
+            DevBuf &operator=(const DevBuf &) = delete;
+        };
+
+        template <class F>
+        int guarded(F &&f)
+        {
+            try
+            {
+                f();
+                return LBX_OK;
+            }
+            catch (const Error &e)
+            {
+                g_err = e.what();
+                return e.code;
+            }
+            catch (const std::exception &e)
+            {
+                g_err = e.what();
+                return LBX_ERR_INVALID;
+            }
+        }
+
+        __global__ void fillKernel(double *p, long long n, double v)
+        {
+            for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+                 i += static_cast<long long>(gridDim.x) * blockDim.x)
+                p[i] = v;
+        }
+
+        // M <- M + M^T for an n x n row-major matrix
+        __global__ void symmetrizeKernel(double *M, int n)
+        {
+            const long long tot = static_cast<long long>(n) * n;
+            for (long long e = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; e < tot;
+                 e += static_cast<long long>(gridDim.x) * blockDim.x)
+            {
+                const int i = static_cast<int>(e / n), j = static_cast<int>(e % n);
+                if (j < i)
+                    continue;
+                const double v = M[static_cast<long long>(i) * n + j] + M[static_cast<long long>(j) * n + i];
+                M[static_cast<long long>(i) * n + j] = v;
+                M[static_cast<long long>(j) * n + i] = v;
+            }
+        }
+
+        void fill(double *p, long long n, double v, hipStream_t s)
+        {
+            if (n <= 0)
+                return;
+            long long b = (n + 255) / 256;
+            hipLaunchKernelGGL(fillKernel, dim3(static_cast<int>(std::min(b, 65536LL))), dim3(256), 0, s,
+                               p, n, v);
+            LBX_HIP(hipGetLastError());
+        }
+
+        // Boys table, one copy per device for the life of the process
+        const double *boysTableDevice()
+        {
+            static std::mutex m;
+            static std::map<int, double *> tabs;
+            int dev = 0;
+            LBX_HIP(hipGetDevice(&dev));
+            std::lock_guard<std::mutex> lk(m);
+            auto it = tabs.find(dev);
+            if (it != tabs.end())
+                return it->second;
+            auto h = boysTable();
+            double *d = nullptr;
+            LBX_HIP(hipMalloc(&d, h.size() * sizeof(double)));
+            LBX_HIP(hipMemcpy(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+            tabs[dev] = d;
+            return d;
+        }
+
+        template <int N>
+        __global__ void boysKernel(long long cnt, const double *x, double *out, const double *tab)
+        {
+            for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < cnt;
+                 i += static_cast<long long>(gridDim.x) * blockDim.x)
+            {
+                double F[N + 1];
+                boysF<N>(x[i], tab, F);
+                for (int k = 0; k <= N; k++)
+                    out[i * (N + 1) + k] = F[k];
+            }
+        }
+
+        template <int N>
+        void launchBoys(long long cnt, const double *x, double *out, const double *tab, hipStream_t s)
+        {
+            long long b = (cnt + 255) / 256;
+            hipLaunchKernelGGL((boysKernel<N>), dim3(static_cast<int>(std::min(b, 65536LL))), dim3(256),
+                               0, s, cnt, x, out, tab);
+        }
+
+        using BoysFn = void (*)(long long, const double *, double *, const double *, hipStream_t);
+        template <int... Ns>
+        constexpr std::array<BoysFn, sizeof...(Ns)> boysFns(std::integer_sequence<int, Ns...>)
+        {
+            return {&launchBoys<Ns>...};
+        }
+
+        // ------------------------------------------------------------------ task tables
+        struct TaskTable
+        {
+            long long ntasks = 0;
+            int nket = 0;
+            std::vector<long long> prefix;
+            std::vector<int> plo, phi;
+            long long *d_prefix = nullptr;
+            int *d_plo = nullptr, *d_phi = nullptr;
+            double quartets = 0, flops = 0;
+        };
+
+        enum class Pairing
+        {
+            Unique,     // p >= q when bra and ket are the same class, all p otherwise
+            All,        // every (p, q)
+            Diagonal    // p == q only
+        };
+
+        // Bra positions [plo, phi) per ket position q, with the Schwarz cut Q_p Q_q >= thresh on
+        // descending-sorted Q (thresh <= 0: no screening).
+        TaskTable buildTasks(Context &c, const PairClass &bra, const PairClass &ket, Pairing mode,
+                             bool same, double thresh, int width)
+        {
+            TaskTable t;
+            t.nket = ket.n_pairs;
+            t.plo.resize(t.nket);
+            t.phi.resize(t.nket);
+            t.prefix.resize(t.nket + 1);
+            std::vector<double> qb(bra.n_pairs), kab_prefix(bra.n_pairs + 1, 0.0);
+            for (int i = 0; i < bra.n_pairs; i++)
+            {
+                qb[i] = bra.Q.empty() ? 1.0 : bra.Q[bra.order[i]];
+                kab_prefix[i + 1] = kab_prefix[i] + bra.pcnt[bra.order[i]];
+            }
+            const int la = bra.la, lb = bra.lb, lc = ket.la, ld = ket.lb;
+            const double Hab = nHerm(la + lb), Hcd = nHerm(lc + ld);
+            const double Sab = nSph(la) * nSph(lb), Scd = nSph(lc) * nSph(ld);
+            const double perKK = quartetFlops(la, lb, lc, ld, 1, 1) - 2 * Sab * Hab * Scd - 12 * Sab * Scd;
+            long long acc = 0;
+            for (int q = 0; q < t.nket; q++)
+            {
+                int lo, hi;
+                if (mode == Pairing::Diagonal)
+                {
+                    lo = q;
+                    hi = q + 1;
+                }
+                else
+                {
+                    lo = (mode == Pairing::Unique && same) ? q : 0;
+                    hi = bra.n_pairs;
+                    if (thresh > 0)
+                    {
+                        const double qk = ket.Q.empty() ? 1.0 : ket.Q[ket.order[q]];
+                        // first p with qb[p] * qk < thresh (qb descending)
+                        int a = 0, b = bra.n_pairs;
+                        while (a < b)
+                        {
+                            int m = (a + b) / 2;
+                            if (qb[m] * qk >= thresh)
+                                a = m + 1;
+                            else
+                                b = m;
+                        }
+                        hi = a;
+                    }
+                    hi = std::max(hi, lo);
+                }
+                t.plo[q] = lo;
+                t.phi[q] = hi;
+                t.prefix[q] = acc;
+                const long long cnt = hi - lo;
+                acc += (cnt + width - 1) / width;
+                if (cnt > 0)
+                {
+                    const double kcd = ket.pcnt[ket.order[q]];
+                    const double skab = kab_prefix[hi] - kab_prefix[lo];
+                    t.quartets += cnt;
+                    t.flops += perKK * kcd * skab + 2 * Sab * Hab * Scd * skab + 12 * Sab * Scd * cnt;
+                }
+            }
+            t.prefix[t.nket] = acc;
+            t.ntasks = acc;
+            t.d_prefix = c.upload(t.prefix);
+            t.d_plo = c.upload(t.plo);
+            t.d_phi = c.upload(t.phi);
+            return t;
+        }
+
+        KArgs baseArgs(Context &c, const PairClass &bra, const PairClass &ket, const TaskTable &t,
+                       bool same)
+        {
+            KArgs a{};
+            a.bra = bra.view();
+            a.ket = ket.view();
+            a.prefix = t.d_prefix;
+            a.plo = t.d_plo;
+            a.phi = t.d_phi;
+            a.nket = t.nket;
+            a.ntasks = t.ntasks;
+            a.same_class = same ? 1 : 0;
+            a.boys = c.d_boys;
+            a.nbf = static_cast<long long>(c.nbf);
+            a.naux = static_cast<long long>(c.nbf_aux);
+            return a;
+        }
+
+        const KernelSlot &slotFor(int mode, const PairClass &bra, const PairClass &ket)
+        {
+            const KernelSlot &s = kernelSlot(mode, bra.la, bra.lb, ket.la, ket.lb);
+            if (!s.fn)
+                throw Error(LBX_ERR_INVALID, "no kernel for class (" + std::to_string(bra.la) +
+                                                     std::to_string(bra.lb) + "|" +
+                                                     std::to_string(ket.la) + std::to_string(ket.lb) +
+                                                     ") mode " + std::to_string(mode));
+            return s;
+        }
+
+        void run(Context &c, int mode, const PairClass &bra, const PairClass &ket, Pairing pairing,
+                 bool same, double thresh, KArgs extra)
+        {
+            if (bra.n_pairs == 0 || ket.n_pairs == 0)
+                return;
+            const KernelSlot &s = slotFor(mode, bra, ket);
+            TaskTable t = buildTasks(c, bra, ket, pairing, same, thresh, s.width);
+            KArgs a = baseArgs(c, bra, ket, t, same);
+            a.D = extra.D;
+            a.J = extra.J;
+            a.K = extra.K;
+            a.out = extra.out;
+            a.kind = extra.kind;
+            a.qout = extra.qout;
+            a.dout = extra.dout;
+            s.fn(a, c.stream);
+            LBX_HIP(hipGetLastError());
+            LBX_HIP(hipStreamSynchronize(c.stream));
+            // the task table's device arrays were allocated from the context; free them now
+            for (void *p : {static_cast<void *>(t.d_prefix), static_cast<void *>(t.d_plo),
+                            static_cast<void *>(t.d_phi)})
+            {
+                auto it = std::find(c.allocations.begin(), c.allocations.end(), p);
+                if (it != c.allocations.end())
+                {
+                    (void)hipFree(p);
+                    c.allocations.erase(it);
+                }
+            }
+        }
+    }
+
+    KernelSlot &kernelSlot(int mode, int la, int lb, int lc, int ld)
+    {
+        static KernelSlot tab[3][kMaxLAux + 1][kMaxLMain + 1][kMaxLAux + 1][kMaxLMain + 1];
+        if (mode < 0 || mode > 2 || la < 0 || la > kMaxLAux || lc < 0 || lc > kMaxLAux || lb < 0 ||
+            lb > kMaxLMain || ld < 0 || ld > kMaxLMain)
+            throw Error(LBX_ERR_INVALID, "kernel slot out of range");
+        return tab[mode][la][lb][lc][ld];
+    }
+
+    // Schwarz factors Q_ab = sqrt(max |(ab|ab)|) per pair of every main class; pairs sorted by
+    // descending Q so that each ket's surviving bra pairs form a prefix.
+    void ensureSchwarz(Context &c)
+    {
+        if (c.have_schwarz)
+            return;
+        for (auto &pc : c.cls)
+        {
+            if (pc.n_pairs == 0)
+                continue;
+            KArgs extra{};
+            extra.qout = pc.d_q;
+            extra.kind = 4;
+            run(c, MODE_DIAG, pc, pc, Pairing::Diagonal, true, 0.0, extra);
+            pc.Q.resize(pc.n_pairs);
+            LBX_HIP(hipMemcpy(pc.Q.data(), pc.d_q, pc.n_pairs * sizeof(double), hipMemcpyDeviceToHost));
+            std::vector<int> order(pc.n_pairs);
+            std::iota(order.begin(), order.end(), 0);
+            std::stable_sort(order.begin(), order.end(),
+                             [&](int x, int y) { return pc.Q[x] > pc.Q[y]; });
+            std::vector<double> qs(pc.n_pairs);
+            for (int i = 0; i < pc.n_pairs; i++)
+                qs[i] = pc.Q[order[i]];
+            pc.order = order;
+            LBX_HIP(hipMemcpy(pc.d_order, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice));
+            LBX_HIP(hipMemcpy(pc.d_q, qs.data(), qs.size() * sizeof(double), hipMemcpyHostToDevice));
+        }
+        c.have_schwarz = true;
+    }
+}
+
+using namespace lbx;
+
+struct lbx_context : Context
+{
+    // J/K task tables per (bra class, ket class, Schwarz threshold); Q does not depend on D
+    std::map<std::tuple<int, int, double>, TaskTable> jk_tasks;
+};
+
+extern "C"
+{
+    const char *lbx_last_error(void) { return g_err.c_str(); }
+
+    int lbx_create(lbx_context **out, const lbx_basis *main_basis, const lbx_basis *aux_basis)
+    {
+        if (!out || !main_basis)
+        {
+            g_err = "lbx_create: null argument";
+            return LBX_ERR_INVALID;
+        }
+        *out = nullptr;
+        auto ctx = std::make_unique<lbx_context>();
+        int rc = guarded([&]
+                         {
+                             Context &c = *ctx;
+                             LBX_HIP(hipGetDevice(&c.device));
+                             c.natoms = main_basis->natoms;
+                             c.shells = buildShells(*main_basis, c.nbf, c.max_l, kMaxLMain);
+                             if (c.shells.empty())
+                                 throw Error(LBX_ERR_INVALID, "lbx_create: empty main basis");
+                             if (aux_basis)
+                             {
+                                 if (aux_basis->natoms != c.natoms)
+                                     throw Error(LBX_ERR_INVALID, "lbx_create: aux basis natoms differs");
+                                 c.shells_aux = buildShells(*aux_basis, c.nbf_aux, c.max_l_aux, kMaxLAux);
+                             }
+                             c.d_boys = const_cast<double *>(boysTableDevice());
+                             buildPairClasses(c);
+                             for (auto &pc : c.cls)
+                                 computeEcoeffs(c, pc, c.shells);
+                             for (auto &pc : c.aux)
+                                 computeEcoeffs(c, pc, c.shells_aux);
+                             LBX_HIP(hipStreamSynchronize(c.stream));
+                         });
+        if (rc != LBX_OK)
+        {
+            ctx->release();
+            return rc;
+        }
+        *out = ctx.release();
+        return LBX_OK;
+    }
+
+    int lbx_destroy(lbx_context *ctx)
+    {
+        if (!ctx)
+            return LBX_OK;
+        int rc = guarded([&]
+                         {
+                             (void)hipStreamSynchronize(ctx->stream);
+                             ctx->release();
+                         });
+        delete ctx;
+        return rc;
+    }
+
+    int lbx_dims(const lbx_context *ctx, long *nbf, long *nbf_aux, long *natoms)
+    {
+        if (!ctx)
+        {
+            g_err = "lbx_dims: null context";
+            return LBX_ERR_INVALID;
+        }
+        if (nbf)
+            *nbf = static_cast<long>(ctx->nbf);
+        if (nbf_aux)
+            *nbf_aux = static_cast<long>(ctx->nbf_aux);
+        if (natoms)
+            *natoms = ctx->natoms;
+        return LBX_OK;
+    }
+
+    int lbx_set_stream(lbx_context *ctx, void *stream)
+    {
+        if (!ctx)
+        {
+            g_err = "lbx_set_stream: null context";
+            return LBX_ERR_INVALID;
+        }
+        ctx->stream = static_cast<hipStream_t>(stream);
+        return LBX_OK;
+    }
+
+    int lbx_synchronize(lbx_context *ctx)
+    {
+        if (!ctx)
+        {
+            g_err = "lbx_synchronize: null context";
+            return LBX_ERR_INVALID;
+        }
+        return guarded([&] { LBX_HIP(hipStreamSynchronize(ctx->stream)); });
+    }
+
+    int lbx_boys(int n, long count, const double *x, double *out, void *stream)
+    {
+        return guarded([&]
+                       {
+                           if (n < 0 || n > kMaxLBoys || count < 0 || (count > 0 && (!x || !out)))
+                               throw Error(LBX_ERR_INVALID, "lbx_boys: bad argument");
+                           if (count == 0)
+                               return;
+                           static const auto fns = boysFns(std::make_integer_sequence<int, kMaxLBoys + 1>{});
+                           fns[n](count, x, out, boysTableDevice(), static_cast<hipStream_t>(stream));
+                           LBX_HIP(hipGetLastError());
+                       });
+    }
+
+    int lbx_eri4(lbx_context *ctx, double *out)
+    {
+        return guarded([&]
+                       {
+                           if (!ctx || !out)
+                               throw Error(LBX_ERR_INVALID, "lbx_eri4: null argument");
+                           std::lock_guard<std::mutex> lk(ctx->mu);
+                           Context &c = *ctx;
+                           const long long n = static_cast<long long>(c.nbf);
+                           fill(out, n * n * n * n, 0.0, c.stream);
+                           KArgs extra{};
+                           extra.out = out;
+                           extra.kind = 4;
+                           for (size_t i = 0; i < c.cls.size(); i++)
+                               for (size_t j = 0; j <= i; j++)
+                                   run(c, MODE_DENSE, c.cls[i], c.cls[j], Pairing::Unique, i == j, 0.0, extra);
+                       });
+    }
+
+    int lbx_eri4_diag(lbx_context *ctx, double *out)
+    {
+        return guarded([&]
+                       {
+                           if (!ctx || !out)
+                               throw Error(LBX_ERR_INVALID, "lbx_eri4_diag: null argument");
+                           std::lock_guard<std::mutex> lk(ctx->mu);
+                           Context &c = *ctx;
+                           const long long n = static_cast<long long>(c.nbf);
+                           fill(out, n * n, 0.0, c.stream);
+                           for (auto &pc : c.cls)
+                           {
+                               if (pc.n_pairs == 0)
+                                   continue;
+                               DevBuf q(pc.n_pairs);
+                               KArgs extra{};
+                               extra.qout = q.p;
+                               extra.dout = out;
+                               extra.kind = 4;
+                               run(c, MODE_DIAG, pc, pc, Pairing::Diagonal, true, 0.0, extra);
+                           }
+                       });
+    }
+
+    int lbx_eri3(lbx_context *ctx, double *out)
+    {
+        return guarded([&]
+                       {
+                           if (!ctx || !out)
+                               throw Error(LBX_ERR_INVALID, "lbx_eri3: null argument");
+                           std::lock_guard<std::mutex> lk(ctx->mu);
+                           Context &c = *ctx;
+                           if (c.shells_aux.empty())
+                               throw Error(LBX_ERR_STATE, "eri3(): RI approximation is not enabled");
+                           const long long n = static_cast<long long>(c.nbf);
+                           fill(out, n * n * static_cast<long long>(c.nbf_aux), 0.0, c.stream);
+                           KArgs extra{};
+                           extra.out = out;
+                           extra.kind = 3;
+                           for (auto &bra : c.cls)
+                               for (auto &ket : c.aux)
+                                   run(c, MODE_DENSE, bra, ket, Pairing::All, false, 0.0, extra);
+                       });
+    }
+
+    int lbx_eri2(lbx_context *ctx, double *out)
+    {
+        return guarded([&]
+                       {
+                           if (!ctx || !out)
+                               throw Error(LBX_ERR_INVALID, "lbx_eri2: null argument");
+                           std::lock_guard<std::mutex> lk(ctx->mu);
+                           Context &c = *ctx;
+                           if (c.shells_aux.empty())
+                               throw Error(LBX_ERR_STATE, "eri2(): RI approximation is not enabled");
+                           const long long n = static_cast<long long>(c.nbf_aux);
+                           fill(out, n * n, 0.0, c.stream);
+                           KArgs extra{};
+                           extra.out = out;
+                           extra.kind = 2;
+                           for (size_t i = 0; i < c.aux.size(); i++)
+                               for (size_t j = 0; j <= i; j++)
+                                   run(c, MODE_DENSE, c.aux[i], c.aux[j], Pairing::Unique, i == j, 0.0, extra);
+                       });
+    }
+
+    int lbx_eri2_diag(lbx_context *ctx, double *out)
+    {
+        return guarded([&]
+                       {
+                           if (!ctx || !out)
+                               throw Error(LBX_ERR_INVALID, "lbx_eri2_diag: null argument");
+                           std::lock_guard<std::mutex> lk(ctx->mu);
+                           Context &c = *ctx;
+                           if (c.shells_aux.empty())
+                               throw Error(LBX_ERR_STATE, "eri2Diagonal(): RI approximation is not enabled");
+                           fill(out, static_cast<long long>(c.nbf_aux), 0.0, c.stream);
+                           for (auto &pc : c.aux)
+                           {
+                               if (pc.n_pairs == 0)
+                                   continue;
+                               DevBuf q(pc.n_pairs);
+                               KArgs extra{};
+                               extra.qout = q.p;
+                               extra.dout = out;
+                               extra.kind = 2;
+                               run(c, MODE_DIAG, pc, pc, Pairing::Diagonal, true, 0.0, extra);
+                           }
+                       });
+    }
+
+    int lbx_jk(lbx_context *ctx, const double *D, double *J, double *K, double schwarz_thresh,
+               int precision, double mixed_thresh, lbx_jk_stats *stats)
+    {
+        (void)mixed_thresh;
+        return guarded([&]
+                       {
+                           if (!ctx || !D || !J || !K)
+                               throw Error(LBX_ERR_INVALID, "lbx_jk: null argument");
+                           if (precision != LBX_FP64)
+                               throw Error(LBX_ERR_INVALID, "lbx_jk: only LBX_FP64 is available");
+                           if (!(schwarz_thresh >= 0))
+                               throw Error(LBX_ERR_INVALID, "lbx_jk: negative Schwarz threshold");
+                           std::lock_guard<std::mutex> lk(ctx->mu);
+                           Context &c = *ctx;
+                           ensureSchwarz(c);
+                           const long long n = static_cast<long long>(c.nbf);
+                           fill(J, n * n, 0.0, c.stream);
+                           fill(K, n * n, 0.0, c.stream);
+                           KArgs extra{};
+                           extra.D = D;
+                           extra.J = J;
+                           extra.K = K;
+                           double nq = 0, fl = 0;
+                           for (size_t i = 0; i < c.cls.size(); i++)
+                               for (size_t j = 0; j <= i; j++)
+                               {
+                                   const PairClass &bra = c.cls[i], &ket = c.cls[j];
+                                   if (bra.n_pairs == 0 || ket.n_pairs == 0)
+                                       continue;
+                                   const KernelSlot &s = slotFor(MODE_JK, bra, ket);
+                                   const auto key = std::make_tuple(static_cast<int>(i), static_cast<int>(j), schwarz_thresh);
+                                   auto it = ctx->jk_tasks.find(key);
+                                   if (it == ctx->jk_tasks.end())
+                                       it = ctx->jk_tasks
+                                                    .emplace(key, buildTasks(c, bra, ket, Pairing::Unique, i == j,
+                                                                             schwarz_thresh, s.width))
+                                                    .first;
+                                   const TaskTable &t = it->second;
+                                   nq += t.quartets;
+                                   fl += t.flops;
+                                   KArgs a = baseArgs(c, bra, ket, t, i == j);
+                                   a.D = D;
+                                   a.J = J;
+                                   a.K = K;
+                                   s.fn(a, c.stream);
+                                   LBX_HIP(hipGetLastError());
+                               }
+                           const int gridn = static_cast<int>(std::min<long long>((n * n + 255) / 256, 65536));
+                           hipLaunchKernelGGL(symmetrizeKernel, dim3(gridn), dim3(256), 0, c.stream, J, static_cast<int>(n));
+                           hipLaunchKernelGGL(symmetrizeKernel, dim3(gridn), dim3(256), 0, c.stream, K, static_cast<int>(n));
+                           LBX_HIP(hipGetLastError());
+                           LBX_HIP(hipStreamSynchronize(c.stream));
+                           if (stats)
+                           {
+                               stats->quartets = nq;
+                               stats->flops = fl;
+                               stats->quartets_fp32 = 0;
+                           }
+                       });
+    }
+}

@@ -1,0 +1,192 @@
+// Device-side building blocks shared by the quartet kernels: Boys function, Hermite R
+// recursion, recursion tables, wave helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "lbx_common.hpp"
+
+namespace lbx
+{
+    typedef double v4d __attribute__((ext_vector_type(4)));
+
+    constexpr double kTwoPi52 = 34.98683665524972;   // 2 pi^{5/2}
+
+    // ---- Boys function F_0..F_L(x) (boys_function.hpp:68-109): x = 0 exact values; x > 30
+    // asymptotic form with upward recursion; otherwise 7-term Taylor expansion of F_L about the
+    // tabulated point below x and downward recursion F_k = (2x F_{k+1} + e^-x)/(2k+1).
+    template <int L>
+    __device__ __forceinline__ void boysF(double x, const double *__restrict__ tab, double *F)
+    {
+        if (x == 0.0)
+        {
+            F[0] = 1.0;
+#pragma unroll
+            for (int k = 1; k <= L; k++)
+                F[k] = 1.0 / (2 * k + 1);
+        }
+        else if (x > kBoysLargeX)
+        {
+            const double rx = 1.0 / x;
+            F[0] = 0.5 * sqrt(M_PI * rx);
+#pragma unroll
+            for (int k = 1; k <= L; k++)
+                F[k] = F[k - 1] * (k - 0.5) * rx;
+        }
+        else
+        {
+            const int iv = static_cast<int>(x * 100.0);
+            const double dx = x - iv * kBoysStep;
+            const double *row = tab + iv * kBoysCols + L;
+            // sum_k F_{L+k}(x0) (-dx)^k / k!
+            const double m = -dx;
+            double s = row[6] * (1.0 / 720.0);
+            s = fma(s, m, row[5] * (1.0 / 120.0));
+            s = fma(s, m, row[4] * (1.0 / 24.0));
+            s = fma(s, m, row[3] * (1.0 / 6.0));
+            s = fma(s, m, row[2] * 0.5);
+            s = fma(s, m, row[1]);
+            s = fma(s, m, row[0]);
+            F[L] = s;
+            const double ex = exp(-x);
+            const double x2 = 2.0 * x;
+#pragma unroll
+            for (int k = L - 1; k >= 0; k--)
+                F[k] = fma(x2, F[k + 1], ex) * (1.0 / (2 * k + 1));
+        }
+    }
+
+    // ---- Hermite R integrals in registers (HJO eqs. 9.9.18-9.9.20).  R holds nHerm(L)+L
+    // slots: R[n] = R^{(n)}_{000} for n = 0..L on entry; on exit R_{tuv} (n = 0) sits at
+    // slot 0 for (0,0,0) and at slot L + hermIdx(t,u,v) otherwise.  Levels n = L-1..0 update the
+    // |tuv| = m slots in place in descending m, so reads of |tuv| = m-1, m-2 still see level n+1.
+    template <int L>
+    constexpr LBX_HD int rSlot(int t, int u, int v, int n)
+    {
+        return (t + u + v == 0) ? n : L + hermIdx(t, u, v);
+    }
+
+    template <int L>
+    __device__ __forceinline__ void hermiteR(double *R, double X, double Y, double Z)
+    {
+#pragma unroll
+        for (int n = L - 1; n >= 0; n--)
+        {
+#pragma unroll
+            for (int m = L - n; m >= 1; m--)
+            {
+#pragma unroll
+                for (int t = m; t >= 0; t--)
+                {
+#pragma unroll
+                    for (int u = m - t; u >= 0; u--)
+                    {
+                        const int v = m - t - u;
+                        const int o = L + hermIdx(t, u, v);
+                        if (t > 0)
+                        {
+                            double r = X * R[rSlot<L>(t - 1, u, v, n + 1)];
+                            if (t > 1)
+                                r = fma(double(t - 1), R[rSlot<L>(t - 2, u, v, n + 1)], r);
+                            R[o] = r;
+                        }
+                        else if (u > 0)
+                        {
+                            double r = Y * R[rSlot<L>(t, u - 1, v, n + 1)];
+                            if (u > 1)
+                                r = fma(double(u - 1), R[rSlot<L>(t, u - 2, v, n + 1)], r);
+                            R[o] = r;
+                        }
+                        else
+                        {
+                            double r = Z * R[rSlot<L>(t, u, v - 1, n + 1)];
+                            if (v > 1)
+                                r = fma(double(v - 1), R[rSlot<L>(t, u, v - 2, n + 1)], r);
+                            R[o] = r;
+                        }
+                    }
+                }
+            }
+        }
+    }
+
+    // slot in the hermiteR buffer of the Hermite index sum (tuv of bra index i) + (tuv of ket j)
+    template <int L>
+    constexpr LBX_HD int rSum(int i, int j)
+    {
+        const int a = hermTuv(i), b = hermTuv(j);
+        const int t = (a & 255) + (b & 255), u = ((a >> 8) & 255) + ((b >> 8) & 255),
+                  v = (a >> 16) + (b >> 16);
+        return rSlot<L>(t, u, v, 0);
+    }
+
+    constexpr LBX_HD int hermLevel(int i)
+    {
+        const int a = hermTuv(i);
+        return (a & 255) + ((a >> 8) & 255) + (a >> 16);
+    }
+
+    // ---- Cooperative (LDS) recursion table: for Hermite index h >= 1, the source indices,
+    // axis and multiplicity of R^{(n)}_h = X_axis R^{(n+1)}_{h-e} + (c-1) R^{(n+1)}_{h-2e}.
+    struct RecTab
+    {
+        unsigned int v[nHerm(kMaxLBoys)];
+        constexpr RecTab() : v()
+        {
+            for (int h = 1; h < nHerm(kMaxLBoys); h++)
+            {
+                const int a = hermTuv(h);
+                const int t = a & 255, u = (a >> 8) & 255, w = a >> 16;
+                int ax, c;
+                if (t > 0)
+                {
+                    ax = 0;
+                    c = t;
+                }
+                else if (u > 0)
+                {
+                    ax = 1;
+                    c = u;
+                }
+                else
+                {
+                    ax = 2;
+                    c = w;
+                }
+                const int dt = ax == 0, du = ax == 1, dw = ax == 2;
+                const int s1 = hermIdx(t - dt, u - du, w - dw);
+                const int s2 = c > 1 ? hermIdx(t - 2 * dt, u - 2 * du, w - 2 * dw) : 0;
+                v[h] = static_cast<unsigned>(s1) | (static_cast<unsigned>(s2) << 11) |
+                       (static_cast<unsigned>(ax) << 22) | (static_cast<unsigned>(c - 1) << 24);
+            }
+        }
+    };
+
+    static __constant__ RecTab kRecTab = RecTab();
+
+    __device__ __forceinline__ double waveSum(double x)
+    {
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1)
+            x += __shfl_xor(x, o, 64);
+        return x;
+    }
+
+    __device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
+    // task t -> ket position q with prefix[q] <= t < prefix[q+1]
+    __device__ __forceinline__ int findKet(const long long *__restrict__ prefix, int nket,
+                                           long long t)
+    {
+        int lo = 0, hi = nket - 1;
+        while (lo < hi)
+        {
+            int mid = (lo + hi + 1) >> 1;
+            if (prefix[mid] <= t)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        return lo;
+    }
+}

@@ -1,0 +1,256 @@
+// SHARK E matrices on the device: for each primitive pair of a pair class and each spherical
+// AO pair (mu, nu), the Hermite expansion
+//   E[tuv][mu nu] = N_a[mu] N_b[nu] d_a d_b sum_{cart a, cart b} c_a c_b E^x_t E^y_u E^z_v
+// of the product of two normalised spherical Gaussians (the quantity of ecoeffsSHARK,
+// ecoeffs.cpp:314-403; single-shell form ecoeffs.cpp:253-312).  One thread per
+// (primitive pair, mu nu).  Written twice: Eb unsigned (bra use) and Ek with (-1)^{t+u+v}
+// folded in (ket use).  Both in [ppair][tuv][mu nu] order.
+#include "device_common.hpp"
+#include "lbx_internal.hpp"
+
+#include <algorithm>
+
+namespace lbx
+{
+    namespace
+    {
+        constexpr int kSphMax = 256;
+        struct SphConst
+        {
+            int start[kMaxLAux + 1][2 * kMaxLAux + 2];   // entry range of each (l, m slot)
+            int cart[kSphMax];
+            double val[kSphMax];
+        };
+        __constant__ SphConst cSph;
+
+        struct PrimIn
+        {
+            double a, b, dadb;
+            int pair;
+        };
+
+        __device__ __forceinline__ void cartOf(int l, int idx, int &i, int &j, int &k)
+        {
+            // inverse of cartIdx within shell l: idx = s(s+1)/2 + k with s = j + k
+            int s = 0;
+            while ((s + 1) * (s + 2) / 2 <= idx)
+                s++;
+            k = idx - s * (s + 1) / 2;
+            j = s - k;
+            i = l - s;
+        }
+
+        // 1-D Hermite coefficients E^{ij}_t, i <= LA, j <= LB (HJO eqs. 9.5.6-9.5.7).
+        template <int LA, int LB>
+        __device__ __forceinline__ void hermite1d(double (&E)[LA + 1][LB + 1][LA + LB + 1], double K,
+                                                  double PA, double PB, double o2p)
+        {
+            for (int i = 0; i <= LA; i++)
+                for (int j = 0; j <= LB; j++)
+                    for (int t = 0; t <= LA + LB; t++)
+                        E[i][j][t] = 0.0;
+            E[0][0][0] = K;
+            for (int i = 1; i <= LA; i++)
+                for (int t = 0; t <= i; t++)
+                {
+                    double v = PA * E[i - 1][0][t];
+                    if (t > 0)
+                        v += o2p * E[i - 1][0][t - 1];
+                    if (t + 1 <= i - 1)
+                        v += (t + 1) * E[i - 1][0][t + 1];
+                    E[i][0][t] = v;
+                }
+            for (int j = 1; j <= LB; j++)
+                for (int i = 0; i <= LA; i++)
+                    for (int t = 0; t <= i + j; t++)
+                    {
+                        double v = PB * E[i][j - 1][t];
+                        if (t > 0)
+                            v += o2p * E[i][j - 1][t - 1];
+                        if (t + 1 <= i + j - 1)
+                            v += (t + 1) * E[i][j - 1][t + 1];
+                        E[i][j][t] = v;
+                    }
+        }
+
+        template <int LA, int LB>
+        __global__ __launch_bounds__(256) void ecoefKernel(int nppairs, const PrimIn *__restrict__ prim,
+                                                           const double *__restrict__ centers,
+                                                           const double *__restrict__ norms,
+                                                           double *__restrict__ eb,
+                                                           double *__restrict__ ek)
+        {
+            constexpr int SA = nSph(LA), SB = nSph(LB), SP = SA * SB, HP = nHerm(LA + LB);
+            const long long gid = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+            if (gid >= static_cast<long long>(nppairs) * SP)
+                return;
+            const int pp = static_cast<int>(gid / SP), mn = static_cast<int>(gid % SP);
+            const int mu = mn / SB, nu = mn % SB;
+            const PrimIn pr = prim[pp];
+            const double a = pr.a, b = pr.b, p = a + b, o2p = 1.0 / (2.0 * p), mur = a * b / p;
+            const double *C = centers + 6 * pr.pair;
+            double Ex[LA + 1][LB + 1][LA + LB + 1], Ey[LA + 1][LB + 1][LA + LB + 1],
+                    Ez[LA + 1][LB + 1][LA + LB + 1];
+            {
+                double P[3], K[3];
+                for (int x = 0; x < 3; x++)
+                {
+                    P[x] = (a * C[x] + b * C[3 + x]) / p;
+                    const double d = C[x] - C[3 + x];
+                    K[x] = exp(-mur * d * d);
+                }
+                hermite1d<LA, LB>(Ex, K[0], P[0] - C[0], P[0] - C[3], o2p);
+                hermite1d<LA, LB>(Ey, K[1], P[1] - C[1], P[1] - C[4], o2p);
+                hermite1d<LA, LB>(Ez, K[2], P[2] - C[2], P[2] - C[5], o2p);
+            }
+            double acc[HP];
+            for (int h = 0; h < HP; h++)
+                acc[h] = 0.0;
+            for (int ea = cSph.start[LA][mu]; ea < cSph.start[LA][mu + 1]; ea++)
+            {
+                int ix, iy, iz;
+                cartOf(LA, cSph.cart[ea], ix, iy, iz);
+                for (int eb_ = cSph.start[LB][nu]; eb_ < cSph.start[LB][nu + 1]; eb_++)
+                {
+                    int jx, jy, jz;
+                    cartOf(LB, cSph.cart[eb_], jx, jy, jz);
+                    const double v = cSph.val[ea] * cSph.val[eb_];
+                    for (int t = 0; t <= ix + jx; t++)
+                        for (int u = 0; u <= iy + jy; u++)
+                            for (int w = 0; w <= iz + jz; w++)
+                                acc[hermIdx(t, u, w)] += v * Ex[ix][jx][t] * Ey[iy][jy][u] * Ez[iz][jz][w];
+                }
+            }
+            const double *nrm = norms + static_cast<size_t>(pr.pair) * (SA + SB);
+            const double sc = nrm[mu] * nrm[SA + nu] * pr.dadb;
+            for (int h = 0; h < HP; h++)
+            {
+                const double val = sc * acc[h];
+                const size_t o = (static_cast<size_t>(pp) * HP + h) * SP + mn;
+                eb[o] = val;
+                ek[o] = (hermLevel(h) & 1) ? -val : val;
+            }
+        }
+
+        using EcoefFn = void (*)(int, const PrimIn *, const double *, const double *, double *,
+                                 double *, hipStream_t);
+
+        template <int LA, int LB>
+        void launchEcoef(int n, const PrimIn *prim, const double *centers, const double *norms,
+                         double *eb, double *ek, hipStream_t s)
+        {
+            constexpr int SP = nSph(LA) * nSph(LB);
+            const long long total = static_cast<long long>(n) * SP;
+            if (total == 0)
+                return;
+            const int grid = static_cast<int>((total + 255) / 256);
+            hipLaunchKernelGGL((ecoefKernel<LA, LB>), dim3(grid), dim3(256), 0, s, n, prim, centers,
+                               norms, eb, ek);
+        }
+
+        EcoefFn ecoefFn(int la, int lb)
+        {
+#define LBX_E(A, B)                                                                                \
+    if (la == A && lb == B)                                                                        \
+        return &launchEcoef<A, B>;
+            LBX_E(0, 0) LBX_E(1, 0) LBX_E(1, 1) LBX_E(2, 0) LBX_E(2, 1) LBX_E(2, 2) LBX_E(3, 0)
+            LBX_E(3, 1) LBX_E(3, 2) LBX_E(3, 3) LBX_E(4, 0) LBX_E(5, 0)
+#undef LBX_E
+            throw Error(LBX_ERR_INVALID, "no E-matrix kernel for this angular momentum pair");
+        }
+
+        void uploadSphTables()
+        {
+            static bool done = false;   // per process; the table is identical on every device
+            static int done_dev = -1;
+            int dev = 0;
+            LBX_HIP(hipGetDevice(&dev));
+            if (done && done_dev == dev)
+                return;
+            SphConst h{};
+            int e = 0;
+            for (int l = 0; l <= kMaxLAux; l++)
+            {
+                const auto &T = sphTable(l);
+                int k = 0;
+                for (int m = 0; m <= 2 * l + 1; m++)
+                {
+                    h.start[l][m] = e;
+                    while (m < 2 * l + 1 && k < static_cast<int>(T.size()) && T[k].sph == m)
+                    {
+                        if (e >= kSphMax)
+                            throw Error(LBX_ERR_INVALID, "spherical table overflow");
+                        h.cart[e] = T[k].cart;
+                        h.val[e] = T[k].val;
+                        e++;
+                        k++;
+                    }
+                }
+            }
+            LBX_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cSph), &h, sizeof h));
+            done = true;
+            done_dev = dev;
+        }
+    }
+
+    // Uploads the pair class to the device and computes its E matrices.
+    void computeEcoeffs(Context &c, PairClass &pc, const std::vector<HShell> &shells)
+    {
+        uploadSphTables();
+        const int SA = nSph(pc.la), SB = nSph(pc.lb), SP = SA * SB, HP = nHerm(pc.la + pc.lb);
+        std::vector<PrimIn> prim(pc.n_ppairs);
+        std::vector<double> centers(6 * static_cast<size_t>(pc.n_pairs));
+        std::vector<double> norms(static_cast<size_t>(pc.n_pairs) * (SA + SB));
+        std::vector<double4> pp(pc.n_ppairs);
+        for (int ip = 0; ip < pc.n_pairs; ip++)
+        {
+            const HShell &A = shells[pc.sh_a[ip]];
+            const HShell *B = pc.single ? nullptr : &shells[pc.sh_b[ip]];
+            for (int x = 0; x < 3; x++)
+            {
+                centers[6 * ip + x] = A.xyz[x];
+                centers[6 * ip + 3 + x] = B ? B->xyz[x] : A.xyz[x];
+            }
+            for (int m = 0; m < SA; m++)
+                norms[static_cast<size_t>(ip) * (SA + SB) + m] = A.norms[m];
+            for (int m = 0; m < SB; m++)
+                norms[static_cast<size_t>(ip) * (SA + SB) + SA + m] = B ? B->norms[m] : 1.0;
+            for (int o = pc.poff[ip]; o < pc.poff[ip] + pc.pcnt[ip]; o++)
+            {
+                const int x = pc.prim_x[o], y = pc.prim_y[o];
+                const double a = A.exps[x], b = B ? B->exps[y] : 0.0;
+                const double da = A.pnorms[x] * A.coeffs[x];
+                const double db = B ? B->pnorms[y] * B->coeffs[y] : 1.0;
+                prim[o] = PrimIn{a, b, da * db, ip};
+                const double p = a + b;
+                double P[3];
+                for (int k = 0; k < 3; k++)
+                    P[k] = (a * A.xyz[k] + b * (B ? B->xyz[k] : A.xyz[k])) / p;
+                pp[o] = double4{p, P[0], P[1], P[2]};
+            }
+        }
+        pc.d_pp = c.upload(pp);
+        pc.d_ao_a = c.upload(pc.ao_a);
+        pc.d_ao_b = c.upload(pc.ao_b);
+        pc.d_sh_a = c.upload(pc.sh_a);
+        pc.d_sh_b = c.upload(pc.sh_b);
+        pc.d_poff = c.upload(pc.poff);
+        pc.d_pcnt = c.upload(pc.pcnt);
+        std::vector<int> ident(pc.n_pairs);
+        for (int i = 0; i < pc.n_pairs; i++)
+            ident[i] = i;
+        pc.order = ident;
+        pc.d_order = c.upload(ident);
+        pc.d_q = c.alloc<double>(pc.n_pairs);
+        const size_t ne = static_cast<size_t>(pc.n_ppairs) * HP * SP;
+        pc.d_eb = c.alloc<double>(ne);
+        pc.d_ek = c.alloc<double>(ne);
+        if (pc.n_ppairs == 0)
+            return;
+        PrimIn *d_prim = c.upload(prim);
+        double *d_centers = c.upload(centers);
+        double *d_norms = c.upload(norms);
+        ecoefFn(pc.la, pc.lb)(pc.n_ppairs, d_prim, d_centers, d_norms, pc.d_eb, pc.d_ek, c.stream);
+        LBX_HIP(hipGetLastError());
+    }
+}

@@ -1,0 +1,140 @@
+// Host-side data model of the lbx library: shells, pair classes and the device arrays behind
+// an lbx_context.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/lbx.h"
+#include "lbx_common.hpp"
+
+namespace lbx
+{
+    struct Error : std::runtime_error
+    {
+        int code;
+        Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+    };
+
+#define LBX_HIP(call)                                                                              \
+    do                                                                                             \
+    {                                                                                              \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            throw ::lbx::Error(e_ == hipErrorOutOfMemory ? LBX_ERR_NOMEM : LBX_ERR_HIP,             \
+                               std::string(#call) + ": " + hipGetErrorString(e_));                 \
+    } while (0)
+
+    // One normalised contracted shell (the reference's Shell, shell.hpp:36-67).
+    struct HShell
+    {
+        int l = 0, atom = 0;
+        size_t ao = 0, idx = 0;
+        std::array<double, 3> xyz{};
+        std::vector<double> exps, coeffs;   // raw coefficients
+        std::vector<double> pnorms;         // primitive norms
+        std::vector<double> norms;          // contracted spherical norms (2l+1)
+    };
+
+    // Device view of one pair class, passed by value to kernels.
+    struct DPairs
+    {
+        const int *ao_a, *ao_b;     // AO offsets of the two shells, per pair
+        const int *sh_a, *sh_b;     // shell indices, per pair (a == b detects diagonal pairs)
+        const int *poff, *pcnt;     // first primitive pair and count, per pair
+        const double4 *pp;          // per primitive pair: (p, Px, Py, Pz)
+        const double *eb;           // E^T [ppair][tuv][mu nu]           (bra use)
+        const double *ek;           // (-1)^{t+u+v} E^T [ppair][tuv][mu nu] (ket use)
+        const int *order;           // sorted position -> pair index
+        const double *q;            // Schwarz factor per sorted position
+        int n;
+    };
+
+    // Pairs of shells (la >= lb), or single shells (lb = 0, single = true: an auxiliary shell
+    // paired with a unit s function at the same centre, so 3- and 2-centre integrals run
+    // through the 4-centre kernels).
+    struct PairClass
+    {
+        int la = 0, lb = 0;
+        bool single = false;
+        int n_pairs = 0, n_ppairs = 0, kmax = 0;
+        // host
+        std::vector<int> ao_a, ao_b, sh_a, sh_b, atom_a, atom_b, poff, pcnt;
+        std::vector<int> prim_x, prim_y;   // surviving primitive pair -> primitive of a, of b
+        std::vector<double> Q;          // Schwarz factor per pair (pair order)
+        std::vector<int> order;         // sorted by Q descending
+        // device
+        int *d_ao_a = nullptr, *d_ao_b = nullptr, *d_sh_a = nullptr, *d_sh_b = nullptr;
+        int *d_poff = nullptr, *d_pcnt = nullptr, *d_order = nullptr;
+        double4 *d_pp = nullptr;
+        double *d_eb = nullptr, *d_ek = nullptr, *d_q = nullptr;
+
+        DPairs view() const
+        {
+            return DPairs{d_ao_a, d_ao_b, d_sh_a, d_sh_b, d_poff, d_pcnt, d_pp, d_eb, d_ek,
+                          d_order, d_q, n_pairs};
+        }
+    };
+
+    struct Context
+    {
+        int device = 0;
+        hipStream_t stream = nullptr;
+        std::mutex mu;
+        int natoms = 0;
+        size_t nbf = 0, nbf_aux = 0;
+        int max_l = 0, max_l_aux = -1;
+        std::vector<HShell> shells, shells_aux;
+        std::vector<PairClass> cls;      // index pairClassIdx(la, lb)
+        std::vector<PairClass> aux;      // index l
+        double *d_boys = nullptr;        // Boys table kBoysRows x kBoysCols
+        bool have_schwarz = false;
+        std::vector<void *> allocations;
+
+        template <class T>
+        T *alloc(size_t n)
+        {
+            void *p = nullptr;
+            if (n == 0)
+                n = 1;
+            LBX_HIP(hipMalloc(&p, n * sizeof(T)));
+            allocations.push_back(p);
+            return static_cast<T *>(p);
+        }
+
+        template <class T>
+        T *upload(const std::vector<T> &v)
+        {
+            T *d = alloc<T>(v.size());
+            if (!v.empty())
+                LBX_HIP(hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+            return d;
+        }
+
+        void release();
+    };
+
+    // host_setup.cpp
+    std::vector<HShell> buildShells(const lbx_basis &b, size_t &nbf, int &max_l, int max_l_allowed);
+    std::vector<double> boysTable();
+    struct SphEntry
+    {
+        int sph, cart;
+        double val;
+    };
+    const std::vector<SphEntry> &sphTable(int l);
+    void buildPairClasses(Context &c);
+    void uploadPairClass(Context &c, PairClass &pc, const std::vector<HShell> &shells,
+                         bool single);
+
+    // ecoef.hip
+    void computeEcoeffs(Context &c, PairClass &pc, const std::vector<HShell> &shells);
+
+    // schwarz / task tables (capi.cpp)
+    void ensureSchwarz(Context &c);
+}

@@ -1,0 +1,168 @@
+"""ctypes binding of liblbx.so, the MI355X SHARK ERI library (C-ABI in include/lbx.h).
+
+Device memory comes from torch (CUDA tensors on the current HIP device); the integrals are
+computed by the library's HIP kernels.  There is no CPU fallback: without the built library or
+without a GPU every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblbx.so")
+
+LBX_FP64 = 0
+LBX_MIXED = 1
+
+# every symbol include/lbx.h declares
+SYMBOLS = ["lbx_create", "lbx_destroy", "lbx_last_error", "lbx_dims", "lbx_set_stream",
+           "lbx_synchronize", "lbx_boys", "lbx_eri4", "lbx_eri4_diag", "lbx_eri3", "lbx_eri2",
+           "lbx_eri2_diag", "lbx_jk"]
+
+
+class LbxError(RuntimeError):
+    pass
+
+
+class _Basis(ctypes.Structure):
+    _fields_ = [("natoms", ctypes.c_int), ("coords", ctypes.c_void_p), ("nshells", ctypes.c_int),
+                ("shell_atom", ctypes.c_void_p), ("shell_l", ctypes.c_void_p),
+                ("shell_nprim", ctypes.c_void_p), ("exps", ctypes.c_void_p),
+                ("coeffs", ctypes.c_void_p)]
+
+
+class JKStats(ctypes.Structure):
+    _fields_ = [("quartets", ctypes.c_double), ("flops", ctypes.c_double),
+                ("quartets_fp32", ctypes.c_double)]
+
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise LbxError(f"{LIB_PATH} is not built (run `python -m lible_amd.build`)")
+        L = ctypes.CDLL(LIB_PATH)
+        p, i, d, l = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_long
+        L.lbx_last_error.restype = ctypes.c_char_p
+        L.lbx_create.argtypes = [ctypes.POINTER(p), ctypes.POINTER(_Basis), ctypes.POINTER(_Basis)]
+        L.lbx_destroy.argtypes = [p]
+        L.lbx_dims.argtypes = [p, ctypes.POINTER(l), ctypes.POINTER(l), ctypes.POINTER(l)]
+        L.lbx_set_stream.argtypes = [p, p]
+        L.lbx_synchronize.argtypes = [p]
+        L.lbx_boys.argtypes = [i, l, p, p, p]
+        for name in ("lbx_eri4", "lbx_eri4_diag", "lbx_eri3", "lbx_eri2", "lbx_eri2_diag"):
+            getattr(L, name).argtypes = [p, p]
+        L.lbx_jk.argtypes = [p, p, p, p, d, i, d, ctypes.POINTER(JKStats)]
+        _LIB = L
+    return _LIB
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise LbxError(f"lbx error {rc}: {lib().lbx_last_error().decode()}")
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise LbxError("no GPU available: the lbx library runs on the MI355X only")
+    return torch
+
+
+def boys(n: int, x):
+    """F_0..F_n at the points of the CUDA tensor x (float64) -> tensor (len(x), n+1)."""
+    torch = _torch()
+    x = x.contiguous().to(torch.float64)
+    out = torch.empty((x.numel(), n + 1), dtype=torch.float64, device=x.device)
+    _check(lib().lbx_boys(n, x.numel(), x.data_ptr(), out.data_ptr(),
+                          torch.cuda.current_stream().cuda_stream))
+    return out
+
+
+class Context:
+    """A molecule (lible_amd.basis.Molecule) resident on the current device."""
+
+    def __init__(self, mol):
+        torch = _torch()
+        self._keep = []
+        main = self._basis(mol.coords, mol.shells, mol.natoms)
+        aux = self._basis(mol.coords, mol.aux, mol.natoms) if mol.aux is not None else None
+        h = ctypes.c_void_p()
+        _check(lib().lbx_create(ctypes.byref(h), ctypes.byref(main),
+                                ctypes.byref(aux) if aux is not None else None))
+        self.h = h
+        _check(lib().lbx_set_stream(self.h, torch.cuda.current_stream().cuda_stream))
+        nbf, nbf_aux, nat = ctypes.c_long(), ctypes.c_long(), ctypes.c_long()
+        _check(lib().lbx_dims(self.h, ctypes.byref(nbf), ctypes.byref(nbf_aux), ctypes.byref(nat)))
+        self.nbf, self.nbf_aux, self.natoms = nbf.value, nbf_aux.value, nat.value
+        self.device = torch.device("cuda", torch.cuda.current_device())
+
+    def _basis(self, coords, sh, natoms) -> _Basis:
+        arrs = [np.ascontiguousarray(coords, np.float64), np.ascontiguousarray(sh.atom, np.int32),
+                np.ascontiguousarray(sh.l, np.int32), np.ascontiguousarray(sh.nprim, np.int32),
+                np.ascontiguousarray(sh.exps, np.float64),
+                np.ascontiguousarray(sh.coeffs, np.float64)]
+        self._keep.extend(arrs)
+        ptr = [a.ctypes.data for a in arrs]
+        return _Basis(natoms, ptr[0], sh.nsh, ptr[1], ptr[2], ptr[3], ptr[4], ptr[5])
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().lbx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _out(self, shape):
+        torch = _torch()
+        return torch.empty(shape, dtype=torch.float64, device=self.device)
+
+    def eri4(self):
+        n = self.nbf
+        out = self._out((n, n, n, n))
+        _check(lib().lbx_eri4(self.h, out.data_ptr()))
+        return out
+
+    def eri4_diag(self):
+        out = self._out((self.nbf, self.nbf))
+        _check(lib().lbx_eri4_diag(self.h, out.data_ptr()))
+        return out
+
+    def eri3(self):
+        out = self._out((self.nbf, self.nbf, self.nbf_aux))
+        _check(lib().lbx_eri3(self.h, out.data_ptr()))
+        return out
+
+    def eri2(self):
+        out = self._out((self.nbf_aux, self.nbf_aux))
+        _check(lib().lbx_eri2(self.h, out.data_ptr()))
+        return out
+
+    def eri2_diag(self):
+        out = self._out((self.nbf_aux,))
+        _check(lib().lbx_eri2_diag(self.h, out.data_ptr()))
+        return out
+
+    def jk(self, D, schwarz: float = 1e-12, precision: int = LBX_FP64, mixed_thresh: float = 1e-5,
+           J=None, K=None):
+        torch = _torch()
+        D = D.to(device=self.device, dtype=torch.float64).contiguous()
+        n = self.nbf
+        if D.shape != (n, n):
+            raise LbxError(f"density must be ({n}, {n})")
+        J = self._out((n, n)) if J is None else J
+        K = self._out((n, n)) if K is None else K
+        st = JKStats()
+        _check(lib().lbx_jk(self.h, D.data_ptr(), J.data_ptr(), K.data_ptr(), schwarz, precision,
+                            mixed_thresh, ctypes.byref(st)))
+        return J, K, {"quartets": st.quartets, "flops": st.flops, "quartets_fp32": st.quartets_fp32}

@@ -1,0 +1,41 @@
+"""CPU-side checks of the C-ABI library: it loads, exports every symbol include/lbx.h declares,
+and rejects bad arguments before touching a GPU."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from tests.conftest import ROOT
+
+
+def header_symbols():
+    text = open(os.path.join(ROOT, "include", "lbx.h")).read()
+    return sorted(set(re.findall(r"\b(lbx_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_matches_binding():
+    from lible_amd import lbx
+    assert header_symbols() == sorted(lbx.SYMBOLS)
+
+
+def test_library_exports_all_symbols():
+    from lible_amd import lbx
+    if not os.path.exists(lbx.LIB_PATH):
+        pytest.skip("liblbx.so not built")
+    L = ctypes.CDLL(lbx.LIB_PATH)
+    for s in header_symbols():
+        assert hasattr(L, s), s
+
+
+def test_null_arguments_rejected():
+    from lible_amd import lbx
+    if not os.path.exists(lbx.LIB_PATH):
+        pytest.skip("liblbx.so not built")
+    L = lbx.lib()
+    assert L.lbx_create(None, None, None) == -1
+    assert L.lbx_dims(None, None, None, None) == -1
+    assert b"null" in L.lbx_last_error()
+    assert L.lbx_boys(99, 10, None, None, None) == -1
+    assert L.lbx_eri4(None, None) == -1
+    assert L.lbx_jk(None, None, None, None, 1e-12, 0, 1e-5, None) == -1

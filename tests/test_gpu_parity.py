@@ -1,0 +1,127 @@
+"""GPU parity: the HIP path (through the C-ABI) against the oracle on the same inputs.
+
+Bar: FP64 integrals element-wise within 1e-12 relative, with an absolute floor of 1e-14 times
+the largest element for values that are themselves near zero (cancellation in a sum of O(1)
+terms leaves ~1e-16 absolute noise whatever the algorithm).  The reference's checksums are
+checked at 1e-12 relative as in test_oracle_kat.py.
+"""
+import numpy as np
+import pytest
+
+from lible_amd import basis as B
+from tests.conftest import H2O, O3, basis_path
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-12
+FLOOR = 1e-14
+
+
+def close(gpu, ref, rel=REL, floor=FLOOR):
+    gpu = np.asarray(gpu)
+    ref = np.asarray(ref)
+    scale = np.abs(ref).max() if ref.size else 0.0
+    err = np.abs(gpu - ref)
+    tol = rel * np.abs(ref) + floor * scale
+    bad = err > tol
+    if bad.any():
+        i = np.unravel_index(np.argmax(err / np.maximum(tol, 1e-300)), ref.shape)
+        raise AssertionError(f"{bad.sum()} of {ref.size} elements out of tolerance; worst at {i}: "
+                             f"gpu {gpu[i]!r} ref {ref[i]!r} (scale {scale:.3e})")
+
+
+@pytest.fixture(scope="module")
+def lbx():
+    from lible_amd import lbx as L
+    L.lib()
+    return L
+
+
+def mol(main, geom, aux=None):
+    nrs, xyz = geom
+    return B.molecule_from_basis_files(basis_path(main), nrs, xyz,
+                                       basis_path(aux) if aux else None)
+
+
+def test_boys(lbx, oracle):
+    import torch
+    rng = np.random.default_rng(1234)
+    x = np.concatenate([rng.uniform(0, 60, 1_000_000),
+                        [0.0, 1e-12, 0.005, 0.01, 29.995, 30.0, 30.0 + 1e-9, 45.0, 1e3]])
+    xt = torch.tensor(x, dtype=torch.float64, device="cuda")
+    for n in (0, 5, 13, 16):
+        got = lbx.boys(n, xt).cpu().numpy()
+        ref = oracle.boys_many(n, x)
+        close(got, ref, rel=1e-13, floor=0.0)
+
+
+@pytest.mark.parametrize("case", ["o3_svp", "h2o_svp", "c1_small"])
+def test_eri4_dense(lbx, oracle, case):
+    if case == "o3_svp":
+        m = mol("def2-svp", O3)
+    elif case == "h2o_svp":
+        m = mol("def2-svp", H2O)
+    else:
+        m = B.synthetic_molecule(77, 3, 4.0, B.RECIPE_SMALL)
+    ctx = lbx.Context(m)
+    got = ctx.eri4().cpu().numpy()
+    ref = oracle.Structure(m).eri4()
+    close(got, ref)
+    if case == "o3_svp":
+        assert abs(np.abs(got).sum() - 8246.711763196334) <= REL * 8246.7   # tests_ints.cpp:748
+
+
+def test_eri4_c1_config(lbx, oracle):
+    # 6 atoms in a 6-bohr cube, 3s(K=3) 2p(K=2) 1d per atom: 84 functions, dense 84^4
+    m = B.config_molecule("c1_eri4_dense")
+    ctx = lbx.Context(m)
+    got = ctx.eri4().cpu().numpy()
+    ref = oracle.Structure(m).eri4()
+    close(got, ref)
+
+
+def test_eri4_diag(lbx, oracle):
+    m = mol("def2-tzvp", O3)
+    got = lbx.Context(m).eri4_diag().cpu().numpy()
+    ref = oracle.Structure(m).eri4_diag()
+    close(got, ref)
+    assert abs(np.abs(got).sum() - 306.806419250122) <= REL * 306.8      # tests_ints.cpp:712
+
+
+def test_eri3(lbx, oracle):
+    m = mol("def2-tzvp", O3, "def2-qzvp-rifit")
+    got = lbx.Context(m).eri3().cpu().numpy()
+    ref = oracle.Structure(m).eri3()
+    close(got, ref)
+    assert abs(np.abs(got).sum() - 25854.105851585416531) <= REL * 25854.1  # :730
+
+
+def test_eri2(lbx, oracle):
+    m = mol("def2-svp", H2O, "def2-qzvp-rifit")
+    got = lbx.Context(m).eri2().cpu().numpy()
+    ref = oracle.Structure(m).eri2()
+    close(got, ref)
+    assert abs(np.abs(got).sum() - 8801.334703460838) <= REL * 8801.3     # :694
+
+
+def test_eri2_diag(lbx, oracle):
+    m = mol("def2-svp", H2O, "def2-universal-jkfit")
+    got = lbx.Context(m).eri2_diag().cpu().numpy()
+    ref = oracle.Structure(m).eri2_diag()
+    close(got, ref)
+    assert abs(np.abs(got).sum() - 830.788976485982) <= REL * 830.79     # :149
+
+
+@pytest.mark.parametrize("thresh", [0.0, 1e-12])
+def test_jk_small(lbx, oracle, thresh):
+    import torch
+    m = B.synthetic_molecule(2024, 4, 5.0, B.RECIPE_MAIN)
+    ctx = lbx.Context(m)
+    D = B.density_from_orbitals(5, ctx.nbf, 20, 0.1)
+    J, K, st = ctx.jk(torch.tensor(D, device="cuda"), schwarz=thresh)
+    Jr, Kr, sr = oracle.Structure(m).jk(D, thresh=thresh)
+    J, K = J.cpu().numpy(), K.cpu().numpy()
+    close(J, Jr, floor=1e-13)
+    close(K, Kr, floor=1e-13)
+    assert st["quartets"] == sr["quartets"]
+    assert abs(st["flops"] - sr["flops"]) <= 1e-9 * sr["flops"]
