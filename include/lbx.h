@@ -88,6 +88,21 @@ typedef struct lbx_jk_stats {
 int lbx_jk(lbx_context *ctx, const double *D, double *J, double *K, double schwarz_thresh,
            int precision, double mixed_thresh, lbx_jk_stats *stats);
 
+/* Per-class timing of lbx_jk (the reference's only timing facility is eri4Benchmark,
+ * eri4.cpp:97-150, which prints per-(la,lb,lc,ld) wall time).  While enabled, every lbx_jk
+ * records a HIP event pair around each class-pair kernel on the context stream;
+ * lbx_profile_read returns the accumulated totals since the last enable. */
+typedef struct lbx_class_profile {
+    int la, lb, lc, ld;
+    int kernel;            /* 0: one lane per quartet (VALU), 1: one wave per quartet (MFMA) */
+    int calls;
+    double ms;             /* summed kernel time */
+    double quartets;       /* summed quartets */
+    double flops;          /* summed algorithmic FLOPs */
+} lbx_class_profile;
+int lbx_profile_enable(lbx_context *ctx, int on);
+int lbx_profile_read(lbx_context *ctx, lbx_class_profile *out, int capacity, int *count);
+
 #ifdef __cplusplus
 }
 #endif

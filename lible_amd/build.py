@@ -22,7 +22,7 @@ OUT = os.path.join(HERE, "liblbx.so")
 N_UNITS = 12
 
 FLAGS = [
-    "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
+    "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950", "-mcode-object-version=5",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-munsafe-fp-atomics", "-ffp-contract=fast",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
@@ -34,6 +34,24 @@ def hipcc() -> str:
         if c and os.path.exists(c):
             return c
     raise RuntimeError("hipcc not found")
+
+
+def runtime_link() -> list[str]:
+    """Which HIP runtime the library binds to.  One process must hold one HIP runtime: when
+    PyTorch-ROCm is installed it ships its own libamdhip64.so (and HSA runtime), and a second
+    runtime from /opt/rocm in the same process cannot open the device.  The library is therefore
+    linked against the runtime torch carries when torch is present (recorded as the unversioned
+    libamdhip64.so, which the loader shares with torch, or resolves to /opt/rocm/lib when torch is
+    absent at run time), and against /opt/rocm's runtime otherwise."""
+    try:
+        import importlib.util
+        spec = importlib.util.find_spec("torch")
+        tlib = os.path.join(os.path.dirname(spec.origin), "lib") if spec and spec.origin else None
+    except Exception:
+        tlib = None
+    if tlib and os.path.exists(os.path.join(tlib, "libamdhip64.so")):
+        return ["-no-hip-rt", f"-L{tlib}", "-l:libamdhip64.so", f"-Wl,-rpath,{tlib}"]
+    return []
 
 
 def _compile(src: str, obj: str, defines: list[str]) -> tuple[str, float]:
@@ -67,7 +85,7 @@ def build(jobs: int | None = None, verbose: bool = True) -> str:
                 print(f"  {os.path.basename(obj)}  {dt:.1f}s", flush=True)
     objs = [w[1] for w in work]
     tmp = OUT + ".tmp"
-    cmd = [hipcc(), "-shared", "-o", tmp, "--offload-arch=gfx950", "-fPIC"] + objs
+    cmd = [hipcc(), "-shared", "-o", tmp, "--offload-arch=gfx950", "-fPIC"] + objs + runtime_link()
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")

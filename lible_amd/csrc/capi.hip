@@ -328,6 +328,41 @@ struct lbx_context : Context
 {
     // J/K task tables per (bra class, ket class, Schwarz threshold); Q does not depend on D
     std::map<std::tuple<int, int, double>, TaskTable> jk_tasks;
+
+    // per-class timing (lbx_profile_enable)
+    struct Prof
+    {
+        lbx_class_profile rec{};
+        hipEvent_t start = nullptr, stop = nullptr;
+        bool pending = false;
+    };
+    bool profiling = false;
+    std::map<std::pair<int, int>, Prof> prof;
+
+    ~lbx_context()
+    {
+        for (auto &kv : prof)
+        {
+            if (kv.second.start)
+                (void)hipEventDestroy(kv.second.start);
+            if (kv.second.stop)
+                (void)hipEventDestroy(kv.second.stop);
+        }
+    }
+
+    void collectProfile()
+    {
+        for (auto &kv : prof)
+        {
+            Prof &p = kv.second;
+            if (!p.pending)
+                continue;
+            float ms = 0;
+            LBX_HIP(hipEventElapsedTime(&ms, p.start, p.stop));
+            p.rec.ms += ms;
+            p.pending = false;
+        }
+    }
 };
 
 extern "C"
@@ -592,20 +627,89 @@ extern "C"
                                    a.D = D;
                                    a.J = J;
                                    a.K = K;
+                                   lbx_context::Prof *pr = nullptr;
+                                   if (ctx->profiling && t.ntasks > 0)
+                                   {
+                                       pr = &ctx->prof[{static_cast<int>(i), static_cast<int>(j)}];
+                                       if (!pr->start)
+                                       {
+                                           LBX_HIP(hipEventCreate(&pr->start));
+                                           LBX_HIP(hipEventCreate(&pr->stop));
+                                           pr->rec.la = bra.la;
+                                           pr->rec.lb = bra.lb;
+                                           pr->rec.lc = ket.la;
+                                           pr->rec.ld = ket.lb;
+                                           pr->rec.kernel = s.width == 64 ? 0 : 1;
+                                       }
+                                       LBX_HIP(hipEventRecord(pr->start, c.stream));
+                                   }
                                    s.fn(a, c.stream);
                                    LBX_HIP(hipGetLastError());
+                                   if (pr)
+                                   {
+                                       LBX_HIP(hipEventRecord(pr->stop, c.stream));
+                                       pr->pending = true;
+                                       pr->rec.calls += 1;
+                                       pr->rec.quartets += t.quartets;
+                                       pr->rec.flops += t.flops;
+                                   }
                                }
                            const int gridn = static_cast<int>(std::min<long long>((n * n + 255) / 256, 65536));
                            hipLaunchKernelGGL(symmetrizeKernel, dim3(gridn), dim3(256), 0, c.stream, J, static_cast<int>(n));
                            hipLaunchKernelGGL(symmetrizeKernel, dim3(gridn), dim3(256), 0, c.stream, K, static_cast<int>(n));
                            LBX_HIP(hipGetLastError());
                            LBX_HIP(hipStreamSynchronize(c.stream));
+                           if (ctx->profiling)
+                               ctx->collectProfile();
                            if (stats)
                            {
                                stats->quartets = nq;
                                stats->flops = fl;
                                stats->quartets_fp32 = 0;
                            }
+                       });
+    }
+
+    int lbx_profile_enable(lbx_context *ctx, int on)
+    {
+        return guarded([&]
+                       {
+                           if (!ctx)
+                               throw Error(LBX_ERR_INVALID, "lbx_profile_enable: null context");
+                           std::lock_guard<std::mutex> lk(ctx->mu);
+                           if (on)
+                               for (auto &kv : ctx->prof)
+                               {
+                                   lbx_class_profile keep = kv.second.rec;
+                                   kv.second.rec = lbx_class_profile{};
+                                   kv.second.rec.la = keep.la;
+                                   kv.second.rec.lb = keep.lb;
+                                   kv.second.rec.lc = keep.lc;
+                                   kv.second.rec.ld = keep.ld;
+                                   kv.second.rec.kernel = keep.kernel;
+                                   kv.second.pending = false;
+                               }
+                           ctx->profiling = on != 0;
+                       });
+    }
+
+    int lbx_profile_read(lbx_context *ctx, lbx_class_profile *out, int capacity, int *count)
+    {
+        return guarded([&]
+                       {
+                           if (!ctx || !count || (capacity > 0 && !out))
+                               throw Error(LBX_ERR_INVALID, "lbx_profile_read: null argument");
+                           std::lock_guard<std::mutex> lk(ctx->mu);
+                           int n = 0;
+                           for (auto &kv : ctx->prof)
+                           {
+                               if (kv.second.rec.calls == 0)
+                                   continue;
+                               if (n < capacity)
+                                   out[n] = kv.second.rec;
+                               n++;
+                           }
+                           *count = n;
                        });
     }
 }

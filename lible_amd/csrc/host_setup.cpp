@@ -174,16 +174,18 @@ namespace lbx
         return shells;
     }
 
-    // Boys function grid: F_n(x_i), x_i = 0.01 i, for n = 0..kBoysCols-1.  Top column by its
-    // power series (HJO eq. 9.8.11, terms until < 1e-16), lower columns by downward recursion
-    // (HJO eq. 9.8.14) -- the construction of boys_function.cpp:37-69.
+    // Boys function grid: F_n(x_i) for n = 0..kBoysCols-1 on the points x_i = 0.01 i.  Top column
+    // by its power series (HJO eq. 9.8.11, terms until < 1e-16), lower columns by downward
+    // recursion (HJO eq. 9.8.14) -- the construction of boys_function.cpp:37-69.  The grid
+    // abscissae are accumulated by repeated addition of 0.01 as there, so that the tabulated
+    // values (and hence the integrals) agree with the reference's to rounding level.
     std::vector<double> boysTable()
     {
         std::vector<double> t(static_cast<size_t>(kBoysRows) * kBoysCols);
         const int top = kBoysCols - 1;
-        for (int i = 0; i < kBoysRows; i++)
+        double x = 0.0;
+        for (int i = 0; i < kBoysRows; i++, x += kBoysStep)
         {
-            const double x = i * kBoysStep;
             double term = 1.0 / (2.0 * top + 1.0), sum = term;
             for (int k = 1;; k++)
             {

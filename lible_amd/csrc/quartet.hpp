@@ -15,8 +15,9 @@
 //             bra primitive pairs are stacked along M, (prim, tuv) rows, so one GEMM per ket
 //             primitive covers all bra primitives: X[(i,tuv), kl] += R[(i,tuv), t'u'v'] Ek[t'u'v', kl];
 //             the bra transform contracts over the stacked rows, I^T = X^T Eb_stack, reading X^T
-//             straight out of the accumulator layout (rows permuted within each K-chunk, the
-//             same permutation applied to Eb).  R is built cooperatively in LDS.
+//             straight out of the accumulator registers: accumulator element c of a 16-row tile
+//             holds rows 4c..4c+3 across the lane groups, which is exactly the A-operand layout
+//             of K-chunk c.  R is built cooperatively in LDS.
 // Epilogues: J/K contraction with atomics, dense tensor scatter, or the (ab|ab) diagonal.
 #pragma once
 
@@ -360,6 +361,7 @@ namespace lbx
         constexpr int L = C::L, HP = C::HP, HQ = C::HQ, SP = C::SP, SQ = C::SQ, HL = C::HL;
         constexpr int NQ = C::NQ, NP = C::NP, KQ = C::KQ, MT = C::MT, RC = C::RC, CHP = C::CHP;
         constexpr int SA = C::SA, SB = C::SB, SC = C::SC, SD = C::SD;
+        static_assert(CHP <= 64, "one lane per primitive pair of a chunk");
         __shared__ double sR[2][CHP][HL];
         __shared__ double sInit[CHP][L + 1];
         __shared__ double sPQ[CHP][3];
@@ -503,7 +505,8 @@ namespace lbx
 #pragma unroll
                     for (int c = 0; c < 4; c++)
                     {
-                        const int gr = rowbase + 16 * mt + 4 * g + c;
+                        // K-chunk c of X^T = accumulator element c = rows 16 mt + 4 c + (0..3)
+                        const int gr = rowbase + 16 * mt + 4 * c + g;
                         const int pr = gr / HP;
                         const int h = gr - pr * HP;
                         const bool rv = pr < bcnt;
@@ -521,7 +524,8 @@ namespace lbx
                     }
                 }
             }
-            // accumulator layout: lane (g, j), element r = I^T[16 nq + 4 g + r][16 np + j]
+            // accumulator layout of v_mfma_f64_16x16x4f64: lane (g, j), element r holds
+            // D[4 r + g][j]; here D = I^T[16 nq + 4 r + g][16 np + j]
 #pragma unroll
             for (int nq = 0; nq < NQ; nq++)
 #pragma unroll
@@ -529,7 +533,7 @@ namespace lbx
 #pragma unroll
                     for (int r = 0; r < 4; r++)
                     {
-                        const int kl = 16 * nq + 4 * g + r, mn = 16 * np + j;
+                        const int kl = 16 * nq + 4 * r + g, mn = 16 * np + j;
                         if (kl < SQ && mn < SP)
                             sI[mn * SQ + kl] = acc2[nq][np][r];
                     }

@@ -20,7 +20,7 @@ LBX_MIXED = 1
 # every symbol include/lbx.h declares
 SYMBOLS = ["lbx_create", "lbx_destroy", "lbx_last_error", "lbx_dims", "lbx_set_stream",
            "lbx_synchronize", "lbx_boys", "lbx_eri4", "lbx_eri4_diag", "lbx_eri3", "lbx_eri2",
-           "lbx_eri2_diag", "lbx_jk"]
+           "lbx_eri2_diag", "lbx_jk", "lbx_profile_enable", "lbx_profile_read"]
 
 
 class LbxError(RuntimeError):
@@ -39,6 +39,13 @@ class JKStats(ctypes.Structure):
                 ("quartets_fp32", ctypes.c_double)]
 
 
+class ClassProfile(ctypes.Structure):
+    _fields_ = [("la", ctypes.c_int), ("lb", ctypes.c_int), ("lc", ctypes.c_int),
+                ("ld", ctypes.c_int), ("kernel", ctypes.c_int), ("calls", ctypes.c_int),
+                ("ms", ctypes.c_double), ("quartets", ctypes.c_double),
+                ("flops", ctypes.c_double)]
+
+
 _LIB = None
 
 
@@ -47,6 +54,10 @@ def lib():
     if _LIB is None:
         if not os.path.exists(LIB_PATH):
             raise LbxError(f"{LIB_PATH} is not built (run `python -m lible_amd.build`)")
+        # One HIP runtime per process: torch carries its own libamdhip64 / HSA runtime.  Loading
+        # torch first makes liblbx.so bind to that runtime (same sonames) instead of mapping a
+        # second one from /opt/rocm, which could not open the device.
+        import torch  # noqa: F401
         L = ctypes.CDLL(LIB_PATH)
         p, i, d, l = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_long
         L.lbx_last_error.restype = ctypes.c_char_p
@@ -59,6 +70,8 @@ def lib():
         for name in ("lbx_eri4", "lbx_eri4_diag", "lbx_eri3", "lbx_eri2", "lbx_eri2_diag"):
             getattr(L, name).argtypes = [p, p]
         L.lbx_jk.argtypes = [p, p, p, p, d, i, d, ctypes.POINTER(JKStats)]
+        L.lbx_profile_enable.argtypes = [p, i]
+        L.lbx_profile_read.argtypes = [p, ctypes.POINTER(ClassProfile), i, ctypes.POINTER(i)]
         _LIB = L
     return _LIB
 
@@ -166,3 +179,15 @@ class Context:
         _check(lib().lbx_jk(self.h, D.data_ptr(), J.data_ptr(), K.data_ptr(), schwarz, precision,
                             mixed_thresh, ctypes.byref(st)))
         return J, K, {"quartets": st.quartets, "flops": st.flops, "quartets_fp32": st.quartets_fp32}
+
+    def profile(self, on: bool):
+        _check(lib().lbx_profile_enable(self.h, int(on)))
+
+    def profile_read(self) -> list[dict]:
+        n = ctypes.c_int()
+        _check(lib().lbx_profile_read(self.h, None, 0, ctypes.byref(n)))
+        buf = (ClassProfile * max(n.value, 1))()
+        _check(lib().lbx_profile_read(self.h, buf, n.value, ctypes.byref(n)))
+        return [{"class": (r.la, r.lb, r.lc, r.ld), "kernel": "valu" if r.kernel == 0 else "mfma",
+                 "calls": r.calls, "ms": r.ms, "quartets": r.quartets, "flops": r.flops}
+                for r in buf[:n.value]]

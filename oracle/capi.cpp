@@ -326,6 +326,8 @@ extern "C"
                          {
                              stats[0] = st.n_quartets;
                              stats[1] = st.flops;
+                             stats[2] = st.seconds_setup;
+                             stats[3] = st.seconds_jk;
                          }
                      });
     }

@@ -57,8 +57,10 @@ namespace orc
     JKStats jk(const Structure &s, const double *D, double *J, double *K, double thresh,
                int stride)
     {
+        const double t0 = omp_get_wtime();
         auto sp = shellPairData(true, s);
         auto Q = schwarz(sp);
+        const double t1 = omp_get_wtime();
         const size_t n = s.nbf;
         JKStats st;
         int nth = omp_get_max_threads();
@@ -113,6 +115,8 @@ namespace orc
                     }
                 }
             }
+        st.seconds_setup = t1 - t0;
+        st.seconds_jk = omp_get_wtime() - t1;
         for (size_t i = 0; i < n * n; i++)
         {
             J[i] = 0;

@@ -205,6 +205,8 @@ namespace orc
     struct JKStats
     {
         double n_quartets = 0, flops = 0;
+        double seconds_setup = 0;   // pair data + Schwarz factors
+        double seconds_jk = 0;      // quartet loop
     };
     // J, K (nbf^2) from symmetric D over unique quartets with Schwarz screening; when
     // sample_stride > 1 only every sample_stride-th bra pair of each class is processed

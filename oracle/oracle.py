@@ -211,9 +211,10 @@ class Structure:
         D = np.ascontiguousarray(D, np.float64)
         J = np.zeros_like(D)
         K = np.zeros_like(D)
-        stats = np.zeros(2)
+        stats = np.zeros(4)
         _check(lib().orc_jk(self.h, _ptr(D), _ptr(J), _ptr(K), thresh, stride, _ptr(stats)))
-        return J, K, {"quartets": stats[0], "flops": stats[1]}
+        return J, K, {"quartets": stats[0], "flops": stats[1], "seconds_setup": stats[2],
+                      "seconds_jk": stats[3]}
 
     def jk_grad(self, D: np.ndarray, thresh: float = 1e-12) -> np.ndarray:
         D = np.ascontiguousarray(D, np.float64)
