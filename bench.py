@@ -63,6 +63,7 @@ def main(argv=None):
     ap.add_argument("--config", default="c2_jk_fp64")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
@@ -89,11 +90,19 @@ def main(argv=None):
     log(f"[bench] {a.config}: nbf={ctx.nbf} atoms={mol.natoms} setup {t_setup:.2f} s")
 
     t0 = time.perf_counter()
+    _, _, st = ctx.jk(D, schwarz=SCHWARZ, J=J, K=K)
+    torch.cuda.synchronize()
+    t_first = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    if not a.no_tune:
+        ctx.tune(D, schwarz=SCHWARZ)
+    torch.cuda.synchronize()
+    t_tune = time.perf_counter() - t0
     for _ in range(max(a.warmup, 1)):
         _, _, st = ctx.jk(D, schwarz=SCHWARZ, J=J, K=K)
     torch.cuda.synchronize()
-    log(f"[bench] warmup (incl. Schwarz factors + task tables) {time.perf_counter() - t0:.2f} s, "
-        f"quartets {st['quartets']:.3e}")
+    log(f"[bench] first call (Schwarz factors + task tables) {t_first:.2f} s, tuning {t_tune:.2f} s,"
+        f" quartets {st['quartets']:.3e}")
 
     ctx.profile(True)
     torch.cuda.synchronize()
@@ -143,9 +152,11 @@ def main(argv=None):
         "total_frac_fp64_peak": tot_tflops / FP64_PEAK_TFLOPS,
         "energy_2e": E,
         "roofline": {
-            "bound": "mfma" if dom["kernel"] == "mfma" else "valu",
-            "kernel": "quartet%s<%d,%d,%d,%d> (J/K)" % ("M" if dom["kernel"] == "mfma" else "V",
-                                                       *dom["class"]),
+            # FP64 compute bound: MI355X's 78.6 TFLOP/s FP64 peak is the same for MFMA and VALU
+            "bound": "mfma",
+            "kernel": "%s<%d,%d,%d,%d> J/K (%s)" % (
+                "quartetM" if dom["kernel"] == "mfma" else "quartetV", *dom["class"],
+                dom["kernel"]),
             "achieved": dom_tflops,
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
@@ -157,9 +168,12 @@ def main(argv=None):
         "classes": sorted(
             [{"class": "".join(map(str, p["class"])), "kernel": p["kernel"],
               "ms": round(p["ms"] / p["calls"], 4),
+              "quartets": p["quartets"] / p["calls"],
               "tflops": round(p["flops"] / max(p["ms"], 1e-9) / 1e9, 3)} for p in prof],
             key=lambda r: -r["ms"]),
         "setup_s": t_setup,
+        "first_call_s": t_first,
+        "tune_s": t_tune,
     }
     if not a.no_cpu_baseline:
         log("[bench] cpu baseline (oracle sample)")

@@ -88,13 +88,24 @@ typedef struct lbx_jk_stats {
 int lbx_jk(lbx_context *ctx, const double *D, double *J, double *K, double schwarz_thresh,
            int precision, double mixed_thresh, lbx_jk_stats *stats);
 
+/* Autotuner (no reference counterpart).  Each class pair (la lb|lc ld) has up to three kernel
+ * variants: one lane per quartet with the bra pairs on the lanes, the same with the ket pairs
+ * on the lanes, and one wave per quartet on FP64 MFMA.  lbx_tune times every variant of every
+ * class pair on this density and threshold and keeps the fastest for later lbx_jk calls on
+ * the context; without it lbx_jk uses a work estimate from the primitive-pair counts. */
+int lbx_tune(lbx_context *ctx, const double *D, double schwarz_thresh);
+/* Fixes the variant (0, 1, 2 as above) of class pair (la lb|lc ld), la >= lb, lc >= ld, bra
+ * class >= ket class in the order (0,0),(1,0),(1,1),(2,0),...; fails if the class pair has no
+ * such variant.  variant -1 restores the default. */
+int lbx_set_variant(lbx_context *ctx, int la, int lb, int lc, int ld, int variant);
+
 /* Per-class timing of lbx_jk (the reference's only timing facility is eri4Benchmark,
  * eri4.cpp:97-150, which prints per-(la,lb,lc,ld) wall time).  While enabled, every lbx_jk
  * records a HIP event pair around each class-pair kernel on the context stream;
  * lbx_profile_read returns the accumulated totals since the last enable. */
 typedef struct lbx_class_profile {
     int la, lb, lc, ld;
-    int kernel;            /* 0: one lane per quartet (VALU), 1: one wave per quartet (MFMA) */
+    int kernel;            /* 0: lane per quartet, bra on lanes; 1: ket on lanes; 2: MFMA */
     int calls;
     double ms;             /* summed kernel time */
     double quartets;       /* summed quartets */

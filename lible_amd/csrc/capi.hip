@@ -150,10 +150,12 @@ namespace lbx
             Diagonal    // p == q only
         };
 
-        // Bra positions [plo, phi) per ket position q, with the Schwarz cut Q_p Q_q >= thresh on
-        // descending-sorted Q (thresh <= 0: no screening).
+        // Lane-side positions [plo, phi) per uniform-side position q, with the Schwarz cut
+        // Q_p Q_q >= thresh on descending-sorted Q (thresh <= 0: no screening).  `bra` is the
+        // lane side and `ket` the uniform side; lane_is_bra says whether that matches the
+        // canonical (bra class >= ket class) roles, which the FLOP count follows.
         TaskTable buildTasks(Context &c, const PairClass &bra, const PairClass &ket, Pairing mode,
-                             bool same, double thresh, int width)
+                             bool same, double thresh, int width, bool lane_is_bra = true)
         {
             TaskTable t;
             t.nket = ket.n_pairs;
@@ -166,7 +168,8 @@ namespace lbx
                 qb[i] = bra.Q.empty() ? 1.0 : bra.Q[bra.order[i]];
                 kab_prefix[i + 1] = kab_prefix[i] + bra.pcnt[bra.order[i]];
             }
-            const int la = bra.la, lb = bra.lb, lc = ket.la, ld = ket.lb;
+            const PairClass &cb = lane_is_bra ? bra : ket, &ck = lane_is_bra ? ket : bra;
+            const int la = cb.la, lb = cb.lb, lc = ck.la, ld = ck.lb;
             const double Hab = nHerm(la + lb), Hcd = nHerm(lc + ld);
             const double Sab = nSph(la) * nSph(lb), Scd = nSph(lc) * nSph(ld);
             const double perKK = quartetFlops(la, lb, lc, ld, 1, 1) - 2 * Sab * Hab * Scd - 12 * Sab * Scd;
@@ -207,10 +210,13 @@ namespace lbx
                 acc += (cnt + width - 1) / width;
                 if (cnt > 0)
                 {
-                    const double kcd = ket.pcnt[ket.order[q]];
-                    const double skab = kab_prefix[hi] - kab_prefix[lo];
+                    const double kq = ket.pcnt[ket.order[q]];
+                    const double skl = kab_prefix[hi] - kab_prefix[lo];
                     t.quartets += cnt;
-                    t.flops += perKK * kcd * skab + 2 * Sab * Hab * Scd * skab + 12 * Sab * Scd * cnt;
+                    if (lane_is_bra)
+                        t.flops += perKK * kq * skl + 2 * Sab * Hab * Scd * skl + 12 * Sab * Scd * cnt;
+                    else
+                        t.flops += perKK * kq * skl + 2 * Sab * Hab * Scd * kq * cnt + 12 * Sab * Scd * cnt;
                 }
             }
             t.prefix[t.nket] = acc;
@@ -239,14 +245,51 @@ namespace lbx
             return a;
         }
 
-        const KernelSlot &slotFor(int mode, const PairClass &bra, const PairClass &ket)
+        int rFlopsHost(int L)
         {
-            const KernelSlot &s = kernelSlot(mode, bra.la, bra.lb, ket.la, ket.lb);
+            int r = 0;
+            for (int n = 0; n < L; n++)
+                for (int m = 1; m <= L - n; m++)
+                    r += (m + 1) * (m + 2) / 2;
+            return 2 * r;
+        }
+
+        // Work estimate of a one-lane-per-quartet variant per quartet, from the classes' mean
+        // primitive-pair counts: ket transform and Boys + R per primitive quartet (R once per
+        // column pass), bra transform per lane-side primitive pair.
+        double valuCost(const PairClass &lane, const PairClass &unif, int nc)
+        {
+            const double kl = lane.n_pairs ? double(lane.n_ppairs) / lane.n_pairs : 1;
+            const double ku = unif.n_pairs ? double(unif.n_ppairs) / unif.n_pairs : 1;
+            const double hl = nHerm(lane.la + lane.lb), hu = nHerm(unif.la + unif.lb);
+            const double sl = nSph(lane.la) * nSph(lane.lb), su = nSph(unif.la) * nSph(unif.lb);
+            const int L = lane.la + lane.lb + unif.la + unif.lb;
+            const double passes = std::ceil(su / nc);
+            return kl * ku * (hl * hu * su + passes * (60 + rFlopsHost(L))) + kl * sl * hl * su;
+        }
+
+        // Default variant before tuning: the cheaper one-lane-per-quartet form by the work
+        // estimate; MFMA when no such form fits the registers.
+        int defaultVariant(int mode, const PairClass &bra, const PairClass &ket)
+        {
+            const KernelSlot &vb = kernelSlot(mode, bra.la, bra.lb, ket.la, ket.lb, VAR_VALU_BRA);
+            const KernelSlot &vk = kernelSlot(mode, bra.la, bra.lb, ket.la, ket.lb, VAR_VALU_KET);
+            double cb = vb.fn ? valuCost(bra, ket, vb.nc) : 1e300;
+            double ck = vk.fn ? valuCost(ket, bra, vk.nc) : 1e300;
+            if (!vb.fn && !vk.fn)
+                return VAR_MFMA;
+            return cb <= ck ? VAR_VALU_BRA : VAR_VALU_KET;
+        }
+
+        const KernelSlot &slotFor(int mode, const PairClass &bra, const PairClass &ket, int variant)
+        {
+            const KernelSlot &s = kernelSlot(mode, bra.la, bra.lb, ket.la, ket.lb, variant);
             if (!s.fn)
                 throw Error(LBX_ERR_INVALID, "no kernel for class (" + std::to_string(bra.la) +
                                                      std::to_string(bra.lb) + "|" +
                                                      std::to_string(ket.la) + std::to_string(ket.lb) +
-                                                     ") mode " + std::to_string(mode));
+                                                     ") mode " + std::to_string(mode) + " variant " +
+                                                     std::to_string(variant));
             return s;
         }
 
@@ -255,7 +298,7 @@ namespace lbx
         {
             if (bra.n_pairs == 0 || ket.n_pairs == 0)
                 return;
-            const KernelSlot &s = slotFor(mode, bra, ket);
+            const KernelSlot &s = slotFor(mode, bra, ket, defaultVariant(mode, bra, ket));
             TaskTable t = buildTasks(c, bra, ket, pairing, same, thresh, s.width);
             KArgs a = baseArgs(c, bra, ket, t, same);
             a.D = extra.D;
@@ -282,13 +325,13 @@ namespace lbx
         }
     }
 
-    KernelSlot &kernelSlot(int mode, int la, int lb, int lc, int ld)
+    KernelSlot &kernelSlot(int mode, int la, int lb, int lc, int ld, int variant)
     {
-        static KernelSlot tab[3][kMaxLAux + 1][kMaxLMain + 1][kMaxLAux + 1][kMaxLMain + 1];
+        static KernelSlot tab[3][kMaxLAux + 1][kMaxLMain + 1][kMaxLAux + 1][kMaxLMain + 1][N_VARIANTS];
         if (mode < 0 || mode > 2 || la < 0 || la > kMaxLAux || lc < 0 || lc > kMaxLAux || lb < 0 ||
-            lb > kMaxLMain || ld < 0 || ld > kMaxLMain)
+            lb > kMaxLMain || ld < 0 || ld > kMaxLMain || variant < 0 || variant >= N_VARIANTS)
             throw Error(LBX_ERR_INVALID, "kernel slot out of range");
-        return tab[mode][la][lb][lc][ld];
+        return tab[mode][la][lb][lc][ld][variant];
     }
 
     // Schwarz factors Q_ab = sqrt(max |(ab|ab)|) per pair of every main class; pairs sorted by
@@ -326,8 +369,42 @@ using namespace lbx;
 
 struct lbx_context : Context
 {
-    // J/K task tables per (bra class, ket class, Schwarz threshold); Q does not depend on D
-    std::map<std::tuple<int, int, double>, TaskTable> jk_tasks;
+    // J/K task tables per (bra class, ket class, Schwarz threshold, variant); Q does not
+    // depend on D
+    std::map<std::tuple<int, int, double, int>, TaskTable> jk_tasks;
+    // kernel variant per class pair chosen by lbx_tune (absent: work-estimate default)
+    std::map<std::pair<int, int>, int> tuned;
+
+    int variantFor(int i, int j)
+    {
+        auto it = tuned.find({i, j});
+        return it != tuned.end() ? it->second : defaultVariant(MODE_JK, cls[i], cls[j]);
+    }
+
+    // Queues the J/K kernel of class pair (i >= j) with the given variant on the context stream.
+    const TaskTable &launchJK(int i, int j, int var, const double *D, double *J, double *K,
+                              double thresh)
+    {
+        const PairClass &bra = cls[i], &ket = cls[j];
+        const KernelSlot &s = slotFor(MODE_JK, bra, ket, var);
+        const auto key = std::make_tuple(i, j, thresh, var);
+        auto it = jk_tasks.find(key);
+        if (it == jk_tasks.end())
+            it = jk_tasks
+                         .emplace(key, s.swap ? buildTasks(*this, ket, bra, Pairing::Unique, false, thresh,
+                                                           s.width, false)
+                                              : buildTasks(*this, bra, ket, Pairing::Unique, i == j, thresh,
+                                                           s.width))
+                         .first;
+        const TaskTable &t = it->second;
+        KArgs a = baseArgs(*this, bra, ket, t, i == j);
+        a.D = D;
+        a.J = J;
+        a.K = K;
+        s.fn(a, stream);
+        LBX_HIP(hipGetLastError());
+        return t;
+    }
 
     // per-class timing (lbx_profile_enable)
     struct Prof
@@ -601,10 +678,6 @@ extern "C"
                            const long long n = static_cast<long long>(c.nbf);
                            fill(J, n * n, 0.0, c.stream);
                            fill(K, n * n, 0.0, c.stream);
-                           KArgs extra{};
-                           extra.D = D;
-                           extra.J = J;
-                           extra.K = K;
                            double nq = 0, fl = 0;
                            for (size_t i = 0; i < c.cls.size(); i++)
                                for (size_t j = 0; j <= i; j++)
@@ -612,23 +685,9 @@ extern "C"
                                    const PairClass &bra = c.cls[i], &ket = c.cls[j];
                                    if (bra.n_pairs == 0 || ket.n_pairs == 0)
                                        continue;
-                                   const KernelSlot &s = slotFor(MODE_JK, bra, ket);
-                                   const auto key = std::make_tuple(static_cast<int>(i), static_cast<int>(j), schwarz_thresh);
-                                   auto it = ctx->jk_tasks.find(key);
-                                   if (it == ctx->jk_tasks.end())
-                                       it = ctx->jk_tasks
-                                                    .emplace(key, buildTasks(c, bra, ket, Pairing::Unique, i == j,
-                                                                             schwarz_thresh, s.width))
-                                                    .first;
-                                   const TaskTable &t = it->second;
-                                   nq += t.quartets;
-                                   fl += t.flops;
-                                   KArgs a = baseArgs(c, bra, ket, t, i == j);
-                                   a.D = D;
-                                   a.J = J;
-                                   a.K = K;
+                                   const int var = ctx->variantFor(static_cast<int>(i), static_cast<int>(j));
                                    lbx_context::Prof *pr = nullptr;
-                                   if (ctx->profiling && t.ntasks > 0)
+                                   if (ctx->profiling)
                                    {
                                        pr = &ctx->prof[{static_cast<int>(i), static_cast<int>(j)}];
                                        if (!pr->start)
@@ -639,12 +698,14 @@ extern "C"
                                            pr->rec.lb = bra.lb;
                                            pr->rec.lc = ket.la;
                                            pr->rec.ld = ket.lb;
-                                           pr->rec.kernel = s.width == 64 ? 0 : 1;
                                        }
+                                       pr->rec.kernel = var;
                                        LBX_HIP(hipEventRecord(pr->start, c.stream));
                                    }
-                                   s.fn(a, c.stream);
-                                   LBX_HIP(hipGetLastError());
+                                   const TaskTable &t = ctx->launchJK(static_cast<int>(i), static_cast<int>(j),
+                                                                      var, D, J, K, schwarz_thresh);
+                                   nq += t.quartets;
+                                   fl += t.flops;
                                    if (pr)
                                    {
                                        LBX_HIP(hipEventRecord(pr->stop, c.stream));
@@ -667,6 +728,89 @@ extern "C"
                                stats->flops = fl;
                                stats->quartets_fp32 = 0;
                            }
+                       });
+    }
+
+    int lbx_tune(lbx_context *ctx, const double *D, double schwarz_thresh)
+    {
+        return guarded([&]
+                       {
+                           if (!ctx || !D)
+                               throw Error(LBX_ERR_INVALID, "lbx_tune: null argument");
+                           if (!(schwarz_thresh >= 0))
+                               throw Error(LBX_ERR_INVALID, "lbx_tune: negative Schwarz threshold");
+                           std::lock_guard<std::mutex> lk(ctx->mu);
+                           Context &c = *ctx;
+                           ensureSchwarz(c);
+                           const long long n = static_cast<long long>(c.nbf);
+                           DevBuf J(n * n), K(n * n);
+                           hipEvent_t e0, e1;
+                           LBX_HIP(hipEventCreate(&e0));
+                           LBX_HIP(hipEventCreate(&e1));
+                           try
+                           {
+                               for (size_t i = 0; i < c.cls.size(); i++)
+                                   for (size_t j = 0; j <= i; j++)
+                                   {
+                                       const PairClass &bra = c.cls[i], &ket = c.cls[j];
+                                       if (bra.n_pairs == 0 || ket.n_pairs == 0)
+                                           continue;
+                                       int best = -1;
+                                       float best_ms = 0;
+                                       for (int v = 0; v < N_VARIANTS; v++)
+                                       {
+                                           if (!kernelSlot(MODE_JK, bra.la, bra.lb, ket.la, ket.lb, v).fn)
+                                               continue;
+                                           // first launch builds the task table and loads the code
+                                           ctx->launchJK(static_cast<int>(i), static_cast<int>(j), v, D, J.p, K.p,
+                                                         schwarz_thresh);
+                                           LBX_HIP(hipEventRecord(e0, c.stream));
+                                           ctx->launchJK(static_cast<int>(i), static_cast<int>(j), v, D, J.p, K.p,
+                                                         schwarz_thresh);
+                                           LBX_HIP(hipEventRecord(e1, c.stream));
+                                           LBX_HIP(hipEventSynchronize(e1));
+                                           float ms = 0;
+                                           LBX_HIP(hipEventElapsedTime(&ms, e0, e1));
+                                           if (best < 0 || ms < best_ms)
+                                           {
+                                               best = v;
+                                               best_ms = ms;
+                                           }
+                                       }
+                                       if (best >= 0)
+                                           ctx->tuned[{static_cast<int>(i), static_cast<int>(j)}] = best;
+                                   }
+                           }
+                           catch (...)
+                           {
+                               (void)hipEventDestroy(e0);
+                               (void)hipEventDestroy(e1);
+                               throw;
+                           }
+                           (void)hipEventDestroy(e0);
+                           (void)hipEventDestroy(e1);
+                       });
+    }
+
+    int lbx_set_variant(lbx_context *ctx, int la, int lb, int lc, int ld, int variant)
+    {
+        return guarded([&]
+                       {
+                           if (!ctx)
+                               throw Error(LBX_ERR_INVALID, "lbx_set_variant: null context");
+                           if (la < lb || lc < ld || la > kMaxLMain || lc > kMaxLMain || lb < 0 || ld < 0 ||
+                               pairClassIdx(la, lb) < pairClassIdx(lc, ld))
+                               throw Error(LBX_ERR_INVALID, "lbx_set_variant: not a canonical class pair");
+                           std::lock_guard<std::mutex> lk(ctx->mu);
+                           const std::pair<int, int> key{pairClassIdx(la, lb), pairClassIdx(lc, ld)};
+                           if (variant < 0)
+                           {
+                               ctx->tuned.erase(key);
+                               return;
+                           }
+                           if (variant >= N_VARIANTS || !kernelSlot(MODE_JK, la, lb, lc, ld, variant).fn)
+                               throw Error(LBX_ERR_INVALID, "lbx_set_variant: class pair has no such variant");
+                           ctx->tuned[key] = variant;
                        });
     }
 

@@ -4,6 +4,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <utility>
+
 #include "lbx_common.hpp"
 
 namespace lbx
@@ -66,48 +68,66 @@ namespace lbx
         return (t + u + v == 0) ? n : L + hermIdx(t, u, v);
     }
 
+    // The recursion as a straight-line program generated at compile time: one operation
+    // R[dst] = axis * R[s1] + coef * R[s2] per (level, tuv), in the in-place order above.
+    struct ROp
+    {
+        short dst, s1, s2;
+        signed char ax, coef;
+    };
+
+    constexpr LBX_HD int rOpCount(int L)
+    {
+        int n = 0;
+        for (int lev = L - 1; lev >= 0; lev--)
+            n += nHerm(L - lev) - 1;
+        return n;
+    }
+
+    template <int L>
+    struct RProgram
+    {
+        static constexpr int N = rOpCount(L);
+        ROp op[N > 0 ? N : 1];
+        constexpr RProgram() : op()
+        {
+            int k = 0;
+            for (int n = L - 1; n >= 0; n--)
+                for (int m = L - n; m >= 1; m--)
+                    for (int t = m; t >= 0; t--)
+                        for (int u = m - t; u >= 0; u--)
+                        {
+                            const int v = m - t - u;
+                            ROp o{};
+                            o.dst = static_cast<short>(L + hermIdx(t, u, v));
+                            const int ax = t > 0 ? 0 : (u > 0 ? 1 : 2);
+                            const int c = ax == 0 ? t : (ax == 1 ? u : v);
+                            const int dt = ax == 0, du = ax == 1, dv = ax == 2;
+                            o.ax = static_cast<signed char>(ax);
+                            o.s1 = static_cast<short>(rSlot<L>(t - dt, u - du, v - dv, n + 1));
+                            o.coef = static_cast<signed char>(c - 1);
+                            o.s2 = static_cast<short>(c > 1 ? rSlot<L>(t - 2 * dt, u - 2 * du, v - 2 * dv, n + 1) : 0);
+                            op[k++] = o;
+                        }
+        }
+    };
+
+    template <int L, size_t... I>
+    __device__ __forceinline__ void hermiteRRun(double *R, double X, double Y, double Z,
+                                                std::index_sequence<I...>)
+    {
+        constexpr RProgram<L> P{};
+        ((R[P.op[I].dst] = (P.op[I].coef > 0)
+                                   ? fma(double(P.op[I].coef), R[P.op[I].s2],
+                                         (P.op[I].ax == 0 ? X : (P.op[I].ax == 1 ? Y : Z)) * R[P.op[I].s1])
+                                   : (P.op[I].ax == 0 ? X : (P.op[I].ax == 1 ? Y : Z)) * R[P.op[I].s1]),
+         ...);
+    }
+
     template <int L>
     __device__ __forceinline__ void hermiteR(double *R, double X, double Y, double Z)
     {
-#pragma unroll
-        for (int n = L - 1; n >= 0; n--)
-        {
-#pragma unroll
-            for (int m = L - n; m >= 1; m--)
-            {
-#pragma unroll
-                for (int t = m; t >= 0; t--)
-                {
-#pragma unroll
-                    for (int u = m - t; u >= 0; u--)
-                    {
-                        const int v = m - t - u;
-                        const int o = L + hermIdx(t, u, v);
-                        if (t > 0)
-                        {
-                            double r = X * R[rSlot<L>(t - 1, u, v, n + 1)];
-                            if (t > 1)
-                                r = fma(double(t - 1), R[rSlot<L>(t - 2, u, v, n + 1)], r);
-                            R[o] = r;
-                        }
-                        else if (u > 0)
-                        {
-                            double r = Y * R[rSlot<L>(t, u - 1, v, n + 1)];
-                            if (u > 1)
-                                r = fma(double(u - 1), R[rSlot<L>(t, u - 2, v, n + 1)], r);
-                            R[o] = r;
-                        }
-                        else
-                        {
-                            double r = Z * R[rSlot<L>(t, u, v - 1, n + 1)];
-                            if (v > 1)
-                                r = fma(double(v - 1), R[rSlot<L>(t, u, v - 2, n + 1)], r);
-                            R[o] = r;
-                        }
-                    }
-                }
-            }
-        }
+        hermiteRRun<L>(R, X, Y, Z, std::make_index_sequence<static_cast<size_t>(rOpCount(L))>{});
     }
 
     // slot in the hermiteR buffer of the Hermite index sum (tuv of bra index i) + (tuv of ket j)
@@ -119,6 +139,19 @@ namespace lbx
                   v = (a >> 16) + (b >> 16);
         return rSlot<L>(t, u, v, 0);
     }
+
+    // rSum over a whole (i < NI, j < NJ) block, evaluated at compile time
+    template <int L, int NI, int NJ>
+    struct RSumTab
+    {
+        short v[NI * NJ];
+        constexpr RSumTab() : v()
+        {
+            for (int i = 0; i < NI; i++)
+                for (int j = 0; j < NJ; j++)
+                    v[i * NJ + j] = static_cast<short>(rSum<L>(i, j));
+        }
+    };
 
     constexpr LBX_HD int hermLevel(int i)
     {

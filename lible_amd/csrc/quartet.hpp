@@ -4,20 +4,25 @@
 // (eri_kernel_funs.hpp:104-193, after doi:10.1002/jcc.26942 eqs. 25-26):
 //   for each bra primitive pair i:  X_i = sum_k R_{ik} . Ek_k          (ket transform)
 //   I = sum_i Eb_i^T X_i                                               (bra transform)
-// with R_{ik}(tuv, t'u'v') = fac R_{t+t',u+u',v+v'} (Hermite integrals, sign of the ket folded
-// into Ek) and Eb, Ek the spherical Hermite-expansion matrices of the primitive pairs.
+// with R_{ik}(tuv, t'u'v') = fac R_{t+t',u+u',v+v'} (Hermite integrals; the ket's sign
+// (-1)^{t'+u'+v'} folded into Ek) and Eb, Ek the spherical Hermite-expansion matrices of the
+// primitive pairs.
 //
 // Two kernels:
-//   quartetV  one lane per quartet, everything in registers (small classes).  A wave holds 64
-//             bra pairs against one ket pair, so every ket operand is a wave-uniform (scalar)
-//             load and the transforms are VALU FMAs with an SGPR operand.
+//   quartetV  one lane per quartet, everything in registers.  A wave holds 64 pairs of one side
+//             (the "lane" side) against one pair of the other (the "uniform" side), so every
+//             uniform-side operand is a scalar load and the transforms are VALU FMAs with an
+//             SGPR operand.  Registers are bounded by splitting the uniform side's spherical
+//             components into column passes (Boys + R recomputed per pass); which side runs on
+//             the lanes is chosen per class to minimise work.
 //   quartetM  one wave per quartet, both transforms on FP64 MFMA (v_mfma_f64_16x16x4f64).  The
 //             bra primitive pairs are stacked along M, (prim, tuv) rows, so one GEMM per ket
 //             primitive covers all bra primitives: X[(i,tuv), kl] += R[(i,tuv), t'u'v'] Ek[t'u'v', kl];
 //             the bra transform contracts over the stacked rows, I^T = X^T Eb_stack, reading X^T
 //             straight out of the accumulator registers: accumulator element c of a 16-row tile
 //             holds rows 4c..4c+3 across the lane groups, which is exactly the A-operand layout
-//             of K-chunk c.  R is built cooperatively in LDS.
+//             of K-chunk c.  R is built cooperatively in LDS.  Used where R alone would not fit
+//             a lane's registers (high total angular momentum).
 // Epilogues: J/K contraction with atomics, dense tensor scatter, or the (ab|ab) diagonal.
 #pragma once
 
@@ -37,8 +42,8 @@ namespace lbx
     struct KArgs
     {
         DPairs bra, ket;
-        const long long *prefix;   // per ket position q: first task (nket + 1 entries)
-        const int *plo, *phi;      // bra sorted-position range [plo, phi) for each q
+        const long long *prefix;   // per uniform-side position q: first task (nket + 1 entries)
+        const int *plo, *phi;      // lane-side sorted-position range [plo, phi) for each q
         int nket;
         long long ntasks;
         int same_class;            // bra class == ket class: quartet p == q is its own image
@@ -56,6 +61,15 @@ namespace lbx
         double *dout;   // (mn|mn) matrix (pair classes) or vector (single classes), may be null
     };
 
+    constexpr LBX_HD int rFlops(int L)
+    {
+        int r = 0;
+        for (int n = 0; n < L; n++)
+            for (int m = 1; m <= L - n; m++)
+                r += (m + 1) * (m + 2) / 2;
+        return 2 * r;
+    }
+
     template <int LA, int LB, int LC, int LD>
     struct Cls
     {
@@ -63,294 +77,375 @@ namespace lbx
         static constexpr int HP = nHerm(LA + LB), HQ = nHerm(LC + LD), HL = nHerm(L);
         static constexpr int SA = nSph(LA), SB = nSph(LB), SC = nSph(LC), SD = nSph(LD);
         static constexpr int SP = SA * SB, SQ = SC * SD;
-        // registers of the one-lane-per-quartet form: I + X + R
-        static constexpr int kValuRegs = SP * SQ + HP * SQ + HL + L;
-        static constexpr bool kValu = kValuRegs <= 160;
-        // MFMA tiling
+
+        // ---- one-lane-per-quartet plan: per-lane doubles of I (S_lane x NC) + X (H_lane x NC)
+        // + R stay within kBudget (the region where the compiler keeps them in registers)
+        static constexpr int kBudget = 160;
+        static constexpr int ncFor(int sl, int hl, int su)
+        {
+            const int nc = (kBudget - (HL + L)) / (sl + hl);
+            return nc < 1 ? 0 : (nc > su ? su : nc);
+        }
+        static constexpr int NC0 = ncFor(SP, HP, SQ);   // lanes = bra, uniform = ket
+        static constexpr int NC1 = ncFor(SQ, HQ, SP);   // lanes = ket, uniform = bra
+
+        // ---- MFMA tiling
         static constexpr int NQ = ceilDiv(SQ, 16), NP = ceilDiv(SP, 16), KQ = ceilDiv(HQ, 4);
-        // row tiles per chunk: accumulators bounded by 64 doubles per lane, and at most 16
-        // primitive pairs' worth of rows (bounds the LDS R buffers)
-        static constexpr int MT0 = (16 / NQ) < 1 ? 1 : (16 / NQ);
-        static constexpr int MT = MT0 < HP ? MT0 : HP;
+        // row tiles per chunk: accumulators (acc1 + acc2) within ~96 doubles per lane, rows for
+        // about four primitive pairs, at least one tile
+        static constexpr int mtBudget = (96 - 4 * NQ * NP) / (4 * NQ);
+        static constexpr int mtRows = ceilDiv(4 * HP, 16);
+        static constexpr int MT0 = mtBudget < mtRows ? mtBudget : mtRows;
+        static constexpr int MT = MT0 < 1 ? 1 : MT0;
         static constexpr int RC = 16 * MT;                          // rows per chunk
         static constexpr int CHP = (RC - 1) / HP + 2;               // primitives a chunk touches
     };
 
-    // ------------------------------------------------------------------------------- epilogues
-    // Quartet integrals available through an accessor I(mn, kl); f = permutational weight.
-    template <class C, class Acc>
-    __device__ __forceinline__ void denseScatter(const KArgs &A, Acc I, int a0, int b0, int c0,
-                                                 int d0, int e0, int ecount, int estep)
+    // ------------------------------------------------------------------------------- dense out
+    template <class C>
+    __device__ __forceinline__ void denseStore(const KArgs &A, double v, int mn, int kl, int a0,
+                                               int b0, int c0, int d0)
     {
         const long long n = A.nbf;
-        for (int e = e0; e < ecount; e += estep)
+        const int mu = mn / C::SB, nu = mn - mu * C::SB;
+        const int ka = kl / C::SD, la = kl - ka * C::SD;
+        const long long i = a0 + mu, j = b0 + nu, k = c0 + ka, l = d0 + la;
+        if (A.kind == 4)
         {
-            const int mn = e / C::SQ, kl = e - mn * C::SQ;
-            const int mu = mn / C::SB, nu = mn - mu * C::SB;
-            const int ka = kl / C::SD, la = kl - ka * C::SD;
-            const double v = I(mn, kl);
-            const long long i = a0 + mu, j = b0 + nu, k = c0 + ka, l = d0 + la;
-            if (A.kind == 4)
-            {
-                double *o = A.out;
-                o[((i * n + j) * n + k) * n + l] = v;
-                o[((i * n + j) * n + l) * n + k] = v;
-                o[((j * n + i) * n + k) * n + l] = v;
-                o[((j * n + i) * n + l) * n + k] = v;
-                o[((k * n + l) * n + i) * n + j] = v;
-                o[((k * n + l) * n + j) * n + i] = v;
-                o[((l * n + k) * n + i) * n + j] = v;
-                o[((l * n + k) * n + j) * n + i] = v;
-            }
-            else if (A.kind == 3)
-            {
-                A.out[(i * n + j) * A.naux + k] = v;
-                A.out[(j * n + i) * A.naux + k] = v;
-            }
-            else
-            {
-                A.out[i * A.naux + k] = v;
-                A.out[k * A.naux + i] = v;
-            }
+            double *o = A.out;
+            o[((i * n + j) * n + k) * n + l] = v;
+            o[((i * n + j) * n + l) * n + k] = v;
+            o[((j * n + i) * n + k) * n + l] = v;
+            o[((j * n + i) * n + l) * n + k] = v;
+            o[((k * n + l) * n + i) * n + j] = v;
+            o[((k * n + l) * n + j) * n + i] = v;
+            o[((l * n + k) * n + i) * n + j] = v;
+            o[((l * n + k) * n + j) * n + i] = v;
+        }
+        else if (A.kind == 3)
+        {
+            A.out[(i * n + j) * A.naux + k] = v;
+            A.out[(j * n + i) * A.naux + k] = v;
+        }
+        else
+        {
+            A.out[i * A.naux + k] = v;
+            A.out[k * A.naux + i] = v;
         }
     }
 
     // ------------------------------------------------------------------------------- quartetV
-    template <int LA, int LB, int LC, int LD, int MODE>
-    __device__ __forceinline__ void quartetVTask(const KArgs &A, long long t, int lane)
+    // One quartet per lane: lane-side pair l (64 consecutive sorted positions) against the
+    // wave-uniform pair u.  SWAP = false: lanes = bra, u = ket; SWAP = true: lanes = ket, u = bra.
+    template <int LA, int LB, int LC, int LD, int MODE, bool SWAP>
+    __device__ __forceinline__ void quartetVTask(const KArgs &A, int q, int local, int lane)
     {
         using C = Cls<LA, LB, LC, LD>;
-        constexpr int L = C::L, HP = C::HP, HQ = C::HQ, SP = C::SP, SQ = C::SQ, HL = C::HL;
+        constexpr int L = C::L, HL = C::HL;
+        // lane side (La, Lb), uniform side (Ua, Ub)
+        constexpr int La = SWAP ? LC : LA, Lb = SWAP ? LD : LB;
+        constexpr int Ua = SWAP ? LA : LC, Ub = SWAP ? LB : LD;
+        constexpr int SLa = nSph(La), SLb = nSph(Lb), SUa = nSph(Ua), SUb = nSph(Ub);
+        constexpr int SL = SLa * SLb, SU = SUa * SUb;
+        constexpr int HLn = nHerm(La + Lb), HU = nHerm(Ua + Ub);
+        constexpr int NC = SWAP ? C::NC1 : C::NC0;
+        constexpr int NPASS = ceilDiv(SU, NC);
+        const DPairs &LP = SWAP ? A.ket : A.bra;
+        const DPairs &UP = SWAP ? A.bra : A.ket;
 
-        const int q = uniform(findKet(A.prefix, A.nket, t));
-        const long long local = t - A.prefix[q];
         const int plo = uniform(A.plo[q]), phi = uniform(A.phi[q]);
-        const int p = plo + static_cast<int>(local) * 64 + lane;
+        const int p = plo + local * 64 + lane;
         const bool active = p < phi;
-        const int cd = uniform(A.ket.order[q]);
-        const int ab = A.bra.order[active ? p : plo];
-        const int kpoff = uniform(A.ket.poff[cd]), kcnt = uniform(A.ket.pcnt[cd]);
-        const int bpoff = A.bra.poff[ab], bcnt = active ? A.bra.pcnt[ab] : 0;
+        const int u = uniform(UP.order[q]);
+        const int l = LP.order[active ? p : plo];
+        const int upoff = uniform(UP.poff[u]), ucnt = uniform(UP.pcnt[u]);
+        const int lpoff = LP.poff[l], lcnt = active ? LP.pcnt[l] : 0;
+        // the ket's E carries the sign (-1)^{t'+u'+v'}
+        const double *__restrict__ eU = (SWAP ? UP.eb : UP.ek) + static_cast<size_t>(upoff) * HU * SU;
+        const double *__restrict__ eL = (SWAP ? LP.ek : LP.eb) + static_cast<size_t>(lpoff) * HLn * SL;
 
-        double I[SP * SQ];
-#pragma unroll
-        for (int e = 0; e < SP * SQ; e++)
-            I[e] = 0.0;
-
-        for (int i = 0; i < bcnt; i++)
-        {
-            const double4 P = A.bra.pp[bpoff + i];
-            double X[HP * SQ];
-#pragma unroll
-            for (int e = 0; e < HP * SQ; e++)
-                X[e] = 0.0;
-            for (int k = 0; k < kcnt; k++)
-            {
-                const double4 Q = A.ket.pp[kpoff + k];
-                const double pq = P.x * Q.x, ps = P.x + Q.x;
-                const double alpha = pq / ps;
-                const double X_ = P.y - Q.y, Y_ = P.z - Q.z, Z_ = P.w - Q.w;
-                const double T = alpha * (X_ * X_ + Y_ * Y_ + Z_ * Z_);
-                double R[HL + L];
-                boysF<L>(T, A.boys, R);
-                const double fac = kTwoPi52 / (pq * sqrt(ps));
-                double sc = fac;
-#pragma unroll
-                for (int n = 0; n <= L; n++)
-                {
-                    R[n] *= sc;
-                    sc *= -2.0 * alpha;
-                }
-                hermiteR<L>(R, X_, Y_, Z_);
-                const double *__restrict__ ek = A.ket.ek + static_cast<size_t>(kpoff + k) * HQ * SQ;
-#pragma unroll
-                for (int h = 0; h < HP; h++)
-#pragma unroll
-                    for (int g = 0; g < HQ; g++)
-                    {
-                        const double r = R[rSum<L>(h, g)];
-#pragma unroll
-                        for (int c = 0; c < SQ; c++)
-                            X[h * SQ + c] = fma(r, ek[g * SQ + c], X[h * SQ + c]);
-                    }
-            }
-            const double *eb = A.bra.eb + static_cast<size_t>(bpoff + i) * HP * SP;
-#pragma unroll
-            for (int h = 0; h < HP; h++)
-#pragma unroll
-                for (int mn = 0; mn < SP; mn++)
-                {
-                    const double e = eb[h * SP + mn];
-#pragma unroll
-                    for (int c = 0; c < SQ; c++)
-                        I[mn * SQ + c] = fma(e, X[h * SQ + c], I[mn * SQ + c]);
-                }
-        }
-
-        const int a0 = active ? A.bra.ao_a[ab] : 0, b0 = active ? A.bra.ao_b[ab] : 0;
-        const int c0 = uniform(A.ket.ao_a[cd]), d0 = uniform(A.ket.ao_b[cd]);
+        const int l0 = active ? LP.ao_a[l] : 0, l1 = active ? LP.ao_b[l] : 0;
+        const int u0 = uniform(UP.ao_a[u]), u1 = uniform(UP.ao_b[u]);
+        double f = 1.0;
         if constexpr (MODE == MODE_JK)
         {
-            double f = 1.0;
-            if (A.bra.sh_a[ab] == A.bra.sh_b[ab])
+            if (LP.sh_a[l] == LP.sh_b[l])
                 f *= 0.5;
-            if (A.ket.sh_a[cd] == A.ket.sh_b[cd])
+            if (UP.sh_a[u] == UP.sh_b[u])
                 f *= 0.5;
             if (A.same_class && p == q)
                 f *= 0.5;
             if (!active)
                 f = 0.0;
-            const long long n = A.nbf;
-            const double *D = A.D;
-            constexpr int SA = C::SA, SB = C::SB, SC = C::SC, SD = C::SD;
-            // J_ab += 2f I D_cd ; J_cd += 2f I^T D_ab
-            double dcd[SQ];
-#pragma unroll
-            for (int ka = 0; ka < SC; ka++)
-#pragma unroll
-                for (int la = 0; la < SD; la++)
-                    dcd[ka * SD + la] = D[(c0 + ka) * n + d0 + la];
-            double jcd[SQ];
-#pragma unroll
-            for (int kl = 0; kl < SQ; kl++)
-                jcd[kl] = 0.0;
-#pragma unroll
-            for (int mu = 0; mu < SA; mu++)
-#pragma unroll
-                for (int nu = 0; nu < SB; nu++)
-                {
-                    const int mn = mu * SB + nu;
-                    const double dab = active ? D[(a0 + mu) * n + b0 + nu] : 0.0;
-                    double s = 0.0;
-#pragma unroll
-                    for (int kl = 0; kl < SQ; kl++)
-                    {
-                        s = fma(I[mn * SQ + kl], dcd[kl], s);
-                        jcd[kl] = fma(I[mn * SQ + kl], dab, jcd[kl]);
-                    }
-                    if (active)
-                        unsafeAtomicAdd(&A.J[(a0 + mu) * n + b0 + nu], 2.0 * f * s);
-                }
-#pragma unroll
-            for (int kl = 0; kl < SQ; kl++)
-            {
-                const double s = waveSum(2.0 * f * jcd[kl]);
-                if (lane == 0)
-                    unsafeAtomicAdd(&A.J[(c0 + kl / SD) * n + d0 + kl % SD], s);
-            }
-            if (active)
-            {
-                // K_ac += f sum_{nu la} I D_bd ; K_bc += f sum_{mu la} I D_ad ;
-                // K_ad += f sum_{nu ka} I D_bc ; K_bd += f sum_{mu ka} I D_ac
-#pragma unroll
-                for (int mu = 0; mu < SA; mu++)
-#pragma unroll
-                    for (int ka = 0; ka < SC; ka++)
-                    {
-                        double s1 = 0.0, s3 = 0.0;
-#pragma unroll
-                        for (int nu = 0; nu < SB; nu++)
-#pragma unroll
-                            for (int la = 0; la < SD; la++)
-                                s1 = fma(I[(mu * SB + nu) * SQ + ka * SD + la],
-                                         D[(b0 + nu) * n + d0 + la], s1);
-                        unsafeAtomicAdd(&A.K[(a0 + mu) * n + c0 + ka], f * s1);
-                        (void)s3;
-                    }
-#pragma unroll
-                for (int nu = 0; nu < SB; nu++)
-#pragma unroll
-                    for (int ka = 0; ka < SC; ka++)
-                    {
-                        double s = 0.0;
-#pragma unroll
-                        for (int mu = 0; mu < SA; mu++)
-#pragma unroll
-                            for (int la = 0; la < SD; la++)
-                                s = fma(I[(mu * SB + nu) * SQ + ka * SD + la],
-                                        D[(a0 + mu) * n + d0 + la], s);
-                        unsafeAtomicAdd(&A.K[(b0 + nu) * n + c0 + ka], f * s);
-                    }
-#pragma unroll
-                for (int mu = 0; mu < SA; mu++)
-#pragma unroll
-                    for (int la = 0; la < SD; la++)
-                    {
-                        double s = 0.0;
-#pragma unroll
-                        for (int nu = 0; nu < SB; nu++)
-#pragma unroll
-                            for (int ka = 0; ka < SC; ka++)
-                                s = fma(I[(mu * SB + nu) * SQ + ka * SD + la],
-                                        D[(b0 + nu) * n + c0 + ka], s);
-                        unsafeAtomicAdd(&A.K[(a0 + mu) * n + d0 + la], f * s);
-                    }
-#pragma unroll
-                for (int nu = 0; nu < SB; nu++)
-#pragma unroll
-                    for (int la = 0; la < SD; la++)
-                    {
-                        double s = 0.0;
-#pragma unroll
-                        for (int mu = 0; mu < SA; mu++)
-#pragma unroll
-                            for (int ka = 0; ka < SC; ka++)
-                                s = fma(I[(mu * SB + nu) * SQ + ka * SD + la],
-                                        D[(a0 + mu) * n + c0 + ka], s);
-                        unsafeAtomicAdd(&A.K[(b0 + nu) * n + d0 + la], f * s);
-                    }
-            }
         }
-        else if constexpr (MODE == MODE_DENSE)
-        {
-            // unrolled so that I stays in registers
-            if (active)
+        const long long n = A.nbf;
+        const double *D = A.D;
+        double jl[SL];
 #pragma unroll
-                for (int e = 0; e < SP * SQ; e++)
-                {
-                    const double v = I[e];
-                    denseScatter<C>(A, [v](int, int) { return v; }, a0, b0, c0, d0, e, e + 1, 1);
-                }
-        }
-        else
+        for (int s = 0; s < SL; s++)
+            jl[s] = 0.0;
+        double dmax = 0.0;
+
+        // passes stay a runtime loop: unrolling them lets the compiler overlap the live ranges
+        // of successive passes and spill
+#pragma unroll 1
+        for (int pass = 0; pass < NPASS; pass++)
         {
-            // (ab|ab): the Schwarz factor and the diagonal (mn|mn)
-            static_assert(MODE != MODE_DIAG || (LA == LC && LB == LD), "diag needs ab == cd");
-            if (active)
+            double I[SL * NC];
+#pragma unroll
+            for (int e = 0; e < SL * NC; e++)
+                I[e] = 0.0;
+
+            for (int i = 0; i < lcnt; i++)
             {
-                double m = 0.0;
+                const double4 Pl = LP.pp[lpoff + i];
+                double X[HLn * NC];
 #pragma unroll
-                for (int mn = 0; mn < SP; mn++)
-                    m = fmax(m, fabs(I[mn * SQ + (mn < SQ ? mn : 0)]));
-                A.qout[ab] = sqrt(m);
-                if (A.dout)
+                for (int e = 0; e < HLn * NC; e++)
+                    X[e] = 0.0;
+                for (int k = 0; k < ucnt; k++)
                 {
-                    constexpr int SB = C::SB;
+                    const double4 Pu = UP.pp[upoff + k];
+                    const double4 &Pb = SWAP ? Pu : Pl;
+                    const double4 &Pk = SWAP ? Pl : Pu;
+                    const double pq = Pb.x * Pk.x, ps = Pb.x + Pk.x;
+                    const double alpha = pq / ps;
+                    const double X_ = Pb.y - Pk.y, Y_ = Pb.z - Pk.z, Z_ = Pb.w - Pk.w;
+                    const double T = alpha * (X_ * X_ + Y_ * Y_ + Z_ * Z_);
+                    double R[HL + L];
+                    boysF<L>(T, A.boys, R);
+                    double sc = kTwoPi52 / (pq * sqrt(ps));
 #pragma unroll
-                    for (int mn = 0; mn < SP; mn++)
+                    for (int m = 0; m <= L; m++)
                     {
-                        const int mu = mn / SB, nu = mn % SB;
-                        const double v = I[mn * SQ + (mn < SQ ? mn : 0)];
-                        if (A.kind == 2)
-                            A.dout[a0 + mu] = v;
-                        else
+                        R[m] *= sc;
+                        sc *= -2.0 * alpha;
+                    }
+                    hermiteR<L>(R, X_, Y_, Z_);
+                    const double *__restrict__ ek = eU + static_cast<size_t>(k) * HU * SU;
+                    constexpr RSumTab<L, HLn, HU> RS{};
+#pragma unroll
+                    for (int h = 0; h < HLn; h++)
+#pragma unroll
+                        for (int g = 0; g < HU; g++)
                         {
-                            A.dout[(a0 + mu) * A.nbf + b0 + nu] = v;
-                            A.dout[(b0 + nu) * A.nbf + a0 + mu] = v;
+                            const double r = R[RS.v[h * HU + g]];
+#pragma unroll
+                            for (int c = 0; c < NC; c++)
+                                if (pass * NC + c < SU)
+                                    X[h * NC + c] = fma(r, ek[g * SU + pass * NC + c], X[h * NC + c]);
+                        }
+                }
+                const double *el = eL + static_cast<size_t>(i) * HLn * SL;
+#pragma unroll
+                for (int h = 0; h < HLn; h++)
+#pragma unroll
+                    for (int s = 0; s < SL; s++)
+                    {
+                        const double e = el[h * SL + s];
+#pragma unroll
+                        for (int c = 0; c < NC; c++)
+                            I[s * NC + c] = fma(e, X[h * NC + c], I[s * NC + c]);
+                    }
+            }
+
+            // ---- epilogue for the columns of this pass: I[s][c] = (l s | u pass*NC + c)
+            if constexpr (MODE == MODE_JK)
+            {
+                // J_l += 2f I D_u (accumulated over passes); J_u += 2f I^T D_l (wave sum)
+#pragma unroll
+                for (int c = 0; c < NC; c++)
+                {
+                    const int su = pass * NC + c;
+                    if (su >= SU)
+                        continue;
+                    const int ga = su / SUb, de = su % SUb;
+                    const double du = D[(u0 + ga) * n + u1 + de];
+                    double ju = 0.0;
+#pragma unroll
+                    for (int s = 0; s < SL; s++)
+                    {
+                        const int al = s / SLb, be = s % SLb;
+                        jl[s] = fma(I[s * NC + c], du, jl[s]);
+                        ju = fma(I[s * NC + c], active ? D[(l0 + al) * n + l1 + be] : 0.0, ju);
+                    }
+                    ju = waveSum(2.0 * f * ju);
+                    if (lane == 0)
+                        unsafeAtomicAdd(&A.J[(u0 + ga) * n + u1 + de], ju);
+                }
+                if (active)
+                {
+                    // K[l0+al][u0+ga] += f sum_{be, de} I D[l1+be][u1+de]
+                    // K[l1+be][u0+ga] += f sum_{al, de} I D[l0+al][u1+de]
+                    // K[l0+al][u1+de] += f sum_{be, ga} I D[l1+be][u0+ga]
+                    // K[l1+be][u1+de] += f sum_{al, ga} I D[l0+al][u0+ga]
+#pragma unroll
+                    for (int ga = 0; ga < SUa; ga++)
+                    {
+                        // columns of this pass with first index ga
+                        bool any = false;
+#pragma unroll
+                        for (int de = 0; de < SUb; de++)
+                            any |= (ga * SUb + de) / NC == pass;
+                        if (!any)
+                            continue;
+#pragma unroll
+                        for (int al = 0; al < SLa; al++)
+                        {
+                            double s = 0.0;
+#pragma unroll
+                            for (int be = 0; be < SLb; be++)
+#pragma unroll
+                                for (int de = 0; de < SUb; de++)
+                                {
+                                    const int su = ga * SUb + de;
+                                    if (su / NC != pass)
+                                        continue;
+                                    s = fma(I[(al * SLb + be) * NC + su % NC],
+                                            D[(l1 + be) * n + u1 + de], s);
+                                }
+                            unsafeAtomicAdd(&A.K[(l0 + al) * n + u0 + ga], f * s);
+                        }
+#pragma unroll
+                        for (int be = 0; be < SLb; be++)
+                        {
+                            double s = 0.0;
+#pragma unroll
+                            for (int al = 0; al < SLa; al++)
+#pragma unroll
+                                for (int de = 0; de < SUb; de++)
+                                {
+                                    const int su = ga * SUb + de;
+                                    if (su / NC != pass)
+                                        continue;
+                                    s = fma(I[(al * SLb + be) * NC + su % NC],
+                                            D[(l0 + al) * n + u1 + de], s);
+                                }
+                            unsafeAtomicAdd(&A.K[(l1 + be) * n + u0 + ga], f * s);
+                        }
+                    }
+#pragma unroll
+                    for (int de = 0; de < SUb; de++)
+                    {
+                        bool any = false;
+#pragma unroll
+                        for (int ga = 0; ga < SUa; ga++)
+                            any |= (ga * SUb + de) / NC == pass;
+                        if (!any)
+                            continue;
+#pragma unroll
+                        for (int al = 0; al < SLa; al++)
+                        {
+                            double s = 0.0;
+#pragma unroll
+                            for (int be = 0; be < SLb; be++)
+#pragma unroll
+                                for (int ga = 0; ga < SUa; ga++)
+                                {
+                                    const int su = ga * SUb + de;
+                                    if (su / NC != pass)
+                                        continue;
+                                    s = fma(I[(al * SLb + be) * NC + su % NC],
+                                            D[(l1 + be) * n + u0 + ga], s);
+                                }
+                            unsafeAtomicAdd(&A.K[(l0 + al) * n + u1 + de], f * s);
+                        }
+#pragma unroll
+                        for (int be = 0; be < SLb; be++)
+                        {
+                            double s = 0.0;
+#pragma unroll
+                            for (int al = 0; al < SLa; al++)
+#pragma unroll
+                                for (int ga = 0; ga < SUa; ga++)
+                                {
+                                    const int su = ga * SUb + de;
+                                    if (su / NC != pass)
+                                        continue;
+                                    s = fma(I[(al * SLb + be) * NC + su % NC],
+                                            D[(l0 + al) * n + u0 + ga], s);
+                                }
+                            unsafeAtomicAdd(&A.K[(l1 + be) * n + u1 + de], f * s);
                         }
                     }
                 }
             }
+            else if constexpr (MODE == MODE_DENSE)
+            {
+                static_assert(!SWAP, "dense output keeps the bra on the lanes");
+                if (active)
+#pragma unroll
+                    for (int s = 0; s < SL; s++)
+#pragma unroll
+                        for (int c = 0; c < NC; c++)
+                            if (pass * NC + c < SU)
+                                denseStore<C>(A, I[s * NC + c], s, pass * NC + c, l0, l1, u0, u1);
+            }
+            else
+            {
+                static_assert(MODE != MODE_DIAG || (LA == LC && LB == LD && !SWAP),
+                              "diagonal needs ab == cd");
+                if (active)
+#pragma unroll
+                    for (int c = 0; c < NC; c++)
+                    {
+                        const int s = pass * NC + c;
+                        if (s >= SU)
+                            continue;
+                        const double v = I[s * NC + c];
+                        dmax = fmax(dmax, fabs(v));
+                        if (A.dout)
+                        {
+                            const int mu = s / SLb, nu = s % SLb;
+                            if (A.kind == 2)
+                                A.dout[l0 + mu] = v;
+                            else
+                            {
+                                A.dout[(l0 + mu) * A.nbf + l1 + nu] = v;
+                                A.dout[(l1 + nu) * A.nbf + l0 + mu] = v;
+                            }
+                        }
+                    }
+            }
+        }
+
+        if constexpr (MODE == MODE_JK)
+        {
+            if (active)
+#pragma unroll
+                for (int s = 0; s < SL; s++)
+                    unsafeAtomicAdd(&A.J[(l0 + s / SLb) * n + l1 + s % SLb], 2.0 * f * jl[s]);
+        }
+        else if constexpr (MODE == MODE_DIAG)
+        {
+            if (active)
+                A.qout[l] = sqrt(dmax);
         }
     }
 
-    template <int LA, int LB, int LC, int LD, int MODE>
+    // Each wave takes a contiguous range of tasks (so consecutive tasks share the uniform pair
+    // and the task -> q search runs once per wave).
+    template <int LA, int LB, int LC, int LD, int MODE, bool SWAP>
     __global__ __launch_bounds__(256) void quartetV(const KArgs A)
     {
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        for (long long t = static_cast<long long>(blockIdx.x) * 4 + wave; t < A.ntasks;
-             t += static_cast<long long>(gridDim.x) * 4)
-            quartetVTask<LA, LB, LC, LD, MODE>(A, t, lane);
+        const long long nw = static_cast<long long>(gridDim.x) * 4;
+        const long long w = static_cast<long long>(blockIdx.x) * 4 + wave;
+        const long long t0 = A.ntasks * w / nw, t1 = A.ntasks * (w + 1) / nw;
+        if (t0 >= t1)
+            return;
+        int q = uniform(findKet(A.prefix, A.nket, t0));
+        long long base = A.prefix[q], next = A.prefix[q + 1];
+        for (long long t = t0; t < t1; t++)
+        {
+            while (t >= next)
+            {
+                q++;
+                base = next;
+                next = A.prefix[q + 1];
+            }
+            quartetVTask<LA, LB, LC, LD, MODE, SWAP>(A, q, static_cast<int>(t - base), lane);
+        }
     }
 
     // ------------------------------------------------------------------------------- quartetM
@@ -389,10 +484,22 @@ namespace lbx
             (&sR[0][0][0])[e] = 0.0;
         __syncthreads();
 
-        for (long long t = blockIdx.x; t < A.ntasks; t += gridDim.x)
+        // contiguous task range per workgroup
+        const long long t0 = A.ntasks * blockIdx.x / gridDim.x;
+        const long long t1 = A.ntasks * (blockIdx.x + 1) / gridDim.x;
+        if (t0 >= t1)
+            return;
+        int q = uniform(findKet(A.prefix, A.nket, t0));
+        long long base = A.prefix[q], next = A.prefix[q + 1];
+        for (long long t = t0; t < t1; t++)
         {
-            const int q = uniform(findKet(A.prefix, A.nket, t));
-            const int p = uniform(A.plo[q] + static_cast<int>(t - A.prefix[q]));
+            while (t >= next)
+            {
+                q++;
+                base = next;
+                next = A.prefix[q + 1];
+            }
+            const int p = uniform(A.plo[q] + static_cast<int>(t - base));
             const int cd = uniform(A.ket.order[q]);
             const int ab = uniform(A.bra.order[p]);
             const int kpoff = uniform(A.ket.poff[cd]), kcnt = uniform(A.ket.pcnt[cd]);
@@ -434,9 +541,9 @@ namespace lbx
                         boysF<L>(T, A.boys, F);
                         double sc = kTwoPi52 / (pq * sqrt(ps));
 #pragma unroll
-                        for (int n = 0; n <= L; n++)
+                        for (int m = 0; m <= L; m++)
                         {
-                            sInit[lane][n] = F[n] * sc;
+                            sInit[lane][m] = F[m] * sc;
                             sc *= -2.0 * alpha;
                         }
                         sPQ[lane][0] = X_;
@@ -623,8 +730,8 @@ namespace lbx
             }
             else if constexpr (MODE == MODE_DENSE)
             {
-                denseScatter<C>(A, [&](int mn, int kl) { return sI[mn * SQ + kl]; }, a0, b0, c0,
-                                d0, lane, SP * SQ, 64);
+                for (int e = lane; e < SP * SQ; e += 64)
+                    denseStore<C>(A, sI[e], e / SQ, e % SQ, a0, b0, c0, d0);
             }
             else
             {
@@ -653,30 +760,53 @@ namespace lbx
         }
     }
 
-    template <int LA, int LB, int LC, int LD, int MODE>
-    void launchQuartet(const KArgs &A, hipStream_t s)
+    template <int LA, int LB, int LC, int LD, int MODE, bool SWAP>
+    void launchValu(const KArgs &A, hipStream_t s)
     {
-        using C = Cls<LA, LB, LC, LD>;
         if (A.ntasks <= 0)
             return;
-        if constexpr (C::kValu)
-        {
-            long long wg = (A.ntasks + 3) / 4;
-            int grid = static_cast<int>(wg < 65536 ? wg : 65536);
-            hipLaunchKernelGGL((quartetV<LA, LB, LC, LD, MODE>), dim3(grid), dim3(256), 0, s, A);
-        }
-        else
-        {
-            int grid = static_cast<int>(A.ntasks < 131072 ? A.ntasks : 131072);
-            hipLaunchKernelGGL((quartetM<LA, LB, LC, LD, MODE>), dim3(grid), dim3(64), 0, s, A);
-        }
+        // contiguous task ranges per wave; up to 8192 workgroups of 4 waves
+        long long wg = (A.ntasks + 3) / 4;
+        int grid = static_cast<int>(wg < 8192 ? wg : 8192);
+        hipLaunchKernelGGL((quartetV<LA, LB, LC, LD, MODE, SWAP>), dim3(grid), dim3(256), 0, s, A);
     }
 
     template <int LA, int LB, int LC, int LD, int MODE>
+    void launchMfma(const KArgs &A, hipStream_t s)
+    {
+        if (A.ntasks <= 0)
+            return;
+        int grid = static_cast<int>(A.ntasks < 16384 ? A.ntasks : 16384);
+        hipLaunchKernelGGL((quartetM<LA, LB, LC, LD, MODE>), dim3(grid), dim3(64), 0, s, A);
+    }
+
+    // Registers every variant the class supports in this mode.  The ket-on-lanes VALU form
+    // only serves J/K between different classes (dense and diagonal outputs keep the bra on
+    // the lanes; for one class both forms coincide).  The MFMA form is always available.
+    template <int LA, int LB, int LC, int LD, int MODE>
     void registerQuartet()
     {
-        KernelSlot &s = kernelSlot(MODE, LA, LB, LC, LD);
-        s.fn = &launchQuartet<LA, LB, LC, LD, MODE>;
-        s.width = Cls<LA, LB, LC, LD>::kValu ? 64 : 1;
+        using C = Cls<LA, LB, LC, LD>;
+        if constexpr (C::NC0 > 0)
+        {
+            KernelSlot &s = kernelSlot(MODE, LA, LB, LC, LD, VAR_VALU_BRA);
+            s.fn = &launchValu<LA, LB, LC, LD, MODE, false>;
+            s.width = 64;
+            s.swap = false;
+            s.nc = C::NC0;
+        }
+        constexpr bool same = LA == LC && LB == LD;
+        if constexpr (MODE == MODE_JK && C::NC1 > 0 && !same)
+        {
+            KernelSlot &s = kernelSlot(MODE, LA, LB, LC, LD, VAR_VALU_KET);
+            s.fn = &launchValu<LA, LB, LC, LD, MODE, true>;
+            s.width = 64;
+            s.swap = true;
+            s.nc = C::NC1;
+        }
+        KernelSlot &s = kernelSlot(MODE, LA, LB, LC, LD, VAR_MFMA);
+        s.fn = &launchMfma<LA, LB, LC, LD, MODE>;
+        s.width = 1;
+        s.swap = false;
     }
 }

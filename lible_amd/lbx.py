@@ -20,7 +20,9 @@ LBX_MIXED = 1
 # every symbol include/lbx.h declares
 SYMBOLS = ["lbx_create", "lbx_destroy", "lbx_last_error", "lbx_dims", "lbx_set_stream",
            "lbx_synchronize", "lbx_boys", "lbx_eri4", "lbx_eri4_diag", "lbx_eri3", "lbx_eri2",
-           "lbx_eri2_diag", "lbx_jk", "lbx_profile_enable", "lbx_profile_read"]
+           "lbx_eri2_diag", "lbx_jk", "lbx_tune", "lbx_set_variant", "lbx_profile_enable",
+           "lbx_profile_read"]
+VARIANTS = ("valu-bra", "valu-ket", "mfma")
 
 
 class LbxError(RuntimeError):
@@ -70,6 +72,8 @@ def lib():
         for name in ("lbx_eri4", "lbx_eri4_diag", "lbx_eri3", "lbx_eri2", "lbx_eri2_diag"):
             getattr(L, name).argtypes = [p, p]
         L.lbx_jk.argtypes = [p, p, p, p, d, i, d, ctypes.POINTER(JKStats)]
+        L.lbx_tune.argtypes = [p, p, d]
+        L.lbx_set_variant.argtypes = [p, i, i, i, i, i]
         L.lbx_profile_enable.argtypes = [p, i]
         L.lbx_profile_read.argtypes = [p, ctypes.POINTER(ClassProfile), i, ctypes.POINTER(i)]
         _LIB = L
@@ -180,6 +184,17 @@ class Context:
                             mixed_thresh, ctypes.byref(st)))
         return J, K, {"quartets": st.quartets, "flops": st.flops, "quartets_fp32": st.quartets_fp32}
 
+    def tune(self, D, schwarz: float = 1e-12):
+        """Times every kernel variant of every class pair on this density and keeps the
+        fastest (lbx_tune)."""
+        torch = _torch()
+        D = D.to(device=self.device, dtype=torch.float64).contiguous()
+        _check(lib().lbx_tune(self.h, D.data_ptr(), schwarz))
+
+    def set_variant(self, cls: tuple, variant: int) -> bool:
+        """Fix the kernel variant of a class pair; False if it has no such variant."""
+        return lib().lbx_set_variant(self.h, *cls, variant) == 0
+
     def profile(self, on: bool):
         _check(lib().lbx_profile_enable(self.h, int(on)))
 
@@ -188,6 +203,6 @@ class Context:
         _check(lib().lbx_profile_read(self.h, None, 0, ctypes.byref(n)))
         buf = (ClassProfile * max(n.value, 1))()
         _check(lib().lbx_profile_read(self.h, buf, n.value, ctypes.byref(n)))
-        return [{"class": (r.la, r.lb, r.lc, r.ld), "kernel": "valu" if r.kernel == 0 else "mfma",
+        return [{"class": (r.la, r.lb, r.lc, r.ld), "kernel": VARIANTS[r.kernel],
                  "calls": r.calls, "ms": r.ms, "quartets": r.quartets, "flops": r.flops}
                 for r in buf[:n.value]]

@@ -115,6 +115,44 @@ def test_eri2_diag(lbx, oracle):
     assert abs(np.abs(got).sum() - 830.788976485982) <= REL * 830.79     # :149
 
 
+CLASS_PAIRS = [(la, lb, lc, ld) for la in range(4) for lb in range(la + 1)
+               for lc in range(4) for ld in range(lc + 1)
+               if la * (la + 1) // 2 + lb >= lc * (lc + 1) // 2 + ld]
+
+
+@pytest.fixture(scope="module")
+def jk_case(oracle):
+    m = B.synthetic_molecule(2024, 4, 5.0, B.RECIPE_MAIN)
+    D = B.density_from_orbitals(5, m.shells.nbf, 20, 0.1)
+    Jr, Kr, sr = oracle.Structure(m).jk(D, thresh=1e-12)
+    return m, D, Jr, Kr, sr
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2])
+def test_jk_every_variant(lbx, jk_case, variant):
+    """Every kernel variant of every class pair (where it exists) gives the oracle's J and K."""
+    import torch
+    m, D, Jr, Kr, sr = jk_case
+    ctx = lbx.Context(m)
+    used = [c for c in CLASS_PAIRS if ctx.set_variant(c, variant)]
+    assert variant == 1 or len(used) > 0
+    J, K, st = ctx.jk(torch.tensor(D, device="cuda"), schwarz=1e-12)
+    close(J.cpu().numpy(), Jr, floor=1e-13)
+    close(K.cpu().numpy(), Kr, floor=1e-13)
+    assert st["quartets"] == sr["quartets"]
+
+
+def test_jk_tuned(lbx, jk_case):
+    import torch
+    m, D, Jr, Kr, _ = jk_case
+    ctx = lbx.Context(m)
+    Dt = torch.tensor(D, device="cuda")
+    ctx.tune(Dt, schwarz=1e-12)
+    J, K, _ = ctx.jk(Dt, schwarz=1e-12)
+    close(J.cpu().numpy(), Jr, floor=1e-13)
+    close(K.cpu().numpy(), Kr, floor=1e-13)
+
+
 @pytest.mark.parametrize("thresh", [0.0, 1e-12])
 def test_jk_small(lbx, oracle, thresh):
     import torch
