@@ -1,0 +1,142 @@
+// lible::ints — the C++ interface of the reference's two-electron-integral path
+// (src/lible/ints/ints.hpp:165-179, shell.hpp:10-34, structure.hpp:12-164), implemented over the
+// lbx C-ABI (include/lbx.h) on the current HIP device.  A caller of the reference's eri4 / eri3 /
+// eri2 / eri4Diagonal / eri2Diagonal keeps its code: same types, same dense row-major output
+// layout (AO order of the shells, spherical m order 0, +1, -1, +2, -2, ...).  Added: the
+// Schwarz-screened Coulomb/exchange build coulombExchange().
+#pragma once
+
+#include <array>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../vectormd.hpp"
+
+struct lbx_context;
+
+namespace lible::ints
+{
+    /// Angular momentum, exponents and raw contraction coefficients of one shell
+    /// (shell.hpp:10-19).
+    struct BasisShell
+    {
+        int l_{};
+        std::vector<double> exps_;
+        std::vector<double> coeffs_;
+    };
+
+    using basis_shells_t = std::vector<BasisShell>;
+
+    /// Atomic number and the shells of one atom (shell.hpp:24-34).
+    struct BasisAtom
+    {
+        int atomic_nr_{};
+        basis_shells_t basis_shells_;
+    };
+
+    using basis_atoms_t = std::vector<BasisAtom>;
+
+    /// Bohr <-> angstrom (defs.hpp:9-12).
+    constexpr double _bohr_to_ang_ = 0.5291772083;
+    constexpr double _ang_to_bohr_ = 1.0 / _bohr_to_ang_;
+
+    /// Directories holding the basis-set text files (ints.hpp:333-362).  Files are found as
+    /// <dir>/**/<name>.txt with the name lower-cased.
+    class BasisPaths
+    {
+    public:
+        static std::string getMainBasisSetsPath();
+        static std::string getAuxBasisSetsPath();
+        static void setMainBasisSetsPath(const std::string &path);
+        static void setAuxBasisSetsPath(const std::string &path);
+    };
+
+    /// Shells of one element from a named basis (basis_sets.cpp:214-304).
+    BasisAtom basisForAtom(int atomic_nr, const std::string &basis_set);
+    BasisAtom basisForAtomAux(int atomic_nr, const std::string &aux_basis_set);
+    basis_atoms_t basisForAtoms(const std::vector<int> &atomic_nrs, const std::string &basis_set);
+    basis_atoms_t basisForAtomsAux(const std::vector<int> &atomic_nrs,
+                                   const std::string &aux_basis_set);
+
+    /// Atoms, coordinates and basis (structure.hpp:12-164).  Coordinates are given in angstrom
+    /// and stored in bohr.  The device-side data (normalised shells, pair lists, E matrices,
+    /// Schwarz factors) is built on first use and shared by copies of the Structure.
+    class Structure
+    {
+    public:
+        Structure() = default;
+        Structure(const std::string &basis_set, const std::vector<int> &atomic_nrs,
+                  const std::vector<std::array<double, 3>> &coords_angstrom);
+        Structure(const std::string &basis_set, const std::string &basis_set_aux,
+                  const std::vector<int> &atomic_nrs,
+                  const std::vector<std::array<double, 3>> &coords_angstrom);
+        Structure(const basis_atoms_t &basis_set, const std::vector<int> &atomic_nrs,
+                  const std::vector<std::array<double, 3>> &coords_angstrom);
+        Structure(const basis_atoms_t &basis_set, const basis_atoms_t &basis_set_aux,
+                  const std::vector<int> &atomic_nrs,
+                  const std::vector<std::array<double, 3>> &coords_angstrom);
+
+        /// Ghost atoms carry basis functions and no nucleus; their shells follow those of the
+        /// real atoms in AO order (structure.cpp:117-243, shell.cpp:137-194).
+        Structure(const std::string &basis_set, const std::string &basis_set_ghost,
+                  const std::vector<int> &atomic_nrs, const std::vector<int> &atomic_nrs_ghost,
+                  const std::vector<std::array<double, 3>> &coords_angstrom,
+                  const std::vector<std::array<double, 3>> &coords_angstrom_ghost);
+        Structure(const std::string &basis_set, const std::string &basis_set_ghost,
+                  const std::string &basis_set_aux, const std::string &basis_set_aux_ghost,
+                  const std::vector<int> &atomic_nrs, const std::vector<int> &atomic_nrs_ghost,
+                  const std::vector<std::array<double, 3>> &coords_angstrom,
+                  const std::vector<std::array<double, 3>> &coords_angstrom_ghost);
+        Structure(const basis_atoms_t &basis_set, const basis_atoms_t &basis_set_ghost,
+                  const std::vector<int> &atomic_nrs, const std::vector<int> &atomic_nrs_ghost,
+                  const std::vector<std::array<double, 3>> &coords_angstrom,
+                  const std::vector<std::array<double, 3>> &coords_angstrom_ghost);
+        Structure(const basis_atoms_t &basis_set, const basis_atoms_t &basis_set_ghost,
+                  const basis_atoms_t &basis_set_aux, const basis_atoms_t &basis_set_ghost_aux,
+                  const std::vector<int> &atomic_nrs, const std::vector<int> &atomic_nrs_ghost,
+                  const std::vector<std::array<double, 3>> &coords_angstrom,
+                  const std::vector<std::array<double, 3>> &coords_angstrom_ghost);
+
+        bool getUseRI() const { return use_ri_; }
+        int getMaxL() const { return max_l_; }
+        int getMaxLAux() const { return max_l_aux_; }
+        size_t getDimAO() const { return dim_ao_; }
+        size_t getDimAOAux() const { return dim_ao_aux_; }
+        size_t getNAtoms() const { return coords_.size(); }
+        int getZ(size_t iatom) const { return atomic_nrs_.at(iatom); }
+        std::array<double, 3> getCoordsAtom(size_t iatom) const { return coords_.at(iatom); }
+
+        /// The lbx context of this structure (created on the current device on first use).
+        lbx_context *context() const;
+
+    private:
+        void init(basis_atoms_t main, const basis_atoms_t *aux);
+
+        bool use_ri_ = false;
+        int max_l_ = 0, max_l_aux_ = 0;
+        size_t dim_ao_ = 0, dim_ao_aux_ = 0;
+        std::vector<int> atomic_nrs_, atomic_nrs_ghost_;
+        std::vector<std::array<double, 3>> coords_, coords_ghost_;   // bohr
+        basis_atoms_t basis_, basis_aux_;                  // real atoms, then ghost atoms
+        mutable std::shared_ptr<lbx_context> ctx_;
+    };
+
+    /// (mn|kl), dense nbf^4 (eri4.cpp:38-95).
+    vec4d eri4(const Structure &structure);
+    /// (mn|mn), nbf x nbf (eri4.cpp:152-174).
+    vec2d eri4Diagonal(const Structure &structure);
+    /// (mn|P), nbf x nbf x naux (eri3.cpp:7-46).
+    vec3d eri3(const Structure &structure);
+    /// (P|Q), naux x naux (eri2.cpp:27-72).
+    vec2d eri2(const Structure &structure);
+    /// (P|P), naux (eri2.cpp:74-96).
+    std::vector<double> eri2Diagonal(const Structure &structure);
+
+    /// J_mn = sum_kl (mn|kl) D_kl and K_mk = sum_nl (mn|kl) D_nl over Schwarz-screened unique
+    /// quartets (no reference counterpart).  D symmetric nbf x nbf.
+    std::pair<vec2d, vec2d> coulombExchange(const Structure &structure, const vec2d &D,
+                                            double schwarz_threshold = 1e-12);
+}

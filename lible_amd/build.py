@@ -15,16 +15,20 @@ import sys
 import time
 
 HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 OUT = os.path.join(HERE, "liblbx.so")
+INCLUDE = os.path.join(ROOT, "include")
+SELFTEST_SRC = os.path.join(ROOT, "tests", "cpp", "ints_selftest.cpp")
+SELFTEST = os.path.join(HERE, "bin", "ints_selftest")
 
 N_UNITS = 12
 
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950", "-mcode-object-version=5",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-munsafe-fp-atomics", "-ffp-contract=fast",
+    "-munsafe-fp-atomics", "-ffp-contract=fast", f"-I{INCLUDE}",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]
 
@@ -70,6 +74,7 @@ def build(jobs: int | None = None, verbose: bool = True) -> str:
         (os.path.join(CSRC, "host_setup.cpp"), os.path.join(BUILD, "host_setup.o"), []),
         (os.path.join(CSRC, "ecoef.hip"), os.path.join(BUILD, "ecoef.o"), []),
         (os.path.join(CSRC, "capi.hip"), os.path.join(BUILD, "capi.o"), []),
+        (os.path.join(CSRC, "ints_api.cpp"), os.path.join(BUILD, "ints_api.o"), []),
     ]
     for u in range(N_UNITS):
         work.append((os.path.join(CSRC, "instances.hip"), os.path.join(BUILD, f"inst_{u}.o"),
@@ -90,9 +95,22 @@ def build(jobs: int | None = None, verbose: bool = True) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, OUT)
+    build_selftest()
     if verbose:
-        print(f"built {OUT} in {time.time() - t0:.1f}s", flush=True)
+        print(f"built {OUT} and {SELFTEST} in {time.time() - t0:.1f}s", flush=True)
     return OUT
+
+
+def build_selftest() -> str:
+    """The C++ known-answer program of the lible::ints interface (tests/cpp/ints_selftest.cpp),
+    a plain host program linked against liblbx.so."""
+    os.makedirs(os.path.dirname(SELFTEST), exist_ok=True)
+    cmd = ["g++", "-O2", "-std=c++17", f"-I{INCLUDE}", SELFTEST_SRC, "-o", SELFTEST,
+           f"-L{HERE}", "-l:liblbx.so", "-Wl,-rpath,$ORIGIN/.."]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"selftest build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return SELFTEST
 
 
 def main(argv=None):

@@ -1,0 +1,220 @@
+// Known-answer checks of the lible::ints C++ interface, written like the reference's own
+// test/tests_ints.cpp: Structures from named basis sets, the integral functions, and the sum of
+// |values| against the reference's answers (tests_ints.cpp:143-166, 694-760).
+//
+//   ints_selftest <basis_dir> cpu   basis reading, Structure bookkeeping, VectorMD (no device)
+//   ints_selftest <basis_dir> gpu   integrals on the current HIP device
+//
+// Prints one "PASS <name>" / "FAIL <name>: ..." line per check; exit status = number of failures.
+#include <lible/ints/ints.hpp>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <string>
+
+namespace lints = lible::ints;
+using lible::vec2d;
+using lible::vec3d;
+using lible::vec4d;
+
+namespace
+{
+    int failures = 0;
+
+    void report(const std::string &name, bool ok, const std::string &detail = "")
+    {
+        std::printf("%s %s%s%s\n", ok ? "PASS" : "FAIL", name.c_str(), detail.empty() ? "" : ": ",
+                    detail.c_str());
+        std::fflush(stdout);
+        failures += ok ? 0 : 1;
+    }
+
+    void guarded(const std::string &name, const std::function<void()> &fn)
+    {
+        try
+        {
+            fn();
+        }
+        catch (const std::exception &e)
+        {
+            report(name, false, std::string("exception: ") + e.what());
+        }
+    }
+
+    template <typename T>
+    double absSum(const T &v)
+    {
+        double s = 0;
+        for (double x : v)
+            s += std::fabs(x);
+        return s;
+    }
+
+    // The reference compares these sums with an absolute 1e-12 (tests_ints.cpp:15), i.e. to the
+    // last bits of a ~1e4 sum; a different summation order moves them by ~1e-13 relative, so the
+    // bar here is 1e-12 relative.
+    void checkSum(const std::string &name, double got, double want)
+    {
+        char buf[160];
+        std::snprintf(buf, sizeof buf, "sum %.15g, reference %.15g, rel diff %.2e", got, want,
+                      std::fabs(got - want) / want);
+        report(name, std::fabs(got - want) <= 1e-12 * std::fabs(want), buf);
+    }
+
+    const std::vector<int> nrs_h2o{8, 1, 1};
+    const std::vector<std::array<double, 3>> xyz_h2o{
+        {0.0, 0.0, 0.0}, {0.757, 0.586, 0.0}, {-0.757, 0.586, 0.0}};   // tests_ints.cpp:62-73
+    const std::vector<int> nrs_o3{8, 8, 8};
+    const std::vector<std::array<double, 3>> xyz_o3{
+        {0, 0, 0}, {0, 1, 1}, {0, 0, 2}};                               // tests_ints.cpp:115-120
+
+    void cpuChecks()
+    {
+        guarded("vectormd_layout", [] {
+            vec3d v(2, 3, 4);
+            bool ok = v.size() == 24 && v.dim<0>() == 2 && v.dim<1>() == 3 && v.dim<2>() == 4;
+            v(1, 2, 3) = 7.0;
+            ok = ok && v.memptr()[1 * 12 + 2 * 4 + 3] == 7.0;
+            lible::VectorMD<double, 2> f(lible::Fill(2.5), 3, 3);
+            ok = ok && absSum(f) == 22.5;
+            report("vectormd_layout", ok);
+        });
+        guarded("dims_o3_svp", [] {
+            lints::Structure s("def2-svp", nrs_o3, xyz_o3);
+            // O in def2-SVP: 3s 2p 1d = 14 spherical functions
+            report("dims_o3_svp", s.getDimAO() == 42 && s.getMaxL() == 2 && !s.getUseRI() &&
+                                      s.getNAtoms() == 3);
+        });
+        guarded("dims_h2o_ri", [] {
+            lints::Structure s("def2-SVP", "def2-universal-jkfit", nrs_h2o, xyz_h2o);
+            const auto h = lints::basisForAtomAux(1, "def2-universal-jkfit");
+            const auto o = lints::basisForAtomAux(8, "def2-universal-jkfit");
+            size_t naux = 0;
+            for (const auto *a : {&o, &h, &h})
+                for (const auto &sh : a->basis_shells_)
+                    naux += 2 * sh.l_ + 1;
+            report("dims_h2o_ri", s.getDimAO() == 24 && s.getUseRI() && s.getDimAOAux() == naux);
+        });
+        guarded("coords_bohr", [] {
+            lints::Structure s("def2-svp", nrs_o3, xyz_o3);
+            const double z = s.getCoordsAtom(2)[2];
+            report("coords_bohr", std::fabs(z - 2.0 / 0.5291772083) < 1e-14);
+        });
+        guarded("ghost_dims", [] {
+            lints::Structure s("def2-svp", "def2-svp", {8, 1}, {1}, {xyz_h2o[0], xyz_h2o[1]},
+                               {xyz_h2o[2]});
+            report("ghost_dims", s.getDimAO() == 24 && s.getNAtoms() == 2);
+        });
+        guarded("missing_element", [] {
+            bool threw = false;
+            try
+            {
+                lints::basisForAtom(92, "def2-svp");
+            }
+            catch (const std::runtime_error &)
+            {
+                threw = true;
+            }
+            report("missing_element", threw);
+        });
+        guarded("missing_basis", [] {
+            bool threw = false;
+            try
+            {
+                lints::Structure s("no-such-basis", nrs_o3, xyz_o3);
+            }
+            catch (const std::runtime_error &)
+            {
+                threw = true;
+            }
+            report("missing_basis", threw);
+        });
+    }
+
+    void gpuChecks()
+    {
+        guarded("eri4", [] {
+            lints::Structure s("def2-svp", nrs_o3, xyz_o3);
+            vec4d e = lints::eri4(s);
+            checkSum("eri4", absSum(e), 8246.711763196334);   // tests_ints.cpp:748
+            const size_t n = e.dim<0>();
+            double asym = 0;
+            for (size_t a = 0; a < n; a += 5)
+                for (size_t b = 0; b < n; b += 3)
+                    for (size_t c = 0; c < n; c += 2)
+                        for (size_t d = 0; d < n; d++)
+                            asym = std::max({asym, std::fabs(e(a, b, c, d) - e(b, a, c, d)),
+                                             std::fabs(e(a, b, c, d) - e(c, d, a, b))});
+            report("eri4_symmetry", asym <= 1e-14, "max asym " + std::to_string(asym));
+
+            // coulombExchange against the contraction of the dense tensor
+            vec2d D(n, n);
+            for (size_t i = 0; i < n; i++)
+                for (size_t j = 0; j <= i; j++)
+                    D(i, j) = D(j, i) = std::cos(0.37 * i + 0.11 * j) / (1.0 + 0.1 * (i + j));
+            auto [J, K] = lints::coulombExchange(s, D, 0.0);
+            double err = 0, scale = 0;
+            for (size_t m = 0; m < n; m++)
+                for (size_t k = 0; k < n; k++)
+                {
+                    double j = 0, x = 0;
+                    for (size_t p = 0; p < n; p++)
+                        for (size_t q = 0; q < n; q++)
+                        {
+                            j += e(m, k, p, q) * D(p, q);
+                            x += e(m, p, k, q) * D(p, q);
+                        }
+                    err = std::max({err, std::fabs(J(m, k) - j), std::fabs(K(m, k) - x)});
+                    scale = std::max({scale, std::fabs(j), std::fabs(x)});
+                }
+            report("coulomb_exchange", err <= 1e-12 * scale,
+                   "max err " + std::to_string(err / scale) + " of scale");
+        });
+        guarded("eri4_diagonal", [] {
+            lints::Structure s("def2-tzvp", nrs_o3, xyz_o3);
+            checkSum("eri4_diagonal", absSum(lints::eri4Diagonal(s)), 306.806419250122);  // :712
+        });
+        guarded("eri3", [] {
+            lints::Structure s("def2-tzvp", "def2-qzvp-rifit", nrs_o3, xyz_o3);
+            checkSum("eri3", absSum(lints::eri3(s)), 25854.105851585416531);             // :730
+        });
+        guarded("eri2", [] {
+            lints::Structure s("def2-SVP", "def2-qzvp-rifit", nrs_h2o, xyz_h2o);
+            checkSum("eri2", absSum(lints::eri2(s)), 8801.334703460838);                 // :694
+        });
+        guarded("eri2_diagonal", [] {
+            lints::Structure s("def2-SVP", "def2-universal-jkfit", nrs_h2o, xyz_h2o);
+            checkSum("eri2_diagonal", absSum(lints::eri2Diagonal(s)), 830.788976485982); // :149
+        });
+        guarded("ghost_atoms", [] {
+            // Integrals do not see nuclear charges: with the last H a ghost atom, the AO order is
+            // unchanged and the tensor equals the all-real one.
+            lints::Structure full("def2-svp", nrs_h2o, xyz_h2o);
+            lints::Structure ghost("def2-svp", "def2-svp", {8, 1}, {1}, {xyz_h2o[0], xyz_h2o[1]},
+                                   {xyz_h2o[2]});
+            vec4d a = lints::eri4(full), b = lints::eri4(ghost);
+            double d = 0;
+            for (size_t i = 0; i < a.size(); i++)
+                d = std::max(d, std::fabs(a[i] - b[i]));
+            report("ghost_atoms", a.size() == b.size() && d == 0.0);
+        });
+    }
+}
+
+int main(int argc, char **argv)
+{
+    if (argc < 3 || (std::strcmp(argv[2], "cpu") != 0 && std::strcmp(argv[2], "gpu") != 0))
+    {
+        std::fprintf(stderr, "usage: %s <basis_dir> cpu|gpu\n", argv[0]);
+        return 2;
+    }
+    lints::BasisPaths::setMainBasisSetsPath(argv[1]);
+    lints::BasisPaths::setAuxBasisSetsPath(argv[1]);
+    if (std::strcmp(argv[2], "cpu") == 0)
+        cpuChecks();
+    else
+        gpuChecks();
+    return failures;
+}
