@@ -88,15 +88,17 @@ typedef struct lbx_jk_stats {
 int lbx_jk(lbx_context *ctx, const double *D, double *J, double *K, double schwarz_thresh,
            int precision, double mixed_thresh, lbx_jk_stats *stats);
 
-/* Autotuner (no reference counterpart).  Each class pair (la lb|lc ld) has up to three kernel
- * variants: one lane per quartet with the bra pairs on the lanes, the same with the ket pairs
- * on the lanes, and one wave per quartet on FP64 MFMA.  lbx_tune times every variant of every
- * class pair on this density and threshold and keeps the fastest for later lbx_jk calls on
- * the context; without it lbx_jk uses a work estimate from the primitive-pair counts. */
+/* Autotuner (no reference counterpart).  Each class pair (la lb|lc ld) has up to five kernel
+ * variants: 0 one lane per quartet with the bra pairs on the lanes, 1 the same with the ket
+ * pairs on the lanes, 2 one wave per quartet on FP64 MFMA, and 3, 4 the forms 0, 1 with R and
+ * both transforms in FP32 (the small-bound quartets of LBX_MIXED).  lbx_tune times every
+ * variant of every class pair on this density and threshold and keeps the fastest FP64 form
+ * for FP64 work and the fastest of all forms for the FP32 tier of mixed builds; without it
+ * lbx_jk picks by a work estimate from the primitive-pair counts. */
 int lbx_tune(lbx_context *ctx, const double *D, double schwarz_thresh);
-/* Fixes the variant (0, 1, 2 as above) of class pair (la lb|lc ld), la >= lb, lc >= ld, bra
- * class >= ket class in the order (0,0),(1,0),(1,1),(2,0),...; fails if the class pair has no
- * such variant.  variant -1 restores the default. */
+/* Fixes the variant of class pair (la lb|lc ld), la >= lb, lc >= ld, bra class >= ket class in
+ * the order (0,0),(1,0),(1,1),(2,0),...: variants 0-2 for FP64 work, 3-4 for the FP32 tier;
+ * fails if the class pair has no such variant.  variant -1 restores both defaults. */
 int lbx_set_variant(lbx_context *ctx, int la, int lb, int lc, int ld, int variant);
 
 /* Per-class timing of lbx_jk (the reference's only timing facility is eri4Benchmark,
@@ -105,7 +107,7 @@ int lbx_set_variant(lbx_context *ctx, int la, int lb, int lc, int ld, int varian
  * lbx_profile_read returns the accumulated totals since the last enable. */
 typedef struct lbx_class_profile {
     int la, lb, lc, ld;
-    int kernel;            /* 0: lane per quartet, bra on lanes; 1: ket on lanes; 2: MFMA */
+    int kernel;            /* variant: 0 bra on lanes, 1 ket on lanes, 2 MFMA, 3 / 4 FP32 0 / 1 */
     int calls;
     double ms;             /* summed kernel time */
     double quartets;       /* summed quartets */

@@ -23,7 +23,7 @@ INCLUDE = os.path.join(ROOT, "include")
 SELFTEST_SRC = os.path.join(ROOT, "tests", "cpp", "ints_selftest.cpp")
 SELFTEST = os.path.join(HERE, "bin", "ints_selftest")
 
-N_UNITS = 12
+N_UNITS = 22   # csrc/instances.hip: 0..11 FP64 forms and dense outputs, 12..21 FP32 J/K forms
 
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950", "-mcode-object-version=5",
@@ -67,7 +67,20 @@ def _compile(src: str, obj: str, defines: list[str]) -> tuple[str, float]:
     return obj, time.time() - t
 
 
-def build(jobs: int | None = None, verbose: bool = True) -> str:
+def _stale(src: str, obj: str) -> bool:
+    """True unless obj is newer than src, every header and this script."""
+    if not os.path.exists(obj):
+        return True
+    deps = [src, os.path.abspath(__file__)]
+    for d in (CSRC, INCLUDE):
+        for base, _, files in os.walk(d):
+            deps += [os.path.join(base, f) for f in files if f.endswith((".h", ".hpp"))]
+    return os.path.getmtime(obj) < max(os.path.getmtime(p) for p in deps)
+
+
+def build(jobs: int | None = None, verbose: bool = True, incremental: bool = False) -> str:
+    """Compiles every unit and links the library (incremental: only units older than their
+    sources or any header)."""
     os.makedirs(BUILD, exist_ok=True)
     jobs = jobs or max(1, min(os.cpu_count() or 4, 8))
     work = [
@@ -79,11 +92,12 @@ def build(jobs: int | None = None, verbose: bool = True) -> str:
     for u in range(N_UNITS):
         work.append((os.path.join(CSRC, "instances.hip"), os.path.join(BUILD, f"inst_{u}.o"),
                      [f"-DLBX_UNIT={u}"]))
-    # heaviest units first
-    work.sort(key=lambda w: -int(w[2][0].split("=")[1]) if w[2] else 1)
+    # heaviest units (highest bra class) first
+    work.sort(key=lambda w: -(int(w[2][0].split("=")[1]) % 12) if w[2] else 1)
     t0 = time.time()
+    todo = [w for w in work if not incremental or _stale(w[0], w[1])]
     with cf.ThreadPoolExecutor(jobs) as ex:
-        futs = [ex.submit(_compile, *w) for w in work]
+        futs = [ex.submit(_compile, *w) for w in todo]
         for f in cf.as_completed(futs):
             obj, dt = f.result()
             if verbose:
@@ -116,8 +130,10 @@ def build_selftest() -> str:
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("-j", type=int, default=None)
+    ap.add_argument("--incremental", action="store_true",
+                    help="recompile only units older than their sources or any header")
     a = ap.parse_args(argv)
-    build(a.j)
+    build(a.j, incremental=a.incremental)
 
 
 if __name__ == "__main__":

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <limits>
 #include <map>
 #include <memory>
 #include <numeric>
@@ -150,12 +151,30 @@ namespace lbx
             Diagonal    // p == q only
         };
 
+        // first sorted position p with qb[p] * qk < thr (qb descending)
+        int firstBelow(const std::vector<double> &qb, double qk, double thr)
+        {
+            int a = 0, b = static_cast<int>(qb.size());
+            while (a < b)
+            {
+                const int m = (a + b) / 2;
+                if (qb[m] * qk >= thr)
+                    a = m + 1;
+                else
+                    b = m;
+            }
+            return a;
+        }
+
+        constexpr double kNoUpper = std::numeric_limits<double>::infinity();
+
         // Lane-side positions [plo, phi) per uniform-side position q, with the Schwarz cut
-        // Q_p Q_q >= thresh on descending-sorted Q (thresh <= 0: no screening).  `bra` is the
-        // lane side and `ket` the uniform side; lane_is_bra says whether that matches the
-        // canonical (bra class >= ket class) roles, which the FLOP count follows.
+        // thresh <= Q_p Q_q < thresh_hi on descending-sorted Q (thresh <= 0: no lower cut).
+        // `bra` is the lane side and `ket` the uniform side; lane_is_bra says whether that
+        // matches the canonical (bra class >= ket class) roles, which the FLOP count follows.
         TaskTable buildTasks(Context &c, const PairClass &bra, const PairClass &ket, Pairing mode,
-                             bool same, double thresh, int width, bool lane_is_bra = true)
+                             bool same, double thresh, int width, bool lane_is_bra = true,
+                             double thresh_hi = kNoUpper)
         {
             TaskTable t;
             t.nket = ket.n_pairs;
@@ -186,21 +205,11 @@ namespace lbx
                 {
                     lo = (mode == Pairing::Unique && same) ? q : 0;
                     hi = bra.n_pairs;
+                    const double qk = ket.Q.empty() ? 1.0 : ket.Q[ket.order[q]];
                     if (thresh > 0)
-                    {
-                        const double qk = ket.Q.empty() ? 1.0 : ket.Q[ket.order[q]];
-                        // first p with qb[p] * qk < thresh (qb descending)
-                        int a = 0, b = bra.n_pairs;
-                        while (a < b)
-                        {
-                            int m = (a + b) / 2;
-                            if (qb[m] * qk >= thresh)
-                                a = m + 1;
-                            else
-                                b = m;
-                        }
-                        hi = a;
-                    }
+                        hi = firstBelow(qb, qk, thresh);
+                    if (thresh_hi < kNoUpper)
+                        lo = std::max(lo, firstBelow(qb, qk, thresh_hi));
                     hi = std::max(hi, lo);
                 }
                 t.plo[q] = lo;
@@ -279,6 +288,43 @@ namespace lbx
             if (!vb.fn && !vk.fn)
                 return VAR_MFMA;
             return cb <= ck ? VAR_VALU_BRA : VAR_VALU_KET;
+        }
+
+        // Default for the FP32 tier of a mixed-precision J/K: the cheaper FP32 one-lane form by
+        // the same estimate, or -1 when the class has none (its quartets then stay FP64).
+        int defaultVariant32(const PairClass &bra, const PairClass &ket)
+        {
+            const KernelSlot &vb = kernelSlot(MODE_JK, bra.la, bra.lb, ket.la, ket.lb, VAR_VALU_BRA32);
+            const KernelSlot &vk = kernelSlot(MODE_JK, bra.la, bra.lb, ket.la, ket.lb, VAR_VALU_KET32);
+            if (!vb.fn && !vk.fn)
+                return -1;
+            double cb = vb.fn ? valuCost(bra, ket, vb.nc) : 1e300;
+            double ck = vk.fn ? valuCost(ket, bra, vk.nc) : 1e300;
+            return cb <= ck ? VAR_VALU_BRA32 : VAR_VALU_KET32;
+        }
+
+        __global__ void toFloatKernel(const double *__restrict__ in, float *__restrict__ out, long long n)
+        {
+            for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+                 i += static_cast<long long>(gridDim.x) * blockDim.x)
+                out[i] = static_cast<float>(in[i]);
+        }
+
+        // FP32 copies of the E matrices, made once per context on first mixed-precision use
+        void ensureF32(Context &c)
+        {
+            for (auto &pc : c.cls)
+            {
+                if (pc.d_eb32 || pc.n_ppairs == 0)
+                    continue;
+                const long long n = static_cast<long long>(pc.eCount());
+                pc.d_eb32 = c.alloc<float>(n);
+                pc.d_ek32 = c.alloc<float>(n);
+                const int grid = static_cast<int>(std::min<long long>((n + 255) / 256, 65536));
+                hipLaunchKernelGGL(toFloatKernel, dim3(grid), dim3(256), 0, c.stream, pc.d_eb, pc.d_eb32, n);
+                hipLaunchKernelGGL(toFloatKernel, dim3(grid), dim3(256), 0, c.stream, pc.d_ek, pc.d_ek32, n);
+                LBX_HIP(hipGetLastError());
+            }
         }
 
         const KernelSlot &slotFor(int mode, const PairClass &bra, const PairClass &ket, int variant)
@@ -369,11 +415,12 @@ using namespace lbx;
 
 struct lbx_context : Context
 {
-    // J/K task tables per (bra class, ket class, Schwarz threshold, variant); Q does not
+    // J/K task tables per (bra class, ket class, bound range [lo, hi), variant); Q does not
     // depend on D
-    std::map<std::tuple<int, int, double, int>, TaskTable> jk_tasks;
-    // kernel variant per class pair chosen by lbx_tune (absent: work-estimate default)
-    std::map<std::pair<int, int>, int> tuned;
+    std::map<std::tuple<int, int, double, double, int>, TaskTable> jk_tasks;
+    // kernel variant per class pair chosen by lbx_tune (absent: work-estimate default), for
+    // the FP64 build and for the FP32 tier of the mixed build
+    std::map<std::pair<int, int>, int> tuned, tuned32;
 
     int variantFor(int i, int j)
     {
@@ -381,20 +428,32 @@ struct lbx_context : Context
         return it != tuned.end() ? it->second : defaultVariant(MODE_JK, cls[i], cls[j]);
     }
 
-    // Queues the J/K kernel of class pair (i >= j) with the given variant on the context stream.
+    // variant for the small-bound quartets of a mixed build: an FP32 form, or an FP64 one
+    // where the class has no FP32 form or tuning found the FP64 form faster
+    int variant32For(int i, int j)
+    {
+        auto it = tuned32.find({i, j});
+        if (it != tuned32.end())
+            return it->second;
+        const int v = defaultVariant32(cls[i], cls[j]);
+        return v >= 0 ? v : variantFor(i, j);
+    }
+
+    // Queues the J/K kernel of class pair (i >= j) with the given variant on the context stream,
+    // over the quartets with lo <= Q_ab Q_cd < hi.
     const TaskTable &launchJK(int i, int j, int var, const double *D, double *J, double *K,
-                              double thresh)
+                              double thresh, double thresh_hi = kNoUpper)
     {
         const PairClass &bra = cls[i], &ket = cls[j];
         const KernelSlot &s = slotFor(MODE_JK, bra, ket, var);
-        const auto key = std::make_tuple(i, j, thresh, var);
+        const auto key = std::make_tuple(i, j, thresh, thresh_hi, var);
         auto it = jk_tasks.find(key);
         if (it == jk_tasks.end())
             it = jk_tasks
                          .emplace(key, s.swap ? buildTasks(*this, ket, bra, Pairing::Unique, false, thresh,
-                                                           s.width, false)
+                                                           s.width, false, thresh_hi)
                                               : buildTasks(*this, bra, ket, Pairing::Unique, i == j, thresh,
-                                                           s.width))
+                                                           s.width, true, thresh_hi))
                          .first;
         const TaskTable &t = it->second;
         KArgs a = baseArgs(*this, bra, ket, t, i == j);
@@ -414,7 +473,7 @@ struct lbx_context : Context
         bool pending = false;
     };
     bool profiling = false;
-    std::map<std::pair<int, int>, Prof> prof;
+    std::map<std::tuple<int, int, int>, Prof> prof;   // (bra class, ket class, tier 0 FP64 / 1 FP32)
 
     ~lbx_context()
     {
@@ -663,57 +722,69 @@ extern "C"
     int lbx_jk(lbx_context *ctx, const double *D, double *J, double *K, double schwarz_thresh,
                int precision, double mixed_thresh, lbx_jk_stats *stats)
     {
-        (void)mixed_thresh;
         return guarded([&]
                        {
                            if (!ctx || !D || !J || !K)
                                throw Error(LBX_ERR_INVALID, "lbx_jk: null argument");
-                           if (precision != LBX_FP64)
-                               throw Error(LBX_ERR_INVALID, "lbx_jk: only LBX_FP64 is available");
+                           if (precision != LBX_FP64 && precision != LBX_MIXED)
+                               throw Error(LBX_ERR_INVALID, "lbx_jk: precision must be LBX_FP64 or LBX_MIXED");
                            if (!(schwarz_thresh >= 0))
                                throw Error(LBX_ERR_INVALID, "lbx_jk: negative Schwarz threshold");
+                           if (precision == LBX_MIXED && !(mixed_thresh >= 0))
+                               throw Error(LBX_ERR_INVALID, "lbx_jk: negative mixed-precision threshold");
                            std::lock_guard<std::mutex> lk(ctx->mu);
                            Context &c = *ctx;
                            ensureSchwarz(c);
+                           // quartets with schwarz <= Q_ab Q_cd < split run in FP32
+                           const bool mixed = precision == LBX_MIXED && mixed_thresh > schwarz_thresh;
+                           const double split = mixed ? mixed_thresh : schwarz_thresh;
+                           if (mixed)
+                               ensureF32(c);
                            const long long n = static_cast<long long>(c.nbf);
                            fill(J, n * n, 0.0, c.stream);
                            fill(K, n * n, 0.0, c.stream);
-                           double nq = 0, fl = 0;
-                           for (size_t i = 0; i < c.cls.size(); i++)
-                               for (size_t j = 0; j <= i; j++)
+                           double nq = 0, fl = 0, nq32 = 0;
+                           auto launch = [&](int i, int j, int tier, int var, double lo, double hi)
+                           {
+                               const PairClass &bra = c.cls[i], &ket = c.cls[j];
+                               lbx_context::Prof *pr = nullptr;
+                               if (ctx->profiling)
                                {
-                                   const PairClass &bra = c.cls[i], &ket = c.cls[j];
-                                   if (bra.n_pairs == 0 || ket.n_pairs == 0)
+                                   pr = &ctx->prof[{i, j, tier}];
+                                   if (!pr->start)
+                                   {
+                                       LBX_HIP(hipEventCreate(&pr->start));
+                                       LBX_HIP(hipEventCreate(&pr->stop));
+                                       pr->rec.la = bra.la;
+                                       pr->rec.lb = bra.lb;
+                                       pr->rec.lc = ket.la;
+                                       pr->rec.ld = ket.lb;
+                                   }
+                                   pr->rec.kernel = var;
+                                   LBX_HIP(hipEventRecord(pr->start, c.stream));
+                               }
+                               const TaskTable &t = ctx->launchJK(i, j, var, D, J, K, lo, hi);
+                               nq += t.quartets;
+                               fl += t.flops;
+                               if (var >= N_FP64_VARIANTS)
+                                   nq32 += t.quartets;
+                               if (pr)
+                               {
+                                   LBX_HIP(hipEventRecord(pr->stop, c.stream));
+                                   pr->pending = true;
+                                   pr->rec.calls += 1;
+                                   pr->rec.quartets += t.quartets;
+                                   pr->rec.flops += t.flops;
+                               }
+                           };
+                           for (int i = 0; i < static_cast<int>(c.cls.size()); i++)
+                               for (int j = 0; j <= i; j++)
+                               {
+                                   if (c.cls[i].n_pairs == 0 || c.cls[j].n_pairs == 0)
                                        continue;
-                                   const int var = ctx->variantFor(static_cast<int>(i), static_cast<int>(j));
-                                   lbx_context::Prof *pr = nullptr;
-                                   if (ctx->profiling)
-                                   {
-                                       pr = &ctx->prof[{static_cast<int>(i), static_cast<int>(j)}];
-                                       if (!pr->start)
-                                       {
-                                           LBX_HIP(hipEventCreate(&pr->start));
-                                           LBX_HIP(hipEventCreate(&pr->stop));
-                                           pr->rec.la = bra.la;
-                                           pr->rec.lb = bra.lb;
-                                           pr->rec.lc = ket.la;
-                                           pr->rec.ld = ket.lb;
-                                       }
-                                       pr->rec.kernel = var;
-                                       LBX_HIP(hipEventRecord(pr->start, c.stream));
-                                   }
-                                   const TaskTable &t = ctx->launchJK(static_cast<int>(i), static_cast<int>(j),
-                                                                      var, D, J, K, schwarz_thresh);
-                                   nq += t.quartets;
-                                   fl += t.flops;
-                                   if (pr)
-                                   {
-                                       LBX_HIP(hipEventRecord(pr->stop, c.stream));
-                                       pr->pending = true;
-                                       pr->rec.calls += 1;
-                                       pr->rec.quartets += t.quartets;
-                                       pr->rec.flops += t.flops;
-                                   }
+                                   launch(i, j, 0, ctx->variantFor(i, j), split, kNoUpper);
+                                   if (mixed)
+                                       launch(i, j, 1, ctx->variant32For(i, j), schwarz_thresh, split);
                                }
                            const int gridn = static_cast<int>(std::min<long long>((n * n + 255) / 256, 65536));
                            hipLaunchKernelGGL(symmetrizeKernel, dim3(gridn), dim3(256), 0, c.stream, J, static_cast<int>(n));
@@ -726,7 +797,7 @@ extern "C"
                            {
                                stats->quartets = nq;
                                stats->flops = fl;
-                               stats->quartets_fp32 = 0;
+                               stats->quartets_fp32 = nq32;
                            }
                        });
     }
@@ -755,12 +826,14 @@ extern "C"
                                        const PairClass &bra = c.cls[i], &ket = c.cls[j];
                                        if (bra.n_pairs == 0 || ket.n_pairs == 0)
                                            continue;
-                                       int best = -1;
-                                       float best_ms = 0;
+                                       int best = -1, best32 = -1;
+                                       float best_ms = 0, best32_ms = 0;
                                        for (int v = 0; v < N_VARIANTS; v++)
                                        {
                                            if (!kernelSlot(MODE_JK, bra.la, bra.lb, ket.la, ket.lb, v).fn)
                                                continue;
+                                           if (v >= N_FP64_VARIANTS)
+                                               ensureF32(c);
                                            // first launch builds the task table and loads the code
                                            ctx->launchJK(static_cast<int>(i), static_cast<int>(j), v, D, J.p, K.p,
                                                          schwarz_thresh);
@@ -771,14 +844,21 @@ extern "C"
                                            LBX_HIP(hipEventSynchronize(e1));
                                            float ms = 0;
                                            LBX_HIP(hipEventElapsedTime(&ms, e0, e1));
-                                           if (best < 0 || ms < best_ms)
+                                           if (v < N_FP64_VARIANTS && (best < 0 || ms < best_ms))
                                            {
                                                best = v;
                                                best_ms = ms;
                                            }
+                                           if (best32 < 0 || ms < best32_ms)
+                                           {
+                                               best32 = v;
+                                               best32_ms = ms;
+                                           }
                                        }
                                        if (best >= 0)
                                            ctx->tuned[{static_cast<int>(i), static_cast<int>(j)}] = best;
+                                       if (best32 >= 0)
+                                           ctx->tuned32[{static_cast<int>(i), static_cast<int>(j)}] = best32;
                                    }
                            }
                            catch (...)
@@ -806,11 +886,15 @@ extern "C"
                            if (variant < 0)
                            {
                                ctx->tuned.erase(key);
+                               ctx->tuned32.erase(key);
                                return;
                            }
                            if (variant >= N_VARIANTS || !kernelSlot(MODE_JK, la, lb, lc, ld, variant).fn)
                                throw Error(LBX_ERR_INVALID, "lbx_set_variant: class pair has no such variant");
-                           ctx->tuned[key] = variant;
+                           if (variant >= N_FP64_VARIANTS)
+                               ctx->tuned32[key] = variant;
+                           else
+                               ctx->tuned[key] = variant;
                        });
     }
 

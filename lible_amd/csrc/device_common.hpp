@@ -112,22 +112,23 @@ namespace lbx
         }
     };
 
-    template <int L, size_t... I>
-    __device__ __forceinline__ void hermiteRRun(double *R, double X, double Y, double Z,
-                                                std::index_sequence<I...>)
+    template <int L, typename T, size_t... I>
+    __device__ __forceinline__ void hermiteRRun(T *R, T X, T Y, T Z, std::index_sequence<I...>)
     {
         constexpr RProgram<L> P{};
         ((R[P.op[I].dst] = (P.op[I].coef > 0)
-                                   ? fma(double(P.op[I].coef), R[P.op[I].s2],
+                                   ? fma(T(P.op[I].coef), R[P.op[I].s2],
                                          (P.op[I].ax == 0 ? X : (P.op[I].ax == 1 ? Y : Z)) * R[P.op[I].s1])
                                    : (P.op[I].ax == 0 ? X : (P.op[I].ax == 1 ? Y : Z)) * R[P.op[I].s1]),
          ...);
     }
 
-    template <int L>
-    __device__ __forceinline__ void hermiteR(double *R, double X, double Y, double Z)
+    // T = double, or float for the reduced-precision quartets (the R^{(n)}_{000} inputs are
+    // formed in double from the double Boys values and rounded once)
+    template <int L, typename T>
+    __device__ __forceinline__ void hermiteR(T *R, T X, T Y, T Z)
     {
-        hermiteRRun<L>(R, X, Y, Z, std::make_index_sequence<static_cast<size_t>(rOpCount(L))>{});
+        hermiteRRun<L, T>(R, X, Y, Z, std::make_index_sequence<static_cast<size_t>(rOpCount(L))>{});
     }
 
     // slot in the hermiteR buffer of the Hermite index sum (tuv of bra index i) + (tuv of ket j)

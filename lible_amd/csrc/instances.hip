@@ -1,8 +1,9 @@
 // Instantiation unit of the quartet kernels.  Compiled once per LBX_UNIT:
-//   0..9   bra = main pair class LBX_UNIT (order (0,0),(1,0),(1,1),(2,0),...): eri4 J/K and dense
-//          for every ket class <= bra class, the (ab|ab) diagonal, and eri3 (ab|c) against every
-//          auxiliary shell l = 0..5 not already covered;
-//   10, 11 bra = single auxiliary shell l = 4, 5: eri2 (a|b) for b <= l and the (a|a) diagonal.
+//   0..9   bra = main pair class LBX_UNIT (order (0,0),(1,0),(1,1),(2,0),...): eri4 J/K (FP64
+//          forms) and dense for every ket class <= bra class, the (ab|ab) diagonal, and eri3
+//          (ab|c) against every auxiliary shell l = 0..5 not already covered;
+//   10, 11 bra = single auxiliary shell l = 4, 5: eri2 (a|b) for b <= l and the (a|a) diagonal;
+//   12..21 bra = main pair class LBX_UNIT - 12: the FP32 J/K forms.
 // Splitting the instantiations lets the build compile the units in parallel.
 #include "quartet.hpp"
 
@@ -14,9 +15,12 @@ namespace lbx
 {
     namespace
     {
+        constexpr bool kF32Unit = LBX_UNIT >= 12;
+        constexpr int kPairUnit = kF32Unit ? LBX_UNIT - 12 : LBX_UNIT;
+
         constexpr int unitLa()
         {
-            constexpr int u = LBX_UNIT;
+            constexpr int u = kPairUnit;
             if (u >= 10)
                 return u - 6;   // 10 -> 4, 11 -> 5
             int la = 0;
@@ -24,7 +28,7 @@ namespace lbx
                 la++;
             return la;
         }
-        constexpr int unitLb() { return LBX_UNIT >= 10 ? 0 : LBX_UNIT - pairClassIdx(unitLa(), 0); }
+        constexpr int unitLb() { return kPairUnit >= 10 ? 0 : kPairUnit - pairClassIdx(unitLa(), 0); }
 
         constexpr int PA = unitLa(), PB = unitLb();
         constexpr int CI = pairClassIdx(PA, PB);
@@ -32,25 +36,30 @@ namespace lbx
         template <int LC, int LD>
         void regEri4()
         {
-            if constexpr (LBX_UNIT < 10 && pairClassIdx(LC, LD) <= CI)
+            if constexpr (kPairUnit < 10 && pairClassIdx(LC, LD) <= CI)
             {
-                registerQuartet<PA, PB, LC, LD, MODE_JK>();
-                registerQuartet<PA, PB, LC, LD, MODE_DENSE>();
+                if constexpr (kF32Unit)
+                    registerQuartet<PA, PB, LC, LD, MODE_JK, 2>();
+                else
+                {
+                    registerQuartet<PA, PB, LC, LD, MODE_JK, 1>();
+                    registerQuartet<PA, PB, LC, LD, MODE_DENSE, 1>();
+                }
             }
         }
 
         template <int LC>
         void regEri3()
         {
-            if constexpr (LBX_UNIT < 10 && !(LC <= 3 && pairClassIdx(LC, 0) <= CI))
-                registerQuartet<PA, PB, LC, 0, MODE_DENSE>();
+            if constexpr (!kF32Unit && kPairUnit < 10 && !(LC <= 3 && pairClassIdx(LC, 0) <= CI))
+                registerQuartet<PA, PB, LC, 0, MODE_DENSE, 1>();
         }
 
         template <int LC>
         void regEri2()
         {
-            if constexpr (LBX_UNIT >= 10 && LC <= PA)
-                registerQuartet<PA, 0, LC, 0, MODE_DENSE>();
+            if constexpr (!kF32Unit && kPairUnit >= 10 && LC <= PA)
+                registerQuartet<PA, 0, LC, 0, MODE_DENSE, 1>();
         }
 
         struct Registrar
@@ -79,7 +88,8 @@ namespace lbx
                 regEri2<3>();
                 regEri2<4>();
                 regEri2<5>();
-                registerQuartet<PA, PB, PA, PB, MODE_DIAG>();
+                if constexpr (!kF32Unit)
+                    registerQuartet<PA, PB, PA, PB, MODE_DIAG, 1>();
             }
         } registrar;
     }

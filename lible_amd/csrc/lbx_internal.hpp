@@ -53,6 +53,7 @@ namespace lbx
         const int *order;           // sorted position -> pair index
         const double *q;            // Schwarz factor per sorted position
         int n;
+        const float *eb32, *ek32;   // eb, ek rounded to FP32 (mixed precision; may be null)
     };
 
     // Pairs of shells (la >= lb), or single shells (lb = 0, single = true: an auxiliary shell
@@ -73,11 +74,18 @@ namespace lbx
         int *d_poff = nullptr, *d_pcnt = nullptr, *d_order = nullptr;
         double4 *d_pp = nullptr;
         double *d_eb = nullptr, *d_ek = nullptr, *d_q = nullptr;
+        float *d_eb32 = nullptr, *d_ek32 = nullptr;   // made on first mixed-precision use
+
+        // doubles in each of d_eb, d_ek
+        size_t eCount() const
+        {
+            return static_cast<size_t>(n_ppairs) * nHerm(la + lb) * nSph(la) * nSph(lb);
+        }
 
         DPairs view() const
         {
             return DPairs{d_ao_a, d_ao_b, d_sh_a, d_sh_b, d_poff, d_pcnt, d_pp, d_eb, d_ek,
-                          d_order, d_q, n_pairs};
+                          d_order, d_q, n_pairs, d_eb32, d_ek32};
         }
     };
 

@@ -88,6 +88,16 @@ namespace lbx
         }
         static constexpr int NC0 = ncFor(SP, HP, SQ);   // lanes = bra, uniform = ket
         static constexpr int NC1 = ncFor(SQ, HQ, SP);   // lanes = ket, uniform = bra
+        // FP32 form: the same register budget holds twice as many values
+        static constexpr int ncFor32(int sl, int hl, int su)
+        {
+            const int nc = (2 * kBudget - (HL + L)) / (sl + hl);
+            return nc < 1 ? 0 : (nc > su ? su : nc);
+        }
+        static constexpr int NC0f = ncFor32(SP, HP, SQ);
+        static constexpr int NC1f = ncFor32(SQ, HQ, SP);
+        template <typename T, bool SWAP>
+        static constexpr int nc() { return sizeof(T) == 4 ? (SWAP ? NC1f : NC0f) : (SWAP ? NC1 : NC0); }
 
         // ---- MFMA tiling
         static constexpr int NQ = ceilDiv(SQ, 16), NP = ceilDiv(SP, 16), KQ = ceilDiv(HQ, 4);
@@ -137,7 +147,9 @@ namespace lbx
     // ------------------------------------------------------------------------------- quartetV
     // One quartet per lane: lane-side pair l (64 consecutive sorted positions) against the
     // wave-uniform pair u.  SWAP = false: lanes = bra, u = ket; SWAP = true: lanes = ket, u = bra.
-    template <int LA, int LB, int LC, int LD, int MODE, bool SWAP>
+    // T: type of R, the E matrices and both transforms (double; float for the reduced-precision
+    // J/K quartets, whose Boys values, epilogue and accumulation stay double).
+    template <int LA, int LB, int LC, int LD, int MODE, bool SWAP, typename T>
     __device__ __forceinline__ void quartetVTask(const KArgs &A, int q, int local, int lane)
     {
         using C = Cls<LA, LB, LC, LD>;
@@ -148,8 +160,10 @@ namespace lbx
         constexpr int SLa = nSph(La), SLb = nSph(Lb), SUa = nSph(Ua), SUb = nSph(Ub);
         constexpr int SL = SLa * SLb, SU = SUa * SUb;
         constexpr int HLn = nHerm(La + Lb), HU = nHerm(Ua + Ub);
-        constexpr int NC = SWAP ? C::NC1 : C::NC0;
+        constexpr int NC = C::template nc<T, SWAP>();
         constexpr int NPASS = ceilDiv(SU, NC);
+        constexpr bool F32 = sizeof(T) == 4;
+        static_assert(!F32 || MODE == MODE_JK, "FP32 form is for J/K only");
         const DPairs &LP = SWAP ? A.ket : A.bra;
         const DPairs &UP = SWAP ? A.bra : A.ket;
 
@@ -161,8 +175,19 @@ namespace lbx
         const int upoff = uniform(UP.poff[u]), ucnt = uniform(UP.pcnt[u]);
         const int lpoff = LP.poff[l], lcnt = active ? LP.pcnt[l] : 0;
         // the ket's E carries the sign (-1)^{t'+u'+v'}
-        const double *__restrict__ eU = (SWAP ? UP.eb : UP.ek) + static_cast<size_t>(upoff) * HU * SU;
-        const double *__restrict__ eL = (SWAP ? LP.ek : LP.eb) + static_cast<size_t>(lpoff) * HLn * SL;
+        const T *eUb, *eLb;
+        if constexpr (F32)
+        {
+            eUb = SWAP ? UP.eb32 : UP.ek32;
+            eLb = SWAP ? LP.ek32 : LP.eb32;
+        }
+        else
+        {
+            eUb = SWAP ? UP.eb : UP.ek;
+            eLb = SWAP ? LP.ek : LP.eb;
+        }
+        const T *__restrict__ eU = eUb + static_cast<size_t>(upoff) * HU * SU;
+        const T *__restrict__ eL = eLb + static_cast<size_t>(lpoff) * HLn * SL;
 
         const int l0 = active ? LP.ao_a[l] : 0, l1 = active ? LP.ao_b[l] : 0;
         const int u0 = uniform(UP.ao_a[u]), u1 = uniform(UP.ao_b[u]);
@@ -191,18 +216,18 @@ namespace lbx
 #pragma unroll 1
         for (int pass = 0; pass < NPASS; pass++)
         {
-            double I[SL * NC];
+            T I[SL * NC];
 #pragma unroll
             for (int e = 0; e < SL * NC; e++)
-                I[e] = 0.0;
+                I[e] = T(0);
 
             for (int i = 0; i < lcnt; i++)
             {
                 const double4 Pl = LP.pp[lpoff + i];
-                double X[HLn * NC];
+                T X[HLn * NC];
 #pragma unroll
                 for (int e = 0; e < HLn * NC; e++)
-                    X[e] = 0.0;
+                    X[e] = T(0);
                 for (int k = 0; k < ucnt; k++)
                 {
                     const double4 Pu = UP.pp[upoff + k];
@@ -211,38 +236,39 @@ namespace lbx
                     const double pq = Pb.x * Pk.x, ps = Pb.x + Pk.x;
                     const double alpha = pq / ps;
                     const double X_ = Pb.y - Pk.y, Y_ = Pb.z - Pk.z, Z_ = Pb.w - Pk.w;
-                    const double T = alpha * (X_ * X_ + Y_ * Y_ + Z_ * Z_);
-                    double R[HL + L];
-                    boysF<L>(T, A.boys, R);
+                    const double Tb = alpha * (X_ * X_ + Y_ * Y_ + Z_ * Z_);
+                    double F[L + 1];
+                    boysF<L>(Tb, A.boys, F);
+                    T R[HL + L];
                     double sc = kTwoPi52 / (pq * sqrt(ps));
 #pragma unroll
                     for (int m = 0; m <= L; m++)
                     {
-                        R[m] *= sc;
+                        R[m] = T(F[m] * sc);
                         sc *= -2.0 * alpha;
                     }
-                    hermiteR<L>(R, X_, Y_, Z_);
-                    const double *__restrict__ ek = eU + static_cast<size_t>(k) * HU * SU;
+                    hermiteR<L, T>(R, T(X_), T(Y_), T(Z_));
+                    const T *__restrict__ ek = eU + static_cast<size_t>(k) * HU * SU;
                     constexpr RSumTab<L, HLn, HU> RS{};
 #pragma unroll
                     for (int h = 0; h < HLn; h++)
 #pragma unroll
                         for (int g = 0; g < HU; g++)
                         {
-                            const double r = R[RS.v[h * HU + g]];
+                            const T r = R[RS.v[h * HU + g]];
 #pragma unroll
                             for (int c = 0; c < NC; c++)
                                 if (pass * NC + c < SU)
                                     X[h * NC + c] = fma(r, ek[g * SU + pass * NC + c], X[h * NC + c]);
                         }
                 }
-                const double *el = eL + static_cast<size_t>(i) * HLn * SL;
+                const T *el = eL + static_cast<size_t>(i) * HLn * SL;
 #pragma unroll
                 for (int h = 0; h < HLn; h++)
 #pragma unroll
                     for (int s = 0; s < SL; s++)
                     {
-                        const double e = el[h * SL + s];
+                        const T e = el[h * SL + s];
 #pragma unroll
                         for (int c = 0; c < NC; c++)
                             I[s * NC + c] = fma(e, X[h * NC + c], I[s * NC + c]);
@@ -266,8 +292,9 @@ namespace lbx
                     for (int s = 0; s < SL; s++)
                     {
                         const int al = s / SLb, be = s % SLb;
-                        jl[s] = fma(I[s * NC + c], du, jl[s]);
-                        ju = fma(I[s * NC + c], active ? D[(l0 + al) * n + l1 + be] : 0.0, ju);
+                        const double iv = I[s * NC + c];
+                        jl[s] = fma(iv, du, jl[s]);
+                        ju = fma(iv, active ? D[(l0 + al) * n + l1 + be] : 0.0, ju);
                     }
                     ju = waveSum(2.0 * f * ju);
                     if (lane == 0)
@@ -301,7 +328,7 @@ namespace lbx
                                     const int su = ga * SUb + de;
                                     if (su / NC != pass)
                                         continue;
-                                    s = fma(I[(al * SLb + be) * NC + su % NC],
+                                    s = fma(double(I[(al * SLb + be) * NC + su % NC]),
                                             D[(l1 + be) * n + u1 + de], s);
                                 }
                             unsafeAtomicAdd(&A.K[(l0 + al) * n + u0 + ga], f * s);
@@ -318,7 +345,7 @@ namespace lbx
                                     const int su = ga * SUb + de;
                                     if (su / NC != pass)
                                         continue;
-                                    s = fma(I[(al * SLb + be) * NC + su % NC],
+                                    s = fma(double(I[(al * SLb + be) * NC + su % NC]),
                                             D[(l0 + al) * n + u1 + de], s);
                                 }
                             unsafeAtomicAdd(&A.K[(l1 + be) * n + u0 + ga], f * s);
@@ -345,7 +372,7 @@ namespace lbx
                                     const int su = ga * SUb + de;
                                     if (su / NC != pass)
                                         continue;
-                                    s = fma(I[(al * SLb + be) * NC + su % NC],
+                                    s = fma(double(I[(al * SLb + be) * NC + su % NC]),
                                             D[(l1 + be) * n + u0 + ga], s);
                                 }
                             unsafeAtomicAdd(&A.K[(l0 + al) * n + u1 + de], f * s);
@@ -362,7 +389,7 @@ namespace lbx
                                     const int su = ga * SUb + de;
                                     if (su / NC != pass)
                                         continue;
-                                    s = fma(I[(al * SLb + be) * NC + su % NC],
+                                    s = fma(double(I[(al * SLb + be) * NC + su % NC]),
                                             D[(l0 + al) * n + u0 + ga], s);
                                 }
                             unsafeAtomicAdd(&A.K[(l1 + be) * n + u1 + de], f * s);
@@ -425,7 +452,7 @@ namespace lbx
 
     // Each wave takes a contiguous range of tasks (so consecutive tasks share the uniform pair
     // and the task -> q search runs once per wave).
-    template <int LA, int LB, int LC, int LD, int MODE, bool SWAP>
+    template <int LA, int LB, int LC, int LD, int MODE, bool SWAP, typename T>
     __global__ __launch_bounds__(256) void quartetV(const KArgs A)
     {
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -444,7 +471,7 @@ namespace lbx
                 base = next;
                 next = A.prefix[q + 1];
             }
-            quartetVTask<LA, LB, LC, LD, MODE, SWAP>(A, q, static_cast<int>(t - base), lane);
+            quartetVTask<LA, LB, LC, LD, MODE, SWAP, T>(A, q, static_cast<int>(t - base), lane);
         }
     }
 
@@ -760,7 +787,7 @@ namespace lbx
         }
     }
 
-    template <int LA, int LB, int LC, int LD, int MODE, bool SWAP>
+    template <int LA, int LB, int LC, int LD, int MODE, bool SWAP, typename T>
     void launchValu(const KArgs &A, hipStream_t s)
     {
         if (A.ntasks <= 0)
@@ -768,7 +795,7 @@ namespace lbx
         // contiguous task ranges per wave; up to 8192 workgroups of 4 waves
         long long wg = (A.ntasks + 3) / 4;
         int grid = static_cast<int>(wg < 8192 ? wg : 8192);
-        hipLaunchKernelGGL((quartetV<LA, LB, LC, LD, MODE, SWAP>), dim3(grid), dim3(256), 0, s, A);
+        hipLaunchKernelGGL((quartetV<LA, LB, LC, LD, MODE, SWAP, T>), dim3(grid), dim3(256), 0, s, A);
     }
 
     template <int LA, int LB, int LC, int LD, int MODE>
@@ -780,17 +807,46 @@ namespace lbx
         hipLaunchKernelGGL((quartetM<LA, LB, LC, LD, MODE>), dim3(grid), dim3(64), 0, s, A);
     }
 
+    // The FP32 one-lane forms (J/K only).
+    template <int LA, int LB, int LC, int LD, int MODE>
+    void registerQuartet32()
+    {
+        using C = Cls<LA, LB, LC, LD>;
+        constexpr bool same = LA == LC && LB == LD;
+        if constexpr (MODE == MODE_JK && C::NC0f > 0)
+        {
+            KernelSlot &s = kernelSlot(MODE, LA, LB, LC, LD, VAR_VALU_BRA32);
+            s.fn = &launchValu<LA, LB, LC, LD, MODE, false, float>;
+            s.width = 64;
+            s.swap = false;
+            s.nc = C::NC0f;
+        }
+        if constexpr (MODE == MODE_JK && C::NC1f > 0 && !same)
+        {
+            KernelSlot &s = kernelSlot(MODE, LA, LB, LC, LD, VAR_VALU_KET32);
+            s.fn = &launchValu<LA, LB, LC, LD, MODE, true, float>;
+            s.width = 64;
+            s.swap = true;
+            s.nc = C::NC1f;
+        }
+    }
+
     // Registers every variant the class supports in this mode.  The ket-on-lanes VALU form
     // only serves J/K between different classes (dense and diagonal outputs keep the bra on
     // the lanes; for one class both forms coincide).  The MFMA form is always available.
-    template <int LA, int LB, int LC, int LD, int MODE>
+    // FORMS: 1 the FP64 forms, 2 the FP32 forms, 3 both (lets the build compile them apart).
+    template <int LA, int LB, int LC, int LD, int MODE, int FORMS = 3>
     void registerQuartet()
     {
         using C = Cls<LA, LB, LC, LD>;
+        if constexpr (FORMS & 2)
+            registerQuartet32<LA, LB, LC, LD, MODE>();
+        if constexpr (!(FORMS & 1))
+            return;
         if constexpr (C::NC0 > 0)
         {
             KernelSlot &s = kernelSlot(MODE, LA, LB, LC, LD, VAR_VALU_BRA);
-            s.fn = &launchValu<LA, LB, LC, LD, MODE, false>;
+            s.fn = &launchValu<LA, LB, LC, LD, MODE, false, double>;
             s.width = 64;
             s.swap = false;
             s.nc = C::NC0;
@@ -799,7 +855,7 @@ namespace lbx
         if constexpr (MODE == MODE_JK && C::NC1 > 0 && !same)
         {
             KernelSlot &s = kernelSlot(MODE, LA, LB, LC, LD, VAR_VALU_KET);
-            s.fn = &launchValu<LA, LB, LC, LD, MODE, true>;
+            s.fn = &launchValu<LA, LB, LC, LD, MODE, true, double>;
             s.width = 64;
             s.swap = true;
             s.nc = C::NC1;

@@ -22,7 +22,7 @@ SYMBOLS = ["lbx_create", "lbx_destroy", "lbx_last_error", "lbx_dims", "lbx_set_s
            "lbx_synchronize", "lbx_boys", "lbx_eri4", "lbx_eri4_diag", "lbx_eri3", "lbx_eri2",
            "lbx_eri2_diag", "lbx_jk", "lbx_tune", "lbx_set_variant", "lbx_profile_enable",
            "lbx_profile_read"]
-VARIANTS = ("valu-bra", "valu-ket", "mfma")
+VARIANTS = ("valu-bra", "valu-ket", "mfma", "valu-bra-f32", "valu-ket-f32")
 
 
 class LbxError(RuntimeError):

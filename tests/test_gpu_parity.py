@@ -142,6 +142,45 @@ def test_jk_every_variant(lbx, jk_case, variant):
     assert st["quartets"] == sr["quartets"]
 
 
+@pytest.mark.parametrize("variant", [3, 4])
+def test_jk_fp32_every_variant(lbx, jk_case, variant):
+    """Every FP32 form, on every quartet (mixed threshold above every bound).  FP32 R and
+    transforms carry ~1e-7 relative rounding per term, so J and K agree with the FP64 oracle to
+    a few 1e-6 of their largest element."""
+    import torch
+    m, D, Jr, Kr, sr = jk_case
+    ctx = lbx.Context(m)
+    used = [c for c in CLASS_PAIRS if ctx.set_variant(c, variant)]
+    assert len(used) > 0
+    J, K, st = ctx.jk(torch.tensor(D, device="cuda"), schwarz=1e-12, precision=lbx.LBX_MIXED,
+                      mixed_thresh=1e30)
+    assert st["quartets"] == sr["quartets"]
+    assert st["quartets_fp32"] > 0.5 * st["quartets"]
+    close(J.cpu().numpy(), Jr, rel=0.0, floor=1e-5)
+    close(K.cpu().numpy(), Kr, rel=0.0, floor=1e-5)
+
+
+def test_jk_mixed_energy(lbx):
+    """Mixed precision at the 1e-5 bound split against the FP64 build of the same library (which
+    the tests above pin to the oracle): the energy moves by <= 1e-10 Eh."""
+    import torch
+    m = B.synthetic_molecule(4242, 10, 9.0, B.RECIPE_MAIN)
+    ctx = lbx.Context(m)
+    D = torch.tensor(B.density_from_orbitals(43, ctx.nbf, 25, 0.1), device="cuda")
+    J, K, st = ctx.jk(D, schwarz=1e-12)
+    Jm, Km, sm = ctx.jk(D, schwarz=1e-12, precision=lbx.LBX_MIXED, mixed_thresh=1e-5)
+    assert sm["quartets"] == st["quartets"]
+    assert sm["quartets_fp32"] > 0.2 * sm["quartets"]
+    E = 0.5 * float((D * J).sum()) - 0.25 * float((D * K).sum())
+    Em = 0.5 * float((D * Jm).sum()) - 0.25 * float((D * Km).sum())
+    dJ = float((Jm - J).abs().max())
+    dK = float((Km - K).abs().max())
+    print(f"mixed: fp32 share {sm['quartets_fp32'] / sm['quartets']:.3f} dE {Em - E:.3e} "
+          f"max dJ {dJ:.3e} max dK {dK:.3e}")
+    assert abs(Em - E) <= 1e-10
+    assert dJ <= 1e-9 and dK <= 1e-9
+
+
 def test_jk_tuned(lbx, jk_case):
     import torch
     m, D, Jr, Kr, _ = jk_case
@@ -151,6 +190,10 @@ def test_jk_tuned(lbx, jk_case):
     J, K, _ = ctx.jk(Dt, schwarz=1e-12)
     close(J.cpu().numpy(), Jr, floor=1e-13)
     close(K.cpu().numpy(), Kr, floor=1e-13)
+    # the tuned FP32 tier
+    J, K, st = ctx.jk(Dt, schwarz=1e-12, precision=lbx.LBX_MIXED, mixed_thresh=1e30)
+    close(J.cpu().numpy(), Jr, rel=0.0, floor=1e-5)
+    close(K.cpu().numpy(), Kr, rel=0.0, floor=1e-5)
 
 
 @pytest.mark.parametrize("thresh", [0.0, 1e-12])
