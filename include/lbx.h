@@ -88,11 +88,42 @@ typedef struct lbx_jk_stats {
 int lbx_jk(lbx_context *ctx, const double *D, double *J, double *K, double schwarz_thresh,
            int precision, double mixed_thresh, lbx_jk_stats *stats);
 
+/* Batched RI integrals over many independent molecules (no reference counterpart: the
+ * reference runs eri3() / eri2() one Structure at a time, eri3.cpp:7-46, eri2.cpp:27-72).
+ * lbx_create_batch builds one context over nmol molecules (main_bases[k], aux_bases[k] for
+ * molecule k, each with its own atoms); shell pairs only form within a molecule, so the
+ * dense-tensor entry points, J/K, the gradient and the tuner reject a batched context.
+ * lbx_ri_sums evaluates every molecule's (mn|P) and (P|Q) and reduces them on the fly: out is a
+ * device (nmol, 4) array of the sums of |value| and of value over the dense eri3 tensor
+ * (nbf, nbf, naux) and over the dense eri2 tensor (naux, naux) of each molecule, i.e. the
+ * checksums the reference's tests take of eri3() and eri2() (tests_ints.cpp:694-746).  The
+ * whole launch sequence is captured into a HIP graph on the first call (per out pointer) and
+ * replayed afterwards.  Also valid on an ordinary context with an auxiliary basis (nmol = 1). */
+typedef struct lbx_ri_stats {
+    double triplets;   /* (ab|P) shell triplets evaluated */
+    double doublets;   /* (P|Q) shell doublets evaluated */
+    double flops;      /* algorithmic FLOPs */
+} lbx_ri_stats;
+int lbx_create_batch(lbx_context **ctx, int nmol, const lbx_basis *main_bases,
+                     const lbx_basis *aux_bases);
+int lbx_ri_sums(lbx_context *ctx, double *out, lbx_ri_stats *stats);
+
+/* Nuclear gradient of E = 1/2 tr(DJ) - 1/4 tr(DK) for a fixed density, no reference counterpart
+ * (the reference supplies the derivative integrals, ERI4D1Kernel, eri_kernels.hpp:113-131 /
+ * eri_kernel_funs.cpp:582-748; this contracts them with the density inside the kernel):
+ *   grad[3 A + x] = sum over unique quartets with Q_ab Q_cd >= schwarz_thresh of
+ *                   f (4 D_mn D_kl - D_mk D_nl - D_ml D_nk) d(mn|kl)/dR_{A,x}
+ * with f the permutational weight.  D: device (nbf, nbf) symmetric; grad: device 3 natoms
+ * (atom-major, x y z), Eh/bohr.  FP64. */
+int lbx_jk_grad(lbx_context *ctx, const double *D, double *grad, double schwarz_thresh,
+                lbx_jk_stats *stats);
+
 /* Autotuner (no reference counterpart).  Each class pair (la lb|lc ld) has up to five kernel
  * variants: 0 one lane per quartet with the bra pairs on the lanes, 1 the same with the ket
  * pairs on the lanes, 2 one wave per quartet on FP64 MFMA, and 3, 4 the forms 0, 1 with R and
  * both transforms in FP32 (the small-bound quartets of LBX_MIXED).  lbx_tune times every
- * variant of every class pair on this density and threshold and keeps the fastest FP64 form
+ * variant of every class pair on this density and threshold (on the leading ~2M quartets of
+ * each class pair, largest bounds first) and keeps the fastest per quartet: the fastest FP64 form
  * for FP64 work and the fastest of all forms for the FP32 tier of mixed builds; without it
  * lbx_jk picks by a work estimate from the primitive-pair counts. */
 int lbx_tune(lbx_context *ctx, const double *D, double schwarz_thresh);

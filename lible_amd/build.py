@@ -24,6 +24,7 @@ SELFTEST_SRC = os.path.join(ROOT, "tests", "cpp", "ints_selftest.cpp")
 SELFTEST = os.path.join(HERE, "bin", "ints_selftest")
 
 N_UNITS = 22   # csrc/instances.hip: 0..11 FP64 forms and dense outputs, 12..21 FP32 J/K forms
+N_GRAD_UNITS = 10   # csrc/grad_instances.hip: one per lane-side main pair class
 
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950", "-mcode-object-version=5",
@@ -92,6 +93,9 @@ def build(jobs: int | None = None, verbose: bool = True, incremental: bool = Fal
     for u in range(N_UNITS):
         work.append((os.path.join(CSRC, "instances.hip"), os.path.join(BUILD, f"inst_{u}.o"),
                      [f"-DLBX_UNIT={u}"]))
+    for u in range(N_GRAD_UNITS):
+        work.append((os.path.join(CSRC, "grad_instances.hip"), os.path.join(BUILD, f"grad_{u}.o"),
+                     [f"-DLBX_GUNIT={u}"]))
     # heaviest units (highest bra class) first
     work.sort(key=lambda w: -(int(w[2][0].split("=")[1]) % 12) if w[2] else 1)
     t0 = time.time()

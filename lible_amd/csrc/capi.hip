@@ -1,5 +1,6 @@
 // The C-ABI of lbx (include/lbx.h): context set-up, Schwarz screening, task tables and the
 // drivers that launch the quartet kernels class pair by class pair.
+#include "grad.hpp"
 #include "quartet.hpp"
 #include "registry.hpp"
 
@@ -142,6 +143,8 @@ namespace lbx
             long long *d_prefix = nullptr;
             int *d_plo = nullptr, *d_phi = nullptr;
             double quartets = 0, flops = 0;
+            // over the quartets: sum of K_bra K_ket, of K_bra, of K_ket (canonical roles)
+            double sum_kk = 0, sum_kbra = 0, sum_kket = 0;
         };
 
         enum class Pairing
@@ -167,6 +170,8 @@ namespace lbx
         }
 
         constexpr double kNoUpper = std::numeric_limits<double>::infinity();
+        // quartets per class pair and variant that the autotuner times
+        constexpr long long kTuneSample = 2LL << 20;
 
         // Lane-side positions [plo, phi) per uniform-side position q, with the Schwarz cut
         // thresh <= Q_p Q_q < thresh_hi on descending-sorted Q (thresh <= 0: no lower cut).
@@ -222,6 +227,9 @@ namespace lbx
                     const double kq = ket.pcnt[ket.order[q]];
                     const double skl = kab_prefix[hi] - kab_prefix[lo];
                     t.quartets += cnt;
+                    t.sum_kk += kq * skl;
+                    t.sum_kbra += lane_is_bra ? skl : kq * cnt;
+                    t.sum_kket += lane_is_bra ? kq * cnt : skl;
                     if (lane_is_bra)
                         t.flops += perKK * kq * skl + 2 * Sab * Hab * Scd * skl + 12 * Sab * Scd * cnt;
                     else
@@ -234,6 +242,69 @@ namespace lbx
             t.d_plo = c.upload(t.plo);
             t.d_phi = c.upload(t.phi);
             return t;
+        }
+
+        // Tasks of the batched RI sums: for every uniform-side position q (identity order: a
+        // batched context is never Schwarz-sorted) the lane-side pairs of q's molecule, and with
+        // `same` only lane positions p >= q.  No screening.
+        TaskTable buildTasksMol(Context &c, const PairClass &lane, const PairClass &unif, bool same,
+                                int width)
+        {
+            TaskTable t;
+            t.nket = unif.n_pairs;
+            t.plo.resize(t.nket);
+            t.phi.resize(t.nket);
+            t.prefix.resize(t.nket + 1);
+            std::vector<int> lo(c.nmol, 0), hi(c.nmol, 0);
+            std::vector<double> kl_prefix(lane.n_pairs + 1, 0.0);
+            for (int p = lane.n_pairs; p-- > 0;)
+                lo[lane.mol[lane.order[p]]] = p;
+            for (int p = 0; p < lane.n_pairs; p++)
+            {
+                hi[lane.mol[lane.order[p]]] = p + 1;
+                kl_prefix[p + 1] = kl_prefix[p] + lane.pcnt[lane.order[p]];
+            }
+            const int la = lane.la, lb = lane.lb, lc = unif.la, ld = unif.lb;
+            const double Hab = nHerm(la + lb), Sab = nSph(la) * nSph(lb), Scd = nSph(lc) * nSph(ld);
+            const double perKK = quartetFlops(la, lb, lc, ld, 1, 1) - 2 * Sab * Hab * Scd - 12 * Sab * Scd;
+            long long acc = 0;
+            for (int q = 0; q < t.nket; q++)
+            {
+                const int m = unif.mol[unif.order[q]];
+                int a = hi[m] > 0 ? lo[m] : 0, b = hi[m];
+                if (same)
+                    a = std::max(a, q);
+                b = std::max(a, b);
+                t.plo[q] = a;
+                t.phi[q] = b;
+                t.prefix[q] = acc;
+                const long long cnt = b - a;
+                acc += (cnt + width - 1) / width;
+                if (cnt > 0)
+                {
+                    const double kq = unif.pcnt[unif.order[q]];
+                    const double skl = kl_prefix[b] - kl_prefix[a];
+                    t.quartets += cnt;
+                    t.sum_kk += kq * skl;
+                    t.sum_kbra += skl;
+                    t.sum_kket += kq * cnt;
+                    // transforms, R and Boys as for J/K; the epilogue sums 2 values per element
+                    t.flops += perKK * kq * skl + 2 * Sab * Hab * Scd * skl + 2 * Sab * Scd * cnt;
+                }
+            }
+            t.prefix[t.nket] = acc;
+            t.ntasks = acc;
+            t.d_prefix = c.upload(t.prefix);
+            t.d_plo = c.upload(t.plo);
+            t.d_phi = c.upload(t.phi);
+            return t;
+        }
+
+        void requireSingle(const Context &c, const char *who)
+        {
+            if (!c.atom_mol.empty())
+                throw Error(LBX_ERR_STATE, std::string(who) + ": not available on a batched context "
+                                                              "(pairs only exist within each molecule)");
         }
 
         KArgs baseArgs(Context &c, const PairClass &bra, const PairClass &ket, const TaskTable &t,
@@ -373,11 +444,30 @@ namespace lbx
 
     KernelSlot &kernelSlot(int mode, int la, int lb, int lc, int ld, int variant)
     {
-        static KernelSlot tab[3][kMaxLAux + 1][kMaxLMain + 1][kMaxLAux + 1][kMaxLMain + 1][N_VARIANTS];
-        if (mode < 0 || mode > 2 || la < 0 || la > kMaxLAux || lc < 0 || lc > kMaxLAux || lb < 0 ||
+        static KernelSlot tab[N_MODES][kMaxLAux + 1][kMaxLMain + 1][kMaxLAux + 1][kMaxLMain + 1][N_VARIANTS];
+        if (mode < 0 || mode >= N_MODES || la < 0 || la > kMaxLAux || lc < 0 || lc > kMaxLAux || lb < 0 ||
             lb > kMaxLMain || ld < 0 || ld > kMaxLMain || variant < 0 || variant >= N_VARIANTS)
             throw Error(LBX_ERR_INVALID, "kernel slot out of range");
         return tab[mode][la][lb][lc][ld][variant];
+    }
+
+    GradSlot &gradSlot(int la, int lb, int lc, int ld)
+    {
+        static GradSlot tab[kMaxLMain + 1][kMaxLMain + 1][kMaxLMain + 1][kMaxLMain + 1];
+        if (la < 0 || la > kMaxLMain || lb < 0 || lb > kMaxLMain || lc < 0 || lc > kMaxLMain || ld < 0 ||
+            ld > kMaxLMain)
+            throw Error(LBX_ERR_INVALID, "gradient kernel slot out of range");
+        return tab[la][lb][lc][ld];
+    }
+
+    // Algorithmic FLOPs of the gradient per quartet (7 ket-sized and 9 bra-sized transforms, R to
+    // order L + 1, the density contraction of 12 derivative blocks), from a task table's sums.
+    double gradFlops(const TaskTable &t, int la, int lb, int lc, int ld)
+    {
+        const double Hab = nHerm(la + lb), Hcd = nHerm(lc + ld);
+        const double Sab = nSph(la) * nSph(lb), Scd = nSph(lc) * nSph(ld);
+        return t.sum_kk * (7 * 2 * Hab * Hcd * Scd + rFlopsHost(la + lb + lc + ld + 1) + Hab * Hcd) +
+               t.sum_kbra * 9 * 2 * Sab * Hab * Scd + t.quartets * 72 * Sab * Scd;
     }
 
     // Schwarz factors Q_ab = sqrt(max |(ab|ab)|) per pair of every main class; pairs sorted by
@@ -460,6 +550,128 @@ struct lbx_context : Context
         a.D = D;
         a.J = J;
         a.K = K;
+        if (sample_quartets > 0)
+        {
+            // leading uniform-side positions up to about sample_quartets quartets
+            const long long cap = std::max(1LL, sample_quartets / s.width);
+            long long nq = 0;
+            int q = 0;
+            while (q < t.nket && t.prefix[q] < cap)
+            {
+                nq += t.phi[q] - t.plo[q];
+                q++;
+            }
+            a.ntasks = t.prefix[q];
+            sampled = static_cast<double>(nq);
+        }
+        s.fn(a, stream);
+        LBX_HIP(hipGetLastError());
+        return t;
+    }
+
+    // when > 0, launchJK runs only the first ~sample_quartets quartets (autotuner timing) and
+    // leaves their count in `sampled`
+    long long sample_quartets = 0;
+    double sampled = 0;
+
+    // gradient task tables per (bra class, ket class, Schwarz threshold)
+    std::map<std::tuple<int, int, double>, TaskTable> grad_tasks;
+
+    // batched RI sums: the whole launch sequence captured once into a graph (per output buffer)
+    hipGraph_t ri_graph = nullptr;
+    hipGraphExec_t ri_exec = nullptr;
+    double *ri_out = nullptr;
+    double ri_triplets = 0, ri_doublets = 0, ri_flops = 0;
+
+    void dropRiGraph()
+    {
+        if (ri_exec)
+            (void)hipGraphExecDestroy(ri_exec);
+        if (ri_graph)
+            (void)hipGraphDestroy(ri_graph);
+        ri_exec = nullptr;
+        ri_graph = nullptr;
+        ri_out = nullptr;
+    }
+
+    // Kernel launches of the RI sums: every eri3 (pair class | aux l) and eri2
+    // (aux l | aux l' <= l) class pair, with task tables built once (device memory cannot be
+    // allocated while a stream is being captured).
+    std::vector<std::pair<Launcher, KArgs>> ri_launches;
+    bool ri_prepared = false;
+
+    void prepareRiSums()
+    {
+        if (ri_prepared)
+            return;
+        ri_triplets = ri_doublets = ri_flops = 0;
+        auto add = [&](const PairClass &lane, const PairClass &unif, bool same, int kind)
+        {
+            if (lane.n_pairs == 0 || unif.n_pairs == 0)
+                return;
+            int var = VAR_VALU_BRA;
+            if (!kernelSlot(MODE_SUM, lane.la, lane.lb, unif.la, unif.lb, var).fn)
+                var = VAR_MFMA;
+            const KernelSlot &ks = slotFor(MODE_SUM, lane, unif, var);
+            TaskTable t = buildTasksMol(*this, lane, unif, same, ks.width);
+            KArgs a = baseArgs(*this, lane, unif, t, same);
+            a.kind = kind;
+            ri_launches.emplace_back(ks.fn, a);
+            (kind == 3 ? ri_triplets : ri_doublets) += t.quartets;
+            ri_flops += t.flops;
+        };
+        for (auto &pc : cls)
+            for (auto &ax : aux)
+                add(pc, ax, false, 3);
+        for (int la = 0; la < static_cast<int>(aux.size()); la++)
+            for (int lc = 0; lc <= la; lc++)
+                add(aux[la], aux[lc], la == lc, 2);
+        ri_prepared = true;
+    }
+
+    // fill + launches on stream s (the capture stream)
+    void queueRiSums(double *out, hipStream_t s)
+    {
+        fill(out, 4LL * nmol, 0.0, s);
+        for (auto &l : ri_launches)
+        {
+            KArgs a = l.second;
+            a.out = out;
+            l.first(a, s);
+            LBX_HIP(hipGetLastError());
+        }
+    }
+
+    // Queues the gradient kernel of class pair (i >= j); returns its task table.
+    const TaskTable &launchGrad(int i, int j, const double *D, double *grad, double thresh)
+    {
+        const PairClass &ci = cls[i], &cj = cls[j];
+        const bool lane_i = gradLaneFirst(ci.la, ci.lb, cj.la, cj.lb);
+        const PairClass &lane = lane_i ? ci : cj, &unif = lane_i ? cj : ci;
+        const GradSlot &s = gradSlot(lane.la, lane.lb, unif.la, unif.lb);
+        if (!s.fn)
+            throw Error(LBX_ERR_INVALID, "no gradient kernel for this class pair");
+        const auto key = std::make_tuple(i, j, thresh);
+        auto it = grad_tasks.find(key);
+        if (it == grad_tasks.end())
+            it = grad_tasks
+                         .emplace(key, buildTasks(*this, lane, unif, Pairing::Unique, i == j, thresh, s.width,
+                                                  lane_i))
+                         .first;
+        const TaskTable &t = it->second;
+        GArgs a{};
+        a.lane = lane.view();
+        a.unif = unif.view();
+        a.prefix = t.d_prefix;
+        a.plo = t.d_plo;
+        a.phi = t.d_phi;
+        a.nket = t.nket;
+        a.ntasks = t.ntasks;
+        a.same_class = i == j ? 1 : 0;
+        a.boys = d_boys;
+        a.D = D;
+        a.nbf = static_cast<long long>(nbf);
+        a.grad = grad;
         s.fn(a, stream);
         LBX_HIP(hipGetLastError());
         return t;
@@ -477,6 +689,7 @@ struct lbx_context : Context
 
     ~lbx_context()
     {
+        dropRiGraph();
         for (auto &kv : prof)
         {
             if (kv.second.start)
@@ -501,6 +714,76 @@ struct lbx_context : Context
     }
 };
 
+namespace
+{
+    // Shells, pair classes, E matrices and per-pair molecule ids of a new context.
+    void setupContext(lbx_context &ctx, const lbx_basis &main_basis, const lbx_basis *aux_basis)
+    {
+        Context &c = ctx;
+        LBX_HIP(hipGetDevice(&c.device));
+        c.natoms = main_basis.natoms;
+        c.shells = buildShells(main_basis, c.nbf, c.max_l, kMaxLMain);
+        if (c.shells.empty())
+            throw Error(LBX_ERR_INVALID, "lbx_create: empty main basis");
+        if (aux_basis)
+        {
+            if (aux_basis->natoms != c.natoms)
+                throw Error(LBX_ERR_INVALID, "lbx_create: aux basis natoms differs");
+            c.shells_aux = buildShells(*aux_basis, c.nbf_aux, c.max_l_aux, kMaxLAux);
+        }
+        c.d_boys = const_cast<double *>(boysTableDevice());
+        buildPairClasses(c);
+        for (auto &pc : c.cls)
+        {
+            computeEcoeffs(c, pc, c.shells);
+            pc.d_mol = c.upload(pc.mol);
+        }
+        for (auto &pc : c.aux)
+        {
+            computeEcoeffs(c, pc, c.shells_aux);
+            pc.d_mol = c.upload(pc.mol);
+        }
+        LBX_HIP(hipStreamSynchronize(c.stream));
+    }
+
+    // Concatenation of several molecules' lbx_basis arrays (atom indices offset).
+    struct FlatBatch
+    {
+        std::vector<double> coords, exps, coeffs;
+        std::vector<int> atom, l, nprim;
+        int natoms = 0;
+        lbx_basis b{};
+
+        void add(const lbx_basis &x)
+        {
+            if (x.natoms < 0 || x.nshells < 0 || (x.natoms > 0 && !x.coords) ||
+                (x.nshells > 0 && (!x.shell_atom || !x.shell_l || !x.shell_nprim || !x.exps || !x.coeffs)))
+                throw Error(LBX_ERR_INVALID, "lbx_create_batch: bad basis");
+            coords.insert(coords.end(), x.coords, x.coords + 3 * x.natoms);
+            long long np = 0;
+            for (int s = 0; s < x.nshells; s++)
+            {
+                if (x.shell_atom[s] < 0 || x.shell_atom[s] >= x.natoms || x.shell_nprim[s] <= 0)
+                    throw Error(LBX_ERR_INVALID, "lbx_create_batch: bad shell");
+                atom.push_back(x.shell_atom[s] + natoms);
+                l.push_back(x.shell_l[s]);
+                nprim.push_back(x.shell_nprim[s]);
+                np += x.shell_nprim[s];
+            }
+            exps.insert(exps.end(), x.exps, x.exps + np);
+            coeffs.insert(coeffs.end(), x.coeffs, x.coeffs + np);
+            natoms += x.natoms;
+        }
+
+        const lbx_basis &view()
+        {
+            b = lbx_basis{natoms, coords.data(), static_cast<int>(l.size()), atom.data(), l.data(),
+                          nprim.data(), exps.data(), coeffs.data()};
+            return b;
+        }
+    };
+}
+
 extern "C"
 {
     const char *lbx_last_error(void) { return g_err.c_str(); }
@@ -514,27 +797,41 @@ extern "C"
         }
         *out = nullptr;
         auto ctx = std::make_unique<lbx_context>();
+        int rc = guarded([&] { setupContext(*ctx, *main_basis, aux_basis); });
+        if (rc != LBX_OK)
+        {
+            ctx->release();
+            return rc;
+        }
+        *out = ctx.release();
+        return LBX_OK;
+    }
+
+    int lbx_create_batch(lbx_context **out, int nmol, const lbx_basis *main_bases,
+                         const lbx_basis *aux_bases)
+    {
+        if (!out || !main_bases || !aux_bases || nmol <= 0)
+        {
+            g_err = "lbx_create_batch: null argument or nmol <= 0";
+            return LBX_ERR_INVALID;
+        }
+        *out = nullptr;
+        auto ctx = std::make_unique<lbx_context>();
         int rc = guarded([&]
                          {
-                             Context &c = *ctx;
-                             LBX_HIP(hipGetDevice(&c.device));
-                             c.natoms = main_basis->natoms;
-                             c.shells = buildShells(*main_basis, c.nbf, c.max_l, kMaxLMain);
-                             if (c.shells.empty())
-                                 throw Error(LBX_ERR_INVALID, "lbx_create: empty main basis");
-                             if (aux_basis)
+                             // one basis over all molecules' atoms, atoms grouped by molecule
+                             FlatBatch m, a;
+                             for (int k = 0; k < nmol; k++)
                              {
-                                 if (aux_basis->natoms != c.natoms)
-                                     throw Error(LBX_ERR_INVALID, "lbx_create: aux basis natoms differs");
-                                 c.shells_aux = buildShells(*aux_basis, c.nbf_aux, c.max_l_aux, kMaxLAux);
+                                 if (main_bases[k].natoms != aux_bases[k].natoms)
+                                     throw Error(LBX_ERR_INVALID, "lbx_create_batch: aux basis natoms differs");
+                                 for (int i = 0; i < main_bases[k].natoms; i++)
+                                     ctx->atom_mol.push_back(k);
+                                 m.add(main_bases[k]);
+                                 a.add(aux_bases[k]);
                              }
-                             c.d_boys = const_cast<double *>(boysTableDevice());
-                             buildPairClasses(c);
-                             for (auto &pc : c.cls)
-                                 computeEcoeffs(c, pc, c.shells);
-                             for (auto &pc : c.aux)
-                                 computeEcoeffs(c, pc, c.shells_aux);
-                             LBX_HIP(hipStreamSynchronize(c.stream));
+                             ctx->nmol = nmol;
+                             setupContext(*ctx, m.view(), &a.view());
                          });
         if (rc != LBX_OK)
         {
@@ -617,6 +914,7 @@ extern "C"
                                throw Error(LBX_ERR_INVALID, "lbx_eri4: null argument");
                            std::lock_guard<std::mutex> lk(ctx->mu);
                            Context &c = *ctx;
+                           requireSingle(c, "lbx_eri4");
                            const long long n = static_cast<long long>(c.nbf);
                            fill(out, n * n * n * n, 0.0, c.stream);
                            KArgs extra{};
@@ -636,6 +934,7 @@ extern "C"
                                throw Error(LBX_ERR_INVALID, "lbx_eri4_diag: null argument");
                            std::lock_guard<std::mutex> lk(ctx->mu);
                            Context &c = *ctx;
+                           requireSingle(c, "lbx_eri4_diag");
                            const long long n = static_cast<long long>(c.nbf);
                            fill(out, n * n, 0.0, c.stream);
                            for (auto &pc : c.cls)
@@ -660,6 +959,7 @@ extern "C"
                                throw Error(LBX_ERR_INVALID, "lbx_eri3: null argument");
                            std::lock_guard<std::mutex> lk(ctx->mu);
                            Context &c = *ctx;
+                           requireSingle(c, "lbx_eri3");
                            if (c.shells_aux.empty())
                                throw Error(LBX_ERR_STATE, "eri3(): RI approximation is not enabled");
                            const long long n = static_cast<long long>(c.nbf);
@@ -681,6 +981,7 @@ extern "C"
                                throw Error(LBX_ERR_INVALID, "lbx_eri2: null argument");
                            std::lock_guard<std::mutex> lk(ctx->mu);
                            Context &c = *ctx;
+                           requireSingle(c, "lbx_eri2");
                            if (c.shells_aux.empty())
                                throw Error(LBX_ERR_STATE, "eri2(): RI approximation is not enabled");
                            const long long n = static_cast<long long>(c.nbf_aux);
@@ -702,6 +1003,7 @@ extern "C"
                                throw Error(LBX_ERR_INVALID, "lbx_eri2_diag: null argument");
                            std::lock_guard<std::mutex> lk(ctx->mu);
                            Context &c = *ctx;
+                           requireSingle(c, "lbx_eri2_diag");
                            if (c.shells_aux.empty())
                                throw Error(LBX_ERR_STATE, "eri2Diagonal(): RI approximation is not enabled");
                            fill(out, static_cast<long long>(c.nbf_aux), 0.0, c.stream);
@@ -734,6 +1036,7 @@ extern "C"
                                throw Error(LBX_ERR_INVALID, "lbx_jk: negative mixed-precision threshold");
                            std::lock_guard<std::mutex> lk(ctx->mu);
                            Context &c = *ctx;
+                           requireSingle(c, "lbx_jk");
                            ensureSchwarz(c);
                            // quartets with schwarz <= Q_ab Q_cd < split run in FP32
                            const bool mixed = precision == LBX_MIXED && mixed_thresh > schwarz_thresh;
@@ -802,6 +1105,99 @@ extern "C"
                        });
     }
 
+    int lbx_ri_sums(lbx_context *ctx, double *out, lbx_ri_stats *stats)
+    {
+        return guarded([&]
+                       {
+                           if (!ctx || !out)
+                               throw Error(LBX_ERR_INVALID, "lbx_ri_sums: null argument");
+                           std::lock_guard<std::mutex> lk(ctx->mu);
+                           Context &c = *ctx;
+                           if (c.shells_aux.empty())
+                               throw Error(LBX_ERR_STATE, "lbx_ri_sums: the context has no auxiliary basis");
+                           if (!ctx->ri_exec || ctx->ri_out != out)
+                           {
+                               ctx->dropRiGraph();
+                               ctx->prepareRiSums();
+                               // capture on a private stream (default priority), replay on the
+                               // context stream
+                               hipStream_t cs = nullptr;
+                               LBX_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+                               try
+                               {
+                                   LBX_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+                                   try
+                                   {
+                                       ctx->queueRiSums(out, cs);
+                                   }
+                                   catch (...)
+                                   {
+                                       hipGraph_t g = nullptr;
+                                       (void)hipStreamEndCapture(cs, &g);
+                                       if (g)
+                                           (void)hipGraphDestroy(g);
+                                       throw;
+                                   }
+                                   LBX_HIP(hipStreamEndCapture(cs, &ctx->ri_graph));
+                                   LBX_HIP(hipGraphInstantiate(&ctx->ri_exec, ctx->ri_graph, nullptr, nullptr, 0));
+                               }
+                               catch (...)
+                               {
+                                   (void)hipStreamDestroy(cs);
+                                   ctx->dropRiGraph();
+                                   throw;
+                               }
+                               (void)hipStreamDestroy(cs);
+                               ctx->ri_out = out;
+                           }
+                           LBX_HIP(hipGraphLaunch(ctx->ri_exec, c.stream));
+                           LBX_HIP(hipStreamSynchronize(c.stream));
+                           if (stats)
+                           {
+                               stats->triplets = ctx->ri_triplets;
+                               stats->doublets = ctx->ri_doublets;
+                               stats->flops = ctx->ri_flops;
+                           }
+                       });
+    }
+
+    int lbx_jk_grad(lbx_context *ctx, const double *D, double *grad, double schwarz_thresh,
+                    lbx_jk_stats *stats)
+    {
+        return guarded([&]
+                       {
+                           if (!ctx || !D || !grad)
+                               throw Error(LBX_ERR_INVALID, "lbx_jk_grad: null argument");
+                           if (!(schwarz_thresh >= 0))
+                               throw Error(LBX_ERR_INVALID, "lbx_jk_grad: negative Schwarz threshold");
+                           std::lock_guard<std::mutex> lk(ctx->mu);
+                           Context &c = *ctx;
+                           requireSingle(c, "lbx_jk_grad");
+                           ensureSchwarz(c);
+                           for (auto &pc : c.cls)
+                               computeEcoeffsD1(c, pc);
+                           fill(grad, 3LL * c.natoms, 0.0, c.stream);
+                           double nq = 0, fl = 0;
+                           for (int i = 0; i < static_cast<int>(c.cls.size()); i++)
+                               for (int j = 0; j <= i; j++)
+                               {
+                                   const PairClass &bra = c.cls[i], &ket = c.cls[j];
+                                   if (bra.n_pairs == 0 || ket.n_pairs == 0)
+                                       continue;
+                                   const TaskTable &t = ctx->launchGrad(i, j, D, grad, schwarz_thresh);
+                                   nq += t.quartets;
+                                   fl += gradFlops(t, bra.la, bra.lb, ket.la, ket.lb);
+                               }
+                           LBX_HIP(hipStreamSynchronize(c.stream));
+                           if (stats)
+                           {
+                               stats->quartets = nq;
+                               stats->flops = fl;
+                               stats->quartets_fp32 = 0;
+                           }
+                       });
+    }
+
     int lbx_tune(lbx_context *ctx, const double *D, double schwarz_thresh)
     {
         return guarded([&]
@@ -812,6 +1208,7 @@ extern "C"
                                throw Error(LBX_ERR_INVALID, "lbx_tune: negative Schwarz threshold");
                            std::lock_guard<std::mutex> lk(ctx->mu);
                            Context &c = *ctx;
+                           requireSingle(c, "lbx_tune");
                            ensureSchwarz(c);
                            const long long n = static_cast<long long>(c.nbf);
                            DevBuf J(n * n), K(n * n);
@@ -827,23 +1224,28 @@ extern "C"
                                        if (bra.n_pairs == 0 || ket.n_pairs == 0)
                                            continue;
                                        int best = -1, best32 = -1;
-                                       float best_ms = 0, best32_ms = 0;
+                                       double best_ms = 0, best32_ms = 0;
                                        for (int v = 0; v < N_VARIANTS; v++)
                                        {
                                            if (!kernelSlot(MODE_JK, bra.la, bra.lb, ket.la, ket.lb, v).fn)
                                                continue;
                                            if (v >= N_FP64_VARIANTS)
                                                ensureF32(c);
-                                           // first launch builds the task table and loads the code
+                                           // first launch builds the task table and loads the code;
+                                           // the timed one runs a sample of leading quartets
+                                           ctx->sample_quartets = kTuneSample;
                                            ctx->launchJK(static_cast<int>(i), static_cast<int>(j), v, D, J.p, K.p,
                                                          schwarz_thresh);
                                            LBX_HIP(hipEventRecord(e0, c.stream));
                                            ctx->launchJK(static_cast<int>(i), static_cast<int>(j), v, D, J.p, K.p,
                                                          schwarz_thresh);
                                            LBX_HIP(hipEventRecord(e1, c.stream));
+                                           ctx->sample_quartets = 0;
                                            LBX_HIP(hipEventSynchronize(e1));
-                                           float ms = 0;
-                                           LBX_HIP(hipEventElapsedTime(&ms, e0, e1));
+                                           float ev = 0;
+                                           LBX_HIP(hipEventElapsedTime(&ev, e0, e1));
+                                           // time per quartet: the variants' samples differ
+                                           const double ms = ev / std::max(ctx->sampled, 1.0);
                                            if (v < N_FP64_VARIANTS && (best < 0 || ms < best_ms))
                                            {
                                                best = v;
@@ -863,6 +1265,7 @@ extern "C"
                            }
                            catch (...)
                            {
+                               ctx->sample_quartets = 0;
                                (void)hipEventDestroy(e0);
                                (void)hipEventDestroy(e1);
                                throw;

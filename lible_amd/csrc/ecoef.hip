@@ -23,12 +23,6 @@ namespace lbx
         };
         __constant__ SphConst cSph;
 
-        struct PrimIn
-        {
-            double a, b, dadb;
-            int pair;
-        };
-
         __device__ __forceinline__ void cartOf(int l, int idx, int &i, int &j, int &k)
         {
             // inverse of cartIdx within shell l: idx = s(s+1)/2 + k with s = j + k
@@ -71,6 +65,123 @@ namespace lbx
                             v += (t + 1) * E[i][j - 1][t + 1];
                         E[i][j][t] = v;
                     }
+        }
+
+        // The same coefficients and their derivatives with respect to X = A_x - B_x.  E depends
+        // on X through K = exp(-mu X^2), PA = -(b/p) X and PB = (a/p) X, so differentiating the
+        // recurrences term by term gives
+        //   dE^{i+1,0}_t = -(b/p) E^{i0}_t + PA dE^{i0}_t + dE^{i0}_{t-1}/(2p) + (t+1) dE^{i0}_{t+1}
+        //   dE^{i,j+1}_t =  (a/p) E^{ij}_t + PB dE^{ij}_t + dE^{ij}_{t-1}/(2p) + (t+1) dE^{ij}_{t+1}
+        // from dE^{00}_0 = -2 mu X K.
+        template <int LA, int LB>
+        __device__ __forceinline__ void hermite1dD1(double (&E)[LA + 1][LB + 1][LA + LB + 1],
+                                                    double (&dE)[LA + 1][LB + 1][LA + LB + 1],
+                                                    double X, double mur, double a, double b)
+        {
+            const double p = a + b, o2p = 1.0 / (2.0 * p), K = exp(-mur * X * X);
+            const double PA = -(b / p) * X, PB = (a / p) * X, dPA = -b / p, dPB = a / p;
+            for (int i = 0; i <= LA; i++)
+                for (int j = 0; j <= LB; j++)
+                    for (int t = 0; t <= LA + LB; t++)
+                    {
+                        E[i][j][t] = 0.0;
+                        dE[i][j][t] = 0.0;
+                    }
+            E[0][0][0] = K;
+            dE[0][0][0] = -2.0 * mur * X * K;
+            for (int i = 1; i <= LA; i++)
+                for (int t = 0; t <= i; t++)
+                {
+                    double v = PA * E[i - 1][0][t], dv = dPA * E[i - 1][0][t] + PA * dE[i - 1][0][t];
+                    if (t > 0)
+                    {
+                        v += o2p * E[i - 1][0][t - 1];
+                        dv += o2p * dE[i - 1][0][t - 1];
+                    }
+                    if (t + 1 <= i - 1)
+                    {
+                        v += (t + 1) * E[i - 1][0][t + 1];
+                        dv += (t + 1) * dE[i - 1][0][t + 1];
+                    }
+                    E[i][0][t] = v;
+                    dE[i][0][t] = dv;
+                }
+            for (int j = 1; j <= LB; j++)
+                for (int i = 0; i <= LA; i++)
+                    for (int t = 0; t <= i + j; t++)
+                    {
+                        double v = PB * E[i][j - 1][t], dv = dPB * E[i][j - 1][t] + PB * dE[i][j - 1][t];
+                        if (t > 0)
+                        {
+                            v += o2p * E[i][j - 1][t - 1];
+                            dv += o2p * dE[i][j - 1][t - 1];
+                        }
+                        if (t + 1 <= i + j - 1)
+                        {
+                            v += (t + 1) * E[i][j - 1][t + 1];
+                            dv += (t + 1) * dE[i][j - 1][t + 1];
+                        }
+                        E[i][j][t] = v;
+                        dE[i][j][t] = dv;
+                    }
+        }
+
+        // d E[tuv][mu nu] / d(A - B)_x for x = 0, 1, 2, one thread per (primitive pair, mu nu);
+        // written [ppair][x][tuv][mu nu], unsigned (e1b) and with (-1)^{t+u+v} (e1k).
+        template <int LA, int LB>
+        __global__ __launch_bounds__(256) void ecoefD1Kernel(int nppairs, const PrimIn *__restrict__ prim,
+                                                             const double *__restrict__ centers,
+                                                             const double *__restrict__ norms,
+                                                             double *__restrict__ e1b,
+                                                             double *__restrict__ e1k)
+        {
+            constexpr int SA = nSph(LA), SB = nSph(LB), SP = SA * SB, HP = nHerm(LA + LB);
+            const long long gid = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+            if (gid >= static_cast<long long>(nppairs) * SP)
+                return;
+            const int pp = static_cast<int>(gid / SP), mn = static_cast<int>(gid % SP);
+            const int mu = mn / SB, nu = mn % SB;
+            const PrimIn pr = prim[pp];
+            const double a = pr.a, b = pr.b, mur = a * b / (a + b);
+            const double *C = centers + 6 * pr.pair;
+            double E[3][LA + 1][LB + 1][LA + LB + 1], dE[3][LA + 1][LB + 1][LA + LB + 1];
+            for (int x = 0; x < 3; x++)
+                hermite1dD1<LA, LB>(E[x], dE[x], C[x] - C[3 + x], mur, a, b);
+            const double *nrm = norms + static_cast<size_t>(pr.pair) * (SA + SB);
+            const double sc = nrm[mu] * nrm[SA + nu] * pr.dadb;
+            for (int x = 0; x < 3; x++)
+            {
+                double acc[HP];
+                for (int h = 0; h < HP; h++)
+                    acc[h] = 0.0;
+                for (int ea = cSph.start[LA][mu]; ea < cSph.start[LA][mu + 1]; ea++)
+                {
+                    int ia[3];
+                    cartOf(LA, cSph.cart[ea], ia[0], ia[1], ia[2]);
+                    for (int eb_ = cSph.start[LB][nu]; eb_ < cSph.start[LB][nu + 1]; eb_++)
+                    {
+                        int jb[3];
+                        cartOf(LB, cSph.cart[eb_], jb[0], jb[1], jb[2]);
+                        const double v = cSph.val[ea] * cSph.val[eb_];
+                        for (int t = 0; t <= ia[0] + jb[0]; t++)
+                            for (int u = 0; u <= ia[1] + jb[1]; u++)
+                                for (int w = 0; w <= ia[2] + jb[2]; w++)
+                                {
+                                    const double fx = x == 0 ? dE[0][ia[0]][jb[0]][t] : E[0][ia[0]][jb[0]][t];
+                                    const double fy = x == 1 ? dE[1][ia[1]][jb[1]][u] : E[1][ia[1]][jb[1]][u];
+                                    const double fz = x == 2 ? dE[2][ia[2]][jb[2]][w] : E[2][ia[2]][jb[2]][w];
+                                    acc[hermIdx(t, u, w)] += v * fx * fy * fz;
+                                }
+                    }
+                }
+                for (int h = 0; h < HP; h++)
+                {
+                    const double val = sc * acc[h];
+                    const size_t o = ((static_cast<size_t>(pp) * 3 + x) * HP + h) * SP + mn;
+                    e1b[o] = val;
+                    e1k[o] = (hermLevel(h) & 1) ? -val : val;
+                }
+            }
         }
 
         template <int LA, int LB>
@@ -146,6 +257,30 @@ namespace lbx
             const int grid = static_cast<int>((total + 255) / 256);
             hipLaunchKernelGGL((ecoefKernel<LA, LB>), dim3(grid), dim3(256), 0, s, n, prim, centers,
                                norms, eb, ek);
+        }
+
+        template <int LA, int LB>
+        void launchEcoefD1(int n, const PrimIn *prim, const double *centers, const double *norms,
+                           double *e1b, double *e1k, hipStream_t s)
+        {
+            constexpr int SP = nSph(LA) * nSph(LB);
+            const long long total = static_cast<long long>(n) * SP;
+            if (total == 0)
+                return;
+            const int grid = static_cast<int>((total + 255) / 256);
+            hipLaunchKernelGGL((ecoefD1Kernel<LA, LB>), dim3(grid), dim3(256), 0, s, n, prim, centers,
+                               norms, e1b, e1k);
+        }
+
+        EcoefFn ecoefD1Fn(int la, int lb)
+        {
+#define LBX_E(A, B)                                                                                \
+    if (la == A && lb == B)                                                                        \
+        return &launchEcoefD1<A, B>;
+            LBX_E(0, 0) LBX_E(1, 0) LBX_E(1, 1) LBX_E(2, 0) LBX_E(2, 1) LBX_E(2, 2) LBX_E(3, 0)
+            LBX_E(3, 1) LBX_E(3, 2) LBX_E(3, 3)
+#undef LBX_E
+            throw Error(LBX_ERR_INVALID, "no derivative E-matrix kernel for this angular momentum pair");
         }
 
         EcoefFn ecoefFn(int la, int lb)
@@ -247,10 +382,31 @@ namespace lbx
         pc.d_ek = c.alloc<double>(ne);
         if (pc.n_ppairs == 0)
             return;
-        PrimIn *d_prim = c.upload(prim);
-        double *d_centers = c.upload(centers);
-        double *d_norms = c.upload(norms);
-        ecoefFn(pc.la, pc.lb)(pc.n_ppairs, d_prim, d_centers, d_norms, pc.d_eb, pc.d_ek, c.stream);
+        pc.d_prim = c.upload(prim);
+        pc.d_centers = c.upload(centers);
+        pc.d_norms = c.upload(norms);
+        ecoefFn(pc.la, pc.lb)(pc.n_ppairs, pc.d_prim, pc.d_centers, pc.d_norms, pc.d_eb, pc.d_ek,
+                              c.stream);
+        LBX_HIP(hipGetLastError());
+    }
+
+    void computeEcoeffsD1(Context &c, PairClass &pc)
+    {
+        if (pc.d_e1b || pc.n_pairs == 0)
+            return;
+        pc.d_atom_a = c.upload(pc.atom_a);
+        pc.d_atom_b = c.upload(pc.atom_b);
+        std::vector<PrimIn> prim(pc.n_ppairs);
+        LBX_HIP(hipMemcpy(prim.data(), pc.d_prim, prim.size() * sizeof(PrimIn), hipMemcpyDeviceToHost));
+        std::vector<double> ra(pc.n_ppairs);
+        for (int o = 0; o < pc.n_ppairs; o++)
+            ra[o] = prim[o].a / (prim[o].a + prim[o].b);
+        pc.d_ra = c.upload(ra);
+        const size_t ne = 3 * pc.eCount();
+        pc.d_e1b = c.alloc<double>(ne);
+        pc.d_e1k = c.alloc<double>(ne);
+        ecoefD1Fn(pc.la, pc.lb)(pc.n_ppairs, pc.d_prim, pc.d_centers, pc.d_norms, pc.d_e1b, pc.d_e1k,
+                                c.stream);
         LBX_HIP(hipGetLastError());
     }
 }

@@ -212,6 +212,21 @@ namespace lbx
         std::map<int, std::vector<const HShell *>> byl;
         for (const auto &s : c.shells)
             byl[s.l].push_back(&s);
+        // molecule of a shell, and per l the range of byl[l] holding each molecule's shells
+        // (shells follow atom order, atoms are grouped by molecule)
+        const bool batched = !c.atom_mol.empty();
+        auto molOf = [&](const HShell &s) { return batched ? c.atom_mol[s.atom] : 0; };
+        std::map<int, std::vector<size_t>> mol_lo, mol_hi;
+        for (auto &kv : byl)
+        {
+            auto &lo = mol_lo[kv.first], &hi = mol_hi[kv.first];
+            lo.assign(c.nmol, 0);
+            hi.assign(c.nmol, 0);
+            for (size_t i = kv.second.size(); i-- > 0;)
+                lo[molOf(*kv.second[i])] = i;
+            for (size_t i = 0; i < kv.second.size(); i++)
+                hi[molOf(*kv.second[i])] = i + 1;
+        }
         c.cls.assign(pairClassIdx(kMaxLMain, kMaxLMain) + 1, PairClass{});
         for (int la = 0; la <= c.max_l; la++)
             for (int lb = 0; lb <= la; lb++)
@@ -224,8 +239,16 @@ namespace lbx
                 int nprim = 0;
                 for (size_t i = 0; i < A.size(); i++)
                 {
-                    size_t jend = (la == lb) ? i + 1 : B.size();
-                    for (size_t j = 0; j < jend; j++)
+                    size_t jbeg = 0, jend = (la == lb) ? i + 1 : B.size();
+                    if (batched)
+                    {
+                        const int m = molOf(*A[i]);
+                        jbeg = mol_lo[lb][m];
+                        jend = (la == lb) ? i + 1 : mol_hi[lb][m];
+                        if (mol_hi[lb][m] == 0)
+                            jend = jbeg;   // no shell of this l in the molecule
+                    }
+                    for (size_t j = jbeg; j < jend; j++)
                     {
                         const HShell &sa = *A[i], &sb = *B[j];
                         double r2 = 0;
@@ -252,6 +275,7 @@ namespace lbx
                         pc.sh_b.push_back(static_cast<int>(sb.idx));
                         pc.atom_a.push_back(sa.atom);
                         pc.atom_b.push_back(sb.atom);
+                        pc.mol.push_back(molOf(sa));
                         pc.poff.push_back(nprim);
                         pc.pcnt.push_back(surv);
                         pc.kmax = std::max(pc.kmax, surv);
@@ -279,6 +303,7 @@ namespace lbx
                 pc.sh_b.push_back(-1);
                 pc.atom_a.push_back(s.atom);
                 pc.atom_b.push_back(s.atom);
+                pc.mol.push_back(molOf(s));
                 pc.poff.push_back(nprim);
                 int K = static_cast<int>(s.exps.size());
                 for (int x = 0; x < K; x++)

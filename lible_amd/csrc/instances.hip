@@ -53,13 +53,19 @@ namespace lbx
         {
             if constexpr (!kF32Unit && kPairUnit < 10 && !(LC <= 3 && pairClassIdx(LC, 0) <= CI))
                 registerQuartet<PA, PB, LC, 0, MODE_DENSE, 1>();
+            // batched RI sums: eri3 (PA PB | LC), and eri2 (PA | LC) when PB = 0
+            if constexpr (!kF32Unit && kPairUnit < 10)
+                registerQuartet<PA, PB, LC, 0, MODE_SUM, 1>();
         }
 
         template <int LC>
         void regEri2()
         {
             if constexpr (!kF32Unit && kPairUnit >= 10 && LC <= PA)
+            {
                 registerQuartet<PA, 0, LC, 0, MODE_DENSE, 1>();
+                registerQuartet<PA, 0, LC, 0, MODE_SUM, 1>();
+            }
         }
 
         struct Registrar

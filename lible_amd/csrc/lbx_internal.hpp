@@ -54,6 +54,18 @@ namespace lbx
         const double *q;            // Schwarz factor per sorted position
         int n;
         const float *eb32, *ek32;   // eb, ek rounded to FP32 (mixed precision; may be null)
+        // nuclear gradient (may be null until the first gradient call)
+        const int *atom_a, *atom_b;   // atom of each shell, per pair
+        const double *ra;             // a / (a + b) per primitive pair
+        const double *e1b, *e1k;      // d eb / d(A - B)_x, d ek / d(A - B)_x: [ppair][x][tuv][mu nu]
+        const int *mol;               // molecule of each pair (batched contexts; may be null)
+    };
+
+    // Per primitive pair input of the E-matrix kernels.
+    struct PrimIn
+    {
+        double a, b, dadb;   // exponents, product of the normalised contraction coefficients
+        int pair;
     };
 
     // Pairs of shells (la >= lb), or single shells (lb = 0, single = true: an auxiliary shell
@@ -66,6 +78,7 @@ namespace lbx
         int n_pairs = 0, n_ppairs = 0, kmax = 0;
         // host
         std::vector<int> ao_a, ao_b, sh_a, sh_b, atom_a, atom_b, poff, pcnt;
+        std::vector<int> mol;              // molecule per pair (batched contexts)
         std::vector<int> prim_x, prim_y;   // surviving primitive pair -> primitive of a, of b
         std::vector<double> Q;          // Schwarz factor per pair (pair order)
         std::vector<int> order;         // sorted by Q descending
@@ -75,6 +88,13 @@ namespace lbx
         double4 *d_pp = nullptr;
         double *d_eb = nullptr, *d_ek = nullptr, *d_q = nullptr;
         float *d_eb32 = nullptr, *d_ek32 = nullptr;   // made on first mixed-precision use
+        // E-matrix inputs kept for the derivative matrices
+        PrimIn *d_prim = nullptr;
+        double *d_centers = nullptr, *d_norms = nullptr;
+        // made on first gradient use
+        int *d_atom_a = nullptr, *d_atom_b = nullptr;
+        double *d_ra = nullptr, *d_e1b = nullptr, *d_e1k = nullptr;
+        int *d_mol = nullptr;
 
         // doubles in each of d_eb, d_ek
         size_t eCount() const
@@ -85,7 +105,8 @@ namespace lbx
         DPairs view() const
         {
             return DPairs{d_ao_a, d_ao_b, d_sh_a, d_sh_b, d_poff, d_pcnt, d_pp, d_eb, d_ek,
-                          d_order, d_q, n_pairs, d_eb32, d_ek32};
+                          d_order, d_q, n_pairs, d_eb32, d_ek32, d_atom_a, d_atom_b, d_ra,
+                          d_e1b, d_e1k, d_mol};
         }
     };
 
@@ -102,6 +123,10 @@ namespace lbx
         std::vector<PairClass> aux;      // index l
         double *d_boys = nullptr;        // Boys table kBoysRows x kBoysCols
         bool have_schwarz = false;
+        // batched contexts: molecule of each atom (atoms grouped by molecule); pairs only form
+        // within a molecule.  Empty for an ordinary (one-molecule) context.
+        std::vector<int> atom_mol;
+        int nmol = 1;
         std::vector<void *> allocations;
 
         template <class T>
@@ -142,6 +167,9 @@ namespace lbx
 
     // ecoef.hip
     void computeEcoeffs(Context &c, PairClass &pc, const std::vector<HShell> &shells);
+    // d E / d (A - B)_x,y,z and the per-pair atoms and a/p ratios for the nuclear gradient
+    // (once per pair class)
+    void computeEcoeffsD1(Context &c, PairClass &pc);
 
     // schwarz / task tables (capi.cpp)
     void ensureSchwarz(Context &c);

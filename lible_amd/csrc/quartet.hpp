@@ -36,7 +36,9 @@ namespace lbx
     {
         MODE_JK = 0,
         MODE_DENSE = 1,
-        MODE_DIAG = 2
+        MODE_DIAG = 2,
+        MODE_SUM = 3,   // per-molecule sums of |value| and value of the dense eri3 / eri2 tensors
+        N_MODES = 4
     };
 
     struct KArgs
@@ -53,6 +55,7 @@ namespace lbx
         double *J, *K;
         long long nbf;
         // MODE_DENSE: kind 4 eri4 (nbf^4), 3 eri3 (nbf, nbf, naux), 2 eri2 (naux, naux)
+        // MODE_SUM: out = (nmol, 4) sums |eri3|, eri3, |eri2|, eri2 by the ket's molecule
         double *out;
         int kind;
         long long naux;
@@ -210,6 +213,7 @@ namespace lbx
         for (int s = 0; s < SL; s++)
             jl[s] = 0.0;
         double dmax = 0.0;
+        double sabs = 0.0, ssum = 0.0;   // MODE_SUM
 
         // passes stay a runtime loop: unrolling them lets the compiler overlap the live ranges
         // of successive passes and spill
@@ -408,6 +412,21 @@ namespace lbx
                             if (pass * NC + c < SU)
                                 denseStore<C>(A, I[s * NC + c], s, pass * NC + c, l0, l1, u0, u1);
             }
+            else if constexpr (MODE == MODE_SUM)
+            {
+                static_assert(!SWAP, "sums keep the bra on the lanes");
+                if (active)
+#pragma unroll
+                    for (int s = 0; s < SL; s++)
+#pragma unroll
+                        for (int c = 0; c < NC; c++)
+                            if (pass * NC + c < SU)
+                            {
+                                const double v = I[s * NC + c];
+                                sabs += fabs(v);
+                                ssum += v;
+                            }
+            }
             else
             {
                 static_assert(MODE != MODE_DIAG || (LA == LC && LB == LD && !SWAP),
@@ -447,6 +466,20 @@ namespace lbx
         {
             if (active)
                 A.qout[l] = sqrt(dmax);
+        }
+        else if constexpr (MODE == MODE_SUM)
+        {
+            // images in the dense tensor: (ab|P) and (ba|P) for distinct shells; (P|Q), (Q|P)
+            const bool one = A.kind == 3 ? LP.sh_a[l] == LP.sh_b[l] : (A.same_class && p == q);
+            const double w = active ? (one ? 1.0 : 2.0) : 0.0;
+            sabs = waveSum(w * sabs);
+            ssum = waveSum(w * ssum);
+            if (lane == 0)
+            {
+                double *o = A.out + 4 * static_cast<long long>(uniform(UP.mol[u])) + (A.kind == 3 ? 0 : 2);
+                unsafeAtomicAdd(o, sabs);
+                unsafeAtomicAdd(o + 1, ssum);
+            }
         }
     }
 
@@ -759,6 +792,25 @@ namespace lbx
             {
                 for (int e = lane; e < SP * SQ; e += 64)
                     denseStore<C>(A, sI[e], e / SQ, e % SQ, a0, b0, c0, d0);
+            }
+            else if constexpr (MODE == MODE_SUM)
+            {
+                double sa = 0.0, ss = 0.0;
+                for (int e = lane; e < SP * SQ; e += 64)
+                {
+                    sa += fabs(sI[e]);
+                    ss += sI[e];
+                }
+                const bool one = A.kind == 3 ? A.bra.sh_a[ab] == A.bra.sh_b[ab] : (A.same_class && p == q);
+                const double w = one ? 1.0 : 2.0;
+                sa = waveSum(w * sa);
+                ss = waveSum(w * ss);
+                if (lane == 0)
+                {
+                    double *o = A.out + 4 * static_cast<long long>(A.ket.mol[cd]) + (A.kind == 3 ? 0 : 2);
+                    unsafeAtomicAdd(o, sa);
+                    unsafeAtomicAdd(o + 1, ss);
+                }
             }
             else
             {

@@ -31,6 +31,6 @@ namespace lbx
         int nc = 0;         // column-pass width of the VALU variants
     };
 
-    // mode 0 J/K, 1 dense, 2 diagonal; la, lc <= 5, lb, ld <= 3
+    // mode 0 J/K, 1 dense, 2 diagonal, 3 per-molecule sums; la, lc <= 5, lb, ld <= 3
     KernelSlot &kernelSlot(int mode, int la, int lb, int lc, int ld, int variant);
 }

@@ -20,7 +20,8 @@ LBX_MIXED = 1
 # every symbol include/lbx.h declares
 SYMBOLS = ["lbx_create", "lbx_destroy", "lbx_last_error", "lbx_dims", "lbx_set_stream",
            "lbx_synchronize", "lbx_boys", "lbx_eri4", "lbx_eri4_diag", "lbx_eri3", "lbx_eri2",
-           "lbx_eri2_diag", "lbx_jk", "lbx_tune", "lbx_set_variant", "lbx_profile_enable",
+           "lbx_eri2_diag", "lbx_jk", "lbx_create_batch", "lbx_ri_sums", "lbx_jk_grad",
+           "lbx_tune", "lbx_set_variant", "lbx_profile_enable",
            "lbx_profile_read"]
 VARIANTS = ("valu-bra", "valu-ket", "mfma", "valu-bra-f32", "valu-ket-f32")
 
@@ -39,6 +40,11 @@ class _Basis(ctypes.Structure):
 class JKStats(ctypes.Structure):
     _fields_ = [("quartets", ctypes.c_double), ("flops", ctypes.c_double),
                 ("quartets_fp32", ctypes.c_double)]
+
+
+class RIStats(ctypes.Structure):
+    _fields_ = [("triplets", ctypes.c_double), ("doublets", ctypes.c_double),
+                ("flops", ctypes.c_double)]
 
 
 class ClassProfile(ctypes.Structure):
@@ -72,6 +78,10 @@ def lib():
         for name in ("lbx_eri4", "lbx_eri4_diag", "lbx_eri3", "lbx_eri2", "lbx_eri2_diag"):
             getattr(L, name).argtypes = [p, p]
         L.lbx_jk.argtypes = [p, p, p, p, d, i, d, ctypes.POINTER(JKStats)]
+        L.lbx_jk_grad.argtypes = [p, p, p, d, ctypes.POINTER(JKStats)]
+        L.lbx_create_batch.argtypes = [ctypes.POINTER(p), i, ctypes.POINTER(_Basis),
+                                       ctypes.POINTER(_Basis)]
+        L.lbx_ri_sums.argtypes = [p, p, ctypes.POINTER(RIStats)]
         L.lbx_tune.argtypes = [p, p, d]
         L.lbx_set_variant.argtypes = [p, i, i, i, i, i]
         L.lbx_profile_enable.argtypes = [p, i]
@@ -106,19 +116,51 @@ class Context:
     """A molecule (lible_amd.basis.Molecule) resident on the current device."""
 
     def __init__(self, mol):
-        torch = _torch()
+        _torch()
         self._keep = []
         main = self._basis(mol.coords, mol.shells, mol.natoms)
         aux = self._basis(mol.coords, mol.aux, mol.natoms) if mol.aux is not None else None
         h = ctypes.c_void_p()
         _check(lib().lbx_create(ctypes.byref(h), ctypes.byref(main),
                                 ctypes.byref(aux) if aux is not None else None))
+        self._attach(h, 1)
+
+    @classmethod
+    def batch(cls, mols):
+        """One context over many independent molecules (lbx_create_batch); each needs an
+        auxiliary basis.  Only ri_sums() is available on it."""
+        _torch()
+        self = cls.__new__(cls)
+        self._keep = []
+        n = len(mols)
+        mains, auxs = (_Basis * n)(), (_Basis * n)()
+        for k, m in enumerate(mols):
+            if m.aux is None:
+                raise LbxError("every molecule of a batch needs an auxiliary basis")
+            mains[k] = self._basis(m.coords, m.shells, m.natoms)
+            auxs[k] = self._basis(m.coords, m.aux, m.natoms)
+        h = ctypes.c_void_p()
+        _check(lib().lbx_create_batch(ctypes.byref(h), n, mains, auxs))
+        self._attach(h, n)
+        return self
+
+    def _attach(self, h, nmol: int):
+        torch = _torch()
         self.h = h
+        self.nmol = nmol
         _check(lib().lbx_set_stream(self.h, torch.cuda.current_stream().cuda_stream))
         nbf, nbf_aux, nat = ctypes.c_long(), ctypes.c_long(), ctypes.c_long()
         _check(lib().lbx_dims(self.h, ctypes.byref(nbf), ctypes.byref(nbf_aux), ctypes.byref(nat)))
         self.nbf, self.nbf_aux, self.natoms = nbf.value, nbf_aux.value, nat.value
         self.device = torch.device("cuda", torch.cuda.current_device())
+
+    def ri_sums(self, out=None):
+        """Per molecule: sum |eri3|, sum eri3, sum |eri2|, sum eri2 over the dense tensors
+        (lbx_ri_sums) -> ((nmol, 4) tensor, stats)."""
+        o = self._out((self.nmol, 4)) if out is None else out
+        st = RIStats()
+        _check(lib().lbx_ri_sums(self.h, o.data_ptr(), ctypes.byref(st)))
+        return o, {"triplets": st.triplets, "doublets": st.doublets, "flops": st.flops}
 
     def _basis(self, coords, sh, natoms) -> _Basis:
         arrs = [np.ascontiguousarray(coords, np.float64), np.ascontiguousarray(sh.atom, np.int32),
@@ -169,6 +211,17 @@ class Context:
         out = self._out((self.nbf_aux,))
         _check(lib().lbx_eri2_diag(self.h, out.data_ptr()))
         return out
+
+    def jk_grad(self, D, schwarz: float = 1e-12, grad=None):
+        """Nuclear gradient (natoms, 3) of E = 1/2 tr(DJ) - 1/4 tr(DK) at fixed D (lbx_jk_grad)."""
+        torch = _torch()
+        D = D.to(device=self.device, dtype=torch.float64).contiguous()
+        if D.shape != (self.nbf, self.nbf):
+            raise LbxError(f"density must be ({self.nbf}, {self.nbf})")
+        g = self._out((self.natoms, 3)) if grad is None else grad
+        st = JKStats()
+        _check(lib().lbx_jk_grad(self.h, D.data_ptr(), g.data_ptr(), schwarz, ctypes.byref(st)))
+        return g, {"quartets": st.quartets, "flops": st.flops}
 
     def jk(self, D, schwarz: float = 1e-12, precision: int = LBX_FP64, mixed_thresh: float = 1e-5,
            J=None, K=None):

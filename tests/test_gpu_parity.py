@@ -181,6 +181,87 @@ def test_jk_mixed_energy(lbx):
     assert dJ <= 1e-9 and dK <= 1e-9
 
 
+def dense_sums(eri3, eri2):
+    return np.array([np.abs(eri3).sum(), eri3.sum(), np.abs(eri2).sum(), eri2.sum()])
+
+
+def test_ri_sums_named_basis(lbx, oracle):
+    """lbx_ri_sums on an ordinary context against the reference's eri3 / eri2 checksums and the
+    oracle's dense tensors."""
+    m = mol("def2-tzvp", O3, "def2-qzvp-rifit")
+    got, st = lbx.Context(m).ri_sums()
+    got = got.cpu().numpy()[0]
+    s = oracle.Structure(m)
+    ref = dense_sums(s.eri3(), s.eri2())
+    close(got, ref)
+    assert abs(got[0] - 25854.105851585416531) <= REL * 25854.1   # tests_ints.cpp:730
+    assert st["triplets"] > 0 and st["doublets"] > 0
+
+
+def test_ri_sums_batch(lbx, oracle):
+    """A batch of independent molecules in one context and one captured graph: each molecule's
+    sums equal those of its own dense tensors; replays and a second output buffer agree."""
+    import torch
+    mols = [B.synthetic_molecule(300 + k, 2 + k % 4, 3.0 + k % 3, B.RECIPE_MAIN, B.RECIPE_AUX)
+            for k in range(7)]
+    ctx = lbx.Context.batch(mols)
+    out, st = ctx.ri_sums()
+    first = out.cpu().numpy().copy()
+    for k, m in enumerate(mols):
+        s = oracle.Structure(m)
+        close(first[k], dense_sums(s.eri3(), s.eri2()))
+    for _ in range(2):
+        ctx.ri_sums(out)
+        # the sums are accumulated with atomics, so replays agree to rounding
+        close(out.cpu().numpy(), first, rel=1e-13, floor=0.0)
+    other = torch.full((len(mols), 4), 7.0, dtype=torch.float64, device="cuda")
+    ctx.ri_sums(other)
+    close(other.cpu().numpy(), first)
+    with pytest.raises(lbx.LbxError):
+        ctx.eri3()
+
+
+@pytest.mark.parametrize("thresh", [0.0, 1e-12])
+def test_jk_grad(lbx, oracle, thresh):
+    """Nuclear gradient of E = 1/2 tr(DJ) - 1/4 tr(DK) against the oracle's contraction of its
+    derivative integrals (3 atoms, s to f shells: every class pair)."""
+    import torch
+    m = B.synthetic_molecule(2025, 3, 4.5, B.RECIPE_MAIN)
+    ctx = lbx.Context(m)
+    D = B.density_from_orbitals(6, ctx.nbf, 15, 0.1)
+    g, st = ctx.jk_grad(torch.tensor(D, device="cuda"), schwarz=thresh)
+    gr = oracle.Structure(m).jk_grad(D, thresh=thresh)
+    g = g.cpu().numpy()
+    close(g, gr, rel=1e-10, floor=1e-12)
+    assert np.abs(g.sum(axis=0)).max() <= 1e-10 * np.abs(g).max()
+    assert st["quartets"] > 0 and st["flops"] > 0
+
+
+def test_jk_grad_finite_difference(lbx):
+    """The GPU gradient against central differences (h = 1e-4 bohr) of the GPU energy."""
+    import torch
+    m = B.synthetic_molecule(11, 3, 3.0, B.RECIPE_SMALL)
+    ctx = lbx.Context(m)
+    D = torch.tensor(B.density_from_orbitals(5, ctx.nbf, 4, 0.3), device="cuda")
+    g, _ = ctx.jk_grad(D, schwarz=0.0)
+    g = g.cpu().numpy()
+
+    def energy(coords):
+        c = lbx.Context(B.Molecule(coords, m.shells))
+        J, K, _ = c.jk(D, schwarz=0.0)
+        return 0.5 * float((D * J).sum()) - 0.25 * float((D * K).sum())
+
+    h = 1e-4
+    fd = np.zeros_like(g)
+    for ia in range(m.natoms):
+        for x in range(3):
+            cp, cm = m.coords.copy(), m.coords.copy()
+            cp[ia, x] += h
+            cm[ia, x] -= h
+            fd[ia, x] = (energy(cp) - energy(cm)) / (2 * h)
+    assert np.abs(g - fd).max() < 1e-7, np.abs(g - fd).max()
+
+
 def test_jk_tuned(lbx, jk_case):
     import torch
     m, D, Jr, Kr, _ = jk_case
