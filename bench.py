@@ -6,7 +6,12 @@ Workload (default `c2_jk_fp64`): 40 atoms placed in a 14-bohr cube (seed 1002, e
 exponents log-uniform in [0.1, 1e3] and coefficients uniform in [0.1, 1] -> 1200 functions;
 density D = C C^T with C 1200 x 100 of N(0, 0.1) entries; Schwarz threshold 1e-12.
 A step is one J/K build (lbx_jk) with the basis, pair data, E matrices and Schwarz factors
-already resident in HBM.  value = screened shell quartets per second.
+already resident in HBM.  value = screened shell quartets per second of the FP64 build.
+After the FP64 steps, --mixed-steps builds of the mixed-precision form (FP32 R and transforms
+for quartets with Schwarz bound < 1e-5) are timed and compared with the FP64 result ("mixed").
+--config c3_jk_mixed runs the 120-atom, 3600-function workload instead; --config c4_ri_batch
+the batched RI integrals of 256 molecules (value = (mn|P) triplets/s); --config c5_gradient
+the nuclear gradient of the 60-atom workload (value = quartets/s of the gradient).
 
 Prints one JSON line.  --gpus N > 1 exits: this build runs on one GPU.
 """
@@ -24,6 +29,7 @@ sys.path.insert(0, ROOT)
 METRIC = "J/K build ms + ERI quartets/sec, mixed vs FP64 GPU at ≤1e-10 Eh; % FP64 peak"
 FP64_PEAK_TFLOPS = 78.6      # MI355X dense FP64 (vector and matrix)
 SCHWARZ = 1e-12
+MIXED_SPLIT = 1e-5           # mixed precision: FP32 for quartets with Q_ab Q_cd below this
 
 
 def log(msg: str):
@@ -55,6 +61,141 @@ def cpu_baseline(mol, D, gpu_stats, target_s: float):
     }
 
 
+def timed(fn, steps: int, label: str):
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = None
+    for k in range(steps):
+        res = fn()
+        log(f"[bench] {label} step {k} done")
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps, res
+
+
+def bench_ri_batch(a):
+    """Batched RI (mn|P) + (P|Q) over 256 molecules in one context; a step is one lbx_ri_sums
+    (replay of the captured graph): every molecule's eri3 and eri2 evaluated and reduced to its
+    checksums."""
+    import torch
+    from lible_amd import basis as B
+    from lible_amd import lbx
+    mols = B.batch_molecules()
+    t0 = time.perf_counter()
+    ctx = lbx.Context.batch(mols)
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    log(f"[bench] c4_ri_batch: {len(mols)} molecules, {sum(m.natoms for m in mols)} atoms, "
+        f"nbf {ctx.nbf}, naux {ctx.nbf_aux}, setup {t_setup:.1f} s")
+    out = torch.empty((len(mols), 4), dtype=torch.float64, device="cuda")
+    t0 = time.perf_counter()
+    _, st = ctx.ri_sums(out)
+    torch.cuda.synchronize()
+    t_first = time.perf_counter() - t0
+    log(f"[bench] first call (task tables + graph capture) {t_first:.2f} s")
+    for k in range(a.warmup):
+        ctx.ri_sums(out)
+        log(f"[bench] warm-up step {k} done")
+    step_s, (_, st) = timed(lambda: ctx.ri_sums(out), a.steps, "ri")
+    tflops = st["flops"] / step_s / 1e12
+    res = {
+        "metric": "batched RI eri3 (mn|P) shell triplets/s (+ eri2 doublets/s)",
+        "value": st["triplets"] / step_s,
+        "unit": "triplets/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": step_s * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "c4_ri_batch", "molecules": len(mols),
+                   "atoms_per_molecule": "3..30", "main": "30 AO/atom", "aux": "56 AO/atom",
+                   "triplets": st["triplets"], "doublets": st["doublets"]},
+        "doublets_per_s": st["doublets"] / step_s,
+        "total_tflops": tflops,
+        "total_frac_fp64_peak": tflops / FP64_PEAK_TFLOPS,
+        "setup_s": t_setup, "first_call_s": t_first,
+        "checksum_eri3_abs_total": float(out[:, 0].sum()),
+    }
+    if not a.no_cpu_baseline:
+        # the oracle's dense eri3 + eri2 of the first small molecules, ~cpu-seconds of work
+        from oracle import oracle as O
+        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        done_t, done_q, names = 0.0, 0.0, []
+        for k, m in enumerate(mols):
+            if m.natoms > 8:
+                continue
+            _, s1 = lbx.Context.batch([m]).ri_sums()
+            s = O.Structure(m)
+            t0 = time.perf_counter()
+            s.eri3()
+            s.eri2()
+            done_t += time.perf_counter() - t0
+            done_q += s1["triplets"]
+            names.append(k)
+            if done_t > a.cpu_seconds:
+                break
+        res["cpu_baseline"] = {"value": done_q / max(done_t, 1e-9), "unit": "triplets/s",
+                               "cores": cores, "kind": "port",
+                               "sample": f"oracle dense eri3 + eri2 of molecules {names} "
+                                         f"({done_q:.0f} triplets, {done_t:.1f} s)"}
+    print(json.dumps(res), flush=True)
+    return 0
+
+
+def bench_gradient(a):
+    """Nuclear gradient of the two-electron energy at fixed density (lbx_jk_grad) on the
+    60-atom, 1800-function workload; checked against a central difference of the energy
+    along the largest gradient component."""
+    import numpy as np
+    import torch
+    from lible_amd import basis as B
+    from lible_amd import lbx
+    cfg = B.CONFIGS["c5_gradient"]
+    mol = B.config_molecule("c5_gradient")
+    ctx = lbx.Context(mol)
+    D = torch.tensor(B.density_from_orbitals(cfg["seed"] + 100, ctx.nbf, cfg["nocc"], cfg["sigma"]),
+                     device="cuda")
+    log(f"[bench] c5_gradient: nbf={ctx.nbf} atoms={mol.natoms}")
+    t0 = time.perf_counter()
+    g, st = ctx.jk_grad(D, schwarz=SCHWARZ)
+    torch.cuda.synchronize()
+    t_first = time.perf_counter() - t0
+    log(f"[bench] first call (Schwarz factors, E' matrices, task tables) {t_first:.2f} s")
+    for k in range(a.warmup):
+        ctx.jk_grad(D, schwarz=SCHWARZ, grad=g)
+        log(f"[bench] warm-up step {k} done")
+    step_s, (_, st) = timed(lambda: ctx.jk_grad(D, schwarz=SCHWARZ, grad=g), a.steps, "gradient")
+    gh = g.cpu().numpy()
+    # central difference along the largest component (h = 1e-4 bohr)
+    ia, x = np.unravel_index(np.argmax(np.abs(gh)), gh.shape)
+    h = 1e-4
+    es = []
+    for sgn in (1, -1):
+        c = mol.coords.copy()
+        c[ia, x] += sgn * h
+        cx = lbx.Context(B.Molecule(c, mol.shells))
+        J, K, _ = cx.jk(D, schwarz=SCHWARZ)
+        es.append(0.5 * float((D * J).sum()) - 0.25 * float((D * K).sum()))
+        cx.close()
+        log(f"[bench] finite-difference energy {len(es)} done")
+    fd = (es[0] - es[1]) / (2 * h)
+    tflops = st["flops"] / step_s / 1e12
+    res = {
+        "metric": "nuclear gradient of the 2e energy: shell quartets/s",
+        "value": st["quartets"] / step_s, "unit": "quartets/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": step_s * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "c5_gradient", "atoms": mol.natoms, "nbf": ctx.nbf,
+                   "schwarz": SCHWARZ, "quartets": st["quartets"], "flops_per_step": st["flops"]},
+        "total_tflops": tflops, "total_frac_fp64_peak": tflops / FP64_PEAK_TFLOPS,
+        "first_call_s": t_first,
+        "fd_check": {"atom": int(ia), "axis": int(x), "analytic": float(gh[ia, x]), "central_diff": fd,
+                     "abs_diff": abs(float(gh[ia, x]) - fd)},
+        "max_abs_translation_sum": float(np.abs(gh.sum(axis=0)).max()),
+    }
+    print(json.dumps(res), flush=True)
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -64,6 +205,8 @@ def main(argv=None):
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--mixed-steps", type=int, default=1,
+                    help="timed mixed-precision builds after the FP64 ones (0: skip)")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
@@ -77,6 +220,10 @@ def main(argv=None):
     from lible_amd import lbx
 
     torch.cuda.set_device(0)
+    if a.config == "c4_ri_batch":
+        return bench_ri_batch(a)
+    if a.config == "c5_gradient":
+        return bench_gradient(a)
     cfg = B.CONFIGS[a.config]
     mol = B.config_molecule(a.config)
     t0 = time.perf_counter()
@@ -93,16 +240,18 @@ def main(argv=None):
     _, _, st = ctx.jk(D, schwarz=SCHWARZ, J=J, K=K)
     torch.cuda.synchronize()
     t_first = time.perf_counter() - t0
+    log(f"[bench] first call (Schwarz factors + task tables) {t_first:.2f} s, "
+        f"quartets {st['quartets']:.3e}")
     t0 = time.perf_counter()
     if not a.no_tune:
         ctx.tune(D, schwarz=SCHWARZ)
     torch.cuda.synchronize()
     t_tune = time.perf_counter() - t0
-    for _ in range(max(a.warmup, 1)):
+    log(f"[bench] tuning {t_tune:.2f} s")
+    for k in range(a.warmup):
         _, _, st = ctx.jk(D, schwarz=SCHWARZ, J=J, K=K)
-    torch.cuda.synchronize()
-    log(f"[bench] first call (Schwarz factors + task tables) {t_first:.2f} s, tuning {t_tune:.2f} s,"
-        f" quartets {st['quartets']:.3e}")
+        torch.cuda.synchronize()
+        log(f"[bench] warm-up step {k} done")
 
     ctx.profile(True)
     torch.cuda.synchronize()
@@ -124,6 +273,34 @@ def main(argv=None):
     tot_tflops = st["flops"] / step_s / 1e12
     # energy as a by-product check of the build
     E = 0.5 * float((D * J).sum()) - 0.25 * float((D * K).sum())
+
+    mixed = None
+    if a.mixed_steps > 0:
+        # the same build with FP32 R and transforms for quartets whose bound is < MIXED_SPLIT
+        Jm, Km = torch.empty_like(D), torch.empty_like(D)
+        ctx.jk(D, schwarz=SCHWARZ, precision=lbx.LBX_MIXED, mixed_thresh=MIXED_SPLIT, J=Jm, K=Km)
+        torch.cuda.synchronize()
+        log("[bench] first mixed call (FP32 task tables) done")
+        t0 = time.perf_counter()
+        for k in range(a.mixed_steps):
+            _, _, sm = ctx.jk(D, schwarz=SCHWARZ, precision=lbx.LBX_MIXED,
+                              mixed_thresh=MIXED_SPLIT, J=Jm, K=Km)
+            log(f"[bench] mixed step {k} done")
+        torch.cuda.synchronize()
+        m_s = (time.perf_counter() - t0) / a.mixed_steps
+        Em = 0.5 * float((D * Jm).sum()) - 0.25 * float((D * Km).sum())
+        mixed = {
+            "split": MIXED_SPLIT,
+            "ms_per_step": m_s * 1e3,
+            "quartets_per_s": sm["quartets"] / m_s,
+            "fp32_share": sm["quartets_fp32"] / max(sm["quartets"], 1),
+            "speedup_vs_fp64": step_s / m_s,
+            "delta_E": Em - E,
+            "delta_E_rel": (Em - E) / abs(E),
+            "max_abs_dJ": float((Jm - J).abs().max()),
+            "max_abs_dK": float((Km - K).abs().max()),
+            "energy_2e": Em,
+        }
 
     out = {
         "metric": METRIC,
@@ -174,6 +351,7 @@ def main(argv=None):
         "setup_s": t_setup,
         "first_call_s": t_first,
         "tune_s": t_tune,
+        "mixed": mixed,
     }
     if not a.no_cpu_baseline:
         log("[bench] cpu baseline (oracle sample)")

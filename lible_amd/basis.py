@@ -169,3 +169,22 @@ CONFIGS = {
 def config_molecule(name: str) -> Molecule:
     c = CONFIGS[name]
     return synthetic_molecule(c["seed"], c["natoms"], c["box"], c["recipe"])
+
+
+# The batched RI workload: 256 independent molecules of 3..30 atoms (count drawn per molecule),
+# main basis 30 functions per atom, auxiliary 6s 4p 3d 2f 1g = 56 per atom, at the atom density
+# of the 40-atom / 14-bohr workload (68.6 bohr^3 per atom).
+RI_BATCH = dict(seed=1004, nmol=256, natoms_lo=3, natoms_hi=30, bohr3_per_atom=68.6)
+
+
+def batch_molecules(nmol: int | None = None, seed: int | None = None) -> list[Molecule]:
+    c = RI_BATCH
+    seed = c["seed"] if seed is None else seed
+    nmol = c["nmol"] if nmol is None else nmol
+    out = []
+    for k in range(nmol):
+        rng = np.random.default_rng(seed * 100000 + k)
+        n = int(rng.integers(c["natoms_lo"], c["natoms_hi"] + 1))
+        box = (c["bohr3_per_atom"] * n) ** (1.0 / 3.0)
+        out.append(synthetic_molecule(seed * 100000 + k, n, box, RECIPE_MAIN, RECIPE_AUX))
+    return out
