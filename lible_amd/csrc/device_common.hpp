@@ -208,6 +208,19 @@ namespace lbx
 
     __device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
 
+    // Read-only data at a wave-uniform address (the uniform side's E matrices and primitive
+    // data), viewed through the constant address space: the compiler then emits scalar loads
+    // into SGPRs, which the VALU FMAs take as operands directly, instead of vector loads that
+    // broadcast one address into 64 VGPR lanes.  Only for memory no kernel writes meanwhile.
+    template <typename T>
+    using cptr = const T __attribute__((address_space(4))) *;
+
+    template <typename T>
+    __device__ __forceinline__ cptr<T> constView(const T *p)
+    {
+        return (cptr<T>)p;
+    }
+
     // task t -> ket position q with prefix[q] <= t < prefix[q+1]
     __device__ __forceinline__ int findKet(const long long *__restrict__ prefix, int nket,
                                            long long t)

@@ -189,7 +189,8 @@ namespace lbx
             eUb = SWAP ? UP.eb : UP.ek;
             eLb = SWAP ? LP.ek : LP.eb;
         }
-        const T *__restrict__ eU = eUb + static_cast<size_t>(upoff) * HU * SU;
+        const cptr<T> eU = constView(eUb + static_cast<size_t>(upoff) * HU * SU);
+        const cptr<double> ppU = constView(reinterpret_cast<const double *>(UP.pp + upoff));
         const T *__restrict__ eL = eLb + static_cast<size_t>(lpoff) * HLn * SL;
 
         const int l0 = active ? LP.ao_a[l] : 0, l1 = active ? LP.ao_b[l] : 0;
@@ -234,7 +235,7 @@ namespace lbx
                     X[e] = T(0);
                 for (int k = 0; k < ucnt; k++)
                 {
-                    const double4 Pu = UP.pp[upoff + k];
+                    const double4 Pu{ppU[4 * k], ppU[4 * k + 1], ppU[4 * k + 2], ppU[4 * k + 3]};
                     const double4 &Pb = SWAP ? Pu : Pl;
                     const double4 &Pk = SWAP ? Pl : Pu;
                     const double pq = Pb.x * Pk.x, ps = Pb.x + Pk.x;
@@ -252,7 +253,7 @@ namespace lbx
                         sc *= -2.0 * alpha;
                     }
                     hermiteR<L, T>(R, T(X_), T(Y_), T(Z_));
-                    const T *__restrict__ ek = eU + static_cast<size_t>(k) * HU * SU;
+                    const cptr<T> ek = eU + static_cast<size_t>(k) * HU * SU;
                     constexpr RSumTab<L, HLn, HU> RS{};
 #pragma unroll
                     for (int h = 0; h < HLn; h++)
@@ -306,10 +307,14 @@ namespace lbx
                 }
                 if (active)
                 {
-                    // K[l0+al][u0+ga] += f sum_{be, de} I D[l1+be][u1+de]
-                    // K[l1+be][u0+ga] += f sum_{al, de} I D[l0+al][u1+de]
-                    // K[l0+al][u1+de] += f sum_{be, ga} I D[l1+be][u0+ga]
-                    // K[l1+be][u1+de] += f sum_{al, ga} I D[l0+al][u0+ga]
+                    // K[u0+ga][l0+al] += f sum_{be, de} I D[u1+de][l1+be]
+                    // K[u0+ga][l1+be] += f sum_{al, de} I D[u1+de][l0+al]
+                    // K[u1+de][l0+al] += f sum_{be, ga} I D[u0+ga][l1+be]
+                    // K[u1+de][l1+be] += f sum_{al, ga} I D[u0+ga][l0+al]
+                    // The wave-uniform shell indexes the row of K (and of the symmetric D): the
+                    // unique-quartet contributions are symmetrised at the end (K + K^T), so
+                    // either triangle may receive them, and with the row fixed across the wave
+                    // its atomics and loads stay on a few rows that remain in L2.
 #pragma unroll
                     for (int ga = 0; ga < SUa; ga++)
                     {
@@ -333,9 +338,9 @@ namespace lbx
                                     if (su / NC != pass)
                                         continue;
                                     s = fma(double(I[(al * SLb + be) * NC + su % NC]),
-                                            D[(l1 + be) * n + u1 + de], s);
+                                            D[(u1 + de) * n + l1 + be], s);
                                 }
-                            unsafeAtomicAdd(&A.K[(l0 + al) * n + u0 + ga], f * s);
+                            unsafeAtomicAdd(&A.K[(u0 + ga) * n + l0 + al], f * s);
                         }
 #pragma unroll
                         for (int be = 0; be < SLb; be++)
@@ -350,9 +355,9 @@ namespace lbx
                                     if (su / NC != pass)
                                         continue;
                                     s = fma(double(I[(al * SLb + be) * NC + su % NC]),
-                                            D[(l0 + al) * n + u1 + de], s);
+                                            D[(u1 + de) * n + l0 + al], s);
                                 }
-                            unsafeAtomicAdd(&A.K[(l1 + be) * n + u0 + ga], f * s);
+                            unsafeAtomicAdd(&A.K[(u0 + ga) * n + l1 + be], f * s);
                         }
                     }
 #pragma unroll
@@ -377,9 +382,9 @@ namespace lbx
                                     if (su / NC != pass)
                                         continue;
                                     s = fma(double(I[(al * SLb + be) * NC + su % NC]),
-                                            D[(l1 + be) * n + u0 + ga], s);
+                                            D[(u0 + ga) * n + l1 + be], s);
                                 }
-                            unsafeAtomicAdd(&A.K[(l0 + al) * n + u1 + de], f * s);
+                            unsafeAtomicAdd(&A.K[(u1 + de) * n + l0 + al], f * s);
                         }
 #pragma unroll
                         for (int be = 0; be < SLb; be++)
@@ -394,9 +399,9 @@ namespace lbx
                                     if (su / NC != pass)
                                         continue;
                                     s = fma(double(I[(al * SLb + be) * NC + su % NC]),
-                                            D[(l0 + al) * n + u0 + ga], s);
+                                            D[(u0 + ga) * n + l0 + al], s);
                                 }
-                            unsafeAtomicAdd(&A.K[(l1 + be) * n + u1 + de], f * s);
+                            unsafeAtomicAdd(&A.K[(u1 + de) * n + l1 + be], f * s);
                         }
                     }
                 }
@@ -755,7 +760,7 @@ namespace lbx
                     for (int nu = 0; nu < SB; nu++)
                         for (int la = 0; la < SD; la++)
                             s = fma(sI[(mu * SB + nu) * SQ + ka * SD + la], sD[2][nu * SD + la], s);
-                    unsafeAtomicAdd(&A.K[(a0 + mu) * n + c0 + ka], f * s);
+                    unsafeAtomicAdd(&A.K[(c0 + ka) * n + a0 + mu], f * s);
                 }
                 // K_bc[nu,ka] += f sum_{mu,la} I D_ad[mu,la]
                 for (int e = lane; e < SB * SC; e += 64)
@@ -765,7 +770,7 @@ namespace lbx
                     for (int mu = 0; mu < SA; mu++)
                         for (int la = 0; la < SD; la++)
                             s = fma(sI[(mu * SB + nu) * SQ + ka * SD + la], sD[3][mu * SD + la], s);
-                    unsafeAtomicAdd(&A.K[(b0 + nu) * n + c0 + ka], f * s);
+                    unsafeAtomicAdd(&A.K[(c0 + ka) * n + b0 + nu], f * s);
                 }
                 // K_ad[mu,la] += f sum_{nu,ka} I D_bc[nu,ka]
                 for (int e = lane; e < SA * SD; e += 64)
@@ -775,7 +780,7 @@ namespace lbx
                     for (int nu = 0; nu < SB; nu++)
                         for (int ka = 0; ka < SC; ka++)
                             s = fma(sI[(mu * SB + nu) * SQ + ka * SD + la], sD[4][nu * SC + ka], s);
-                    unsafeAtomicAdd(&A.K[(a0 + mu) * n + d0 + la], f * s);
+                    unsafeAtomicAdd(&A.K[(d0 + la) * n + a0 + mu], f * s);
                 }
                 // K_bd[nu,la] += f sum_{mu,ka} I D_ac[mu,ka]
                 for (int e = lane; e < SB * SD; e += 64)
@@ -785,7 +790,7 @@ namespace lbx
                     for (int mu = 0; mu < SA; mu++)
                         for (int ka = 0; ka < SC; ka++)
                             s = fma(sI[(mu * SB + nu) * SQ + ka * SD + la], sD[5][mu * SC + ka], s);
-                    unsafeAtomicAdd(&A.K[(b0 + nu) * n + d0 + la], f * s);
+                    unsafeAtomicAdd(&A.K[(d0 + la) * n + b0 + nu], f * s);
                 }
             }
             else if constexpr (MODE == MODE_DENSE)
