@@ -219,7 +219,8 @@ namespace lbx
 #pragma unroll 1
                     for (int k = 0; k < ucnt; k++)
                     {
-                        const double4 Pu = UP.pp[upoff + k];
+                        const cptr<double> pu = constView(reinterpret_cast<const double *>(UP.pp + upoff + k));
+                        const double4 Pu{pu[0], pu[1], pu[2], pu[3]};
                         const double pq = Pl.x * Pu.x, ps = Pl.x + Pu.x, alpha = pq / ps;
                         const double PX = Pl.y - Pu.y, PY = Pl.z - Pu.z, PZ = Pl.w - Pu.w;
                         double F[L1 + 1];
@@ -264,8 +265,8 @@ namespace lbx
                         if (valid)
                         {
                             const size_t kk = static_cast<size_t>(upoff + k);
-                            const double *ek = UP.ek + kk * HQ * SQ + cd0;
-                            const double *e1 = UP.e1k + kk * 3 * HQ * SQ + cd0;
+                            const cptr<double> ek = constView(UP.ek + kk * HQ * SQ + cd0);
+                            const cptr<double> e1 = constView(UP.e1k + kk * 3 * HQ * SQ + cd0);
                             double tk[3] = {0, 0, 0};
 #pragma unroll
                             for (int g = 0; g < HQ; g++)
@@ -310,7 +311,7 @@ namespace lbx
                                     gC[2] = fma(r0, zz, gC[2]);
                                 }
                             }
-                            const double rc = UP.ra[upoff + k];   // c / q
+                            const double rc = constView(UP.ra)[upoff + k];   // c / q
 #pragma unroll
                             for (int x = 0; x < 3; x++)
                             {
