@@ -59,10 +59,22 @@ def runtime_link() -> list[str]:
     return []
 
 
+def _big_stack():
+    """The optimiser recurses over the control-flow graph of the large, fully unrolled quartet
+    kernels deeply enough to overflow the default 8 MiB stack; give the compiler an unlimited
+    one."""
+    import resource
+    try:
+        resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+    except (ValueError, OSError):
+        soft, hard = resource.getrlimit(resource.RLIMIT_STACK)
+        resource.setrlimit(resource.RLIMIT_STACK, (hard, hard))
+
+
 def _compile(src: str, obj: str, defines: list[str]) -> tuple[str, float]:
     t = time.time()
     cmd = [hipcc(), "-x", "hip", "-c", src, "-o", obj] + FLAGS + defines
-    r = subprocess.run(cmd, capture_output=True, text=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, preexec_fn=_big_stack)
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     return obj, time.time() - t

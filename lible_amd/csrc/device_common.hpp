@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
 #include <utility>
 
 #include "lbx_common.hpp"
@@ -49,6 +50,69 @@ namespace lbx
             s = fma(s, m, row[2] * 0.5);
             s = fma(s, m, row[1]);
             s = fma(s, m, row[0]);
+            F[L] = s;
+            const double ex = exp(-x);
+            const double x2 = 2.0 * x;
+#pragma unroll
+            for (int k = L - 1; k >= 0; k--)
+                F[k] = fma(x2, F[k + 1], ex) * (1.0 / (2 * k + 1));
+        }
+    }
+
+    // The same evaluation split in two so that the table row can be fetched one iteration ahead
+    // of its use (the row load is a random gather whose latency otherwise stalls the wave):
+    // boysFetch loads the 7 tabulated values (any x: the row index is clamped to the table),
+    // boysEval finishes with exactly the operations of boysF.
+    struct BoysRow
+    {
+        double x;
+        double r[7];
+    };
+
+    template <int L>
+    __device__ __forceinline__ void boysFetch(double x, const double *__restrict__ tab, BoysRow &b)
+    {
+        b.x = x;
+        int iv = static_cast<int>(x * 100.0);
+        iv = iv < kBoysRows - 1 ? iv : kBoysRows - 1;
+        iv = iv > 0 ? iv : 0;
+        const double *row = tab + iv * kBoysCols + L;
+#pragma unroll
+        for (int j = 0; j < 7; j++)
+            b.r[j] = row[j];
+    }
+
+    template <int L>
+    __device__ __forceinline__ void boysEval(const BoysRow &b, double *F)
+    {
+        const double x = b.x;
+        if (x == 0.0)
+        {
+            F[0] = 1.0;
+#pragma unroll
+            for (int k = 1; k <= L; k++)
+                F[k] = 1.0 / (2 * k + 1);
+        }
+        else if (x > kBoysLargeX)
+        {
+            const double rx = 1.0 / x;
+            F[0] = 0.5 * sqrt(M_PI * rx);
+#pragma unroll
+            for (int k = 1; k <= L; k++)
+                F[k] = F[k - 1] * (k - 0.5) * rx;
+        }
+        else
+        {
+            const int iv = static_cast<int>(x * 100.0);
+            const double dx = x - iv * kBoysStep;
+            const double m = -dx;
+            double s = b.r[6] * (1.0 / 720.0);
+            s = fma(s, m, b.r[5] * (1.0 / 120.0));
+            s = fma(s, m, b.r[4] * (1.0 / 24.0));
+            s = fma(s, m, b.r[3] * (1.0 / 6.0));
+            s = fma(s, m, b.r[2] * 0.5);
+            s = fma(s, m, b.r[1]);
+            s = fma(s, m, b.r[0]);
             F[L] = s;
             const double ex = exp(-x);
             const double x2 = 2.0 * x;
@@ -154,6 +218,56 @@ namespace lbx
         }
     };
 
+    // ---- Structural zeros of the spherical E matrices.  The real solid harmonic S_{l,m}
+    // (m = 0, +1, -1, +2, -2, ... for spherical slot 0, 1, 2, ...) is a combination of the
+    // Cartesian monomials x^{|m|-p+2s} y^{p+2(t-s)} z^{l-|m|-2t} (0 <= s <= t <= (l-|m|)/2, p <= |m|,
+    // p even for m >= 0 and odd for m < 0), and the 1-D Hermite coefficient E^{ij}_t vanishes
+    // for t > i + j; so E[tuv][mu nu] is identically zero unless some monomial pair of mu and nu
+    // covers (t, u, v).  Every other entry is generically nonzero.
+    constexpr LBX_HD bool ePatternNz(int la, int lb, int h, int mn)
+    {
+        const int tuv = hermTuv(h);
+        const int t = tuv & 255, u = (tuv >> 8) & 255, v = tuv >> 16;
+        const int sb = 2 * lb + 1;
+        const int mu = mn / sb, nu = mn % sb;
+        const int ma = mu == 0 ? 0 : ((mu & 1) ? (mu + 1) / 2 : -(mu / 2));
+        const int mb = nu == 0 ? 0 : ((nu & 1) ? (nu + 1) / 2 : -(nu / 2));
+        const int aa = ma < 0 ? -ma : ma, ab = mb < 0 ? -mb : mb;
+        for (int ta = 0; 2 * ta <= la - aa; ta++)
+            for (int sa = 0; sa <= ta; sa++)
+                for (int pa = 0; pa <= aa; pa++)
+                {
+                    if ((ma >= 0) != (pa % 2 == 0))
+                        continue;
+                    const int xa = aa - pa + 2 * sa, ya = pa + 2 * (ta - sa), za = la - aa - 2 * ta;
+                    for (int tb = 0; 2 * tb <= lb - ab; tb++)
+                        for (int sb2 = 0; sb2 <= tb; sb2++)
+                            for (int pb = 0; pb <= ab; pb++)
+                            {
+                                if ((mb >= 0) != (pb % 2 == 0))
+                                    continue;
+                                const int xb = ab - pb + 2 * sb2, yb = pb + 2 * (tb - sb2),
+                                          zb = lb - ab - 2 * tb;
+                                if (t <= xa + xb && u <= ya + yb && v <= za + zb)
+                                    return true;
+                            }
+                }
+        return false;
+    }
+
+    template <int LA, int LB>
+    struct EPattern
+    {
+        static constexpr int H = nHerm(LA + LB), S = (2 * LA + 1) * (2 * LB + 1);
+        bool nz[H][S];
+        constexpr EPattern() : nz()
+        {
+            for (int h = 0; h < H; h++)
+                for (int s = 0; s < S; s++)
+                    nz[h][s] = ePatternNz(LA, LB, h, s);
+        }
+    };
+
     constexpr LBX_HD int hermLevel(int i)
     {
         const int a = hermTuv(i);
@@ -207,6 +321,32 @@ namespace lbx
     }
 
     __device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
+    // for (p = 0; p < N; p++) f(p) as a runtime loop.  Up to kStaticPasses passes each index
+    // gets its own instance of f (called with integral_constant<int, p>) and one instance runs
+    // per iteration; beyond that (code size and compile time) f is called with a plain int.
+    constexpr int kStaticPasses = 3;
+
+    template <int N, class F, int... P>
+    __device__ __forceinline__ void forEachPassImpl(F &f, std::integer_sequence<int, P...>)
+    {
+#pragma unroll 1
+        for (int p = 0; p < N; p++)
+            ((p == P ? (f(std::integral_constant<int, P>{}), 0) : 0), ...);
+    }
+
+    template <int N, class F>
+    __device__ __forceinline__ void forEachPass(F &f)
+    {
+        if constexpr (N <= kStaticPasses)
+            forEachPassImpl<N>(f, std::make_integer_sequence<int, N>{});
+        else
+        {
+#pragma unroll 1
+            for (int p = 0; p < N; p++)
+                f(p);
+        }
+    }
 
     // Read-only data at a wave-uniform address (the uniform side's E matrices and primitive
     // data), viewed through the constant address space: the compiler then emits scalar loads

@@ -216,15 +216,27 @@ namespace lbx
                     }
                     const double4 Pl = LP.pp[ip];
                     double Ts[3] = {0, 0, 0};
+                    const cptr<double> ppU = constView(reinterpret_cast<const double *>(UP.pp + upoff));
+                    // Boys table row of ket primitive k + 1 fetched while k is computed
+                    auto argOf = [&](int k) {
+                        const double X_ = Pl.y - ppU[4 * k + 1], Y_ = Pl.z - ppU[4 * k + 2],
+                                     Z_ = Pl.w - ppU[4 * k + 3];
+                        const double pq = Pl.x * ppU[4 * k], ps = Pl.x + ppU[4 * k];
+                        return pq / ps * (X_ * X_ + Y_ * Y_ + Z_ * Z_);
+                    };
+                    BoysRow next;
+                    boysFetch<L1>(argOf(0), A.boys, next);
 #pragma unroll 1
                     for (int k = 0; k < ucnt; k++)
                     {
-                        const cptr<double> pu = constView(reinterpret_cast<const double *>(UP.pp + upoff + k));
-                        const double4 Pu{pu[0], pu[1], pu[2], pu[3]};
+                        const BoysRow cur = next;
+                        if (k + 1 < ucnt)
+                            boysFetch<L1>(argOf(k + 1), A.boys, next);
+                        const double4 Pu{ppU[4 * k], ppU[4 * k + 1], ppU[4 * k + 2], ppU[4 * k + 3]};
                         const double pq = Pl.x * Pu.x, ps = Pl.x + Pu.x, alpha = pq / ps;
                         const double PX = Pl.y - Pu.y, PY = Pl.z - Pu.z, PZ = Pl.w - Pu.w;
                         double F[L1 + 1];
-                        boysF<L1>(alpha * (PX * PX + PY * PY + PZ * PZ), A.boys, F);
+                        boysEval<L1>(cur, F);
                         const double fac = kTwoPi52 / (pq * sqrt(ps));
                         double init[L1 + 1];
                         {

@@ -216,11 +216,14 @@ namespace lbx
         double dmax = 0.0;
         double sabs = 0.0, ssum = 0.0;   // MODE_SUM
 
-        // passes stay a runtime loop: unrolling them lets the compiler overlap the live ranges
-        // of successive passes and spill
-#pragma unroll 1
-        for (int pass = 0; pass < NPASS; pass++)
-        {
+        // Passes stay a runtime loop (unrolling them lets the compiler overlap the live ranges of
+        // successive passes and spill), but each pass runs its own instance of the body with a
+        // compile-time pass index, so the column tests fold away and the uniform side's
+        // structural zeros are skipped per column.
+        auto passBody = [&](auto passC) {
+            // a compile-time constant for the instanced passes, a runtime index otherwise
+            const int pass = static_cast<int>(passC);
+            constexpr bool kStaticPass = !std::is_same_v<decltype(passC), int>;
             T I[SL * NC];
 #pragma unroll
             for (int e = 0; e < SL * NC; e++)
@@ -255,6 +258,7 @@ namespace lbx
                     hermiteR<L, T>(R, T(X_), T(Y_), T(Z_));
                     const cptr<T> ek = eU + static_cast<size_t>(k) * HU * SU;
                     constexpr RSumTab<L, HLn, HU> RS{};
+                    constexpr EPattern<Ua, Ub> PU{};   // the uniform side's structural zeros
 #pragma unroll
                     for (int h = 0; h < HLn; h++)
 #pragma unroll
@@ -263,16 +267,20 @@ namespace lbx
                             const T r = R[RS.v[h * HU + g]];
 #pragma unroll
                             for (int c = 0; c < NC; c++)
-                                if (pass * NC + c < SU)
+                                if (pass * NC + c < SU && (!kStaticPass || PU.nz[g][pass * NC + c]))
                                     X[h * NC + c] = fma(r, ek[g * SU + pass * NC + c], X[h * NC + c]);
                         }
                 }
                 const T *el = eL + static_cast<size_t>(i) * HLn * SL;
+                // structural zeros of the lane side's E (about half the entries) are skipped
+                constexpr EPattern<La, Lb> PL{};
 #pragma unroll
                 for (int h = 0; h < HLn; h++)
 #pragma unroll
                     for (int s = 0; s < SL; s++)
                     {
+                        if (!PL.nz[h][s])
+                            continue;
                         const T e = el[h * SL + s];
 #pragma unroll
                         for (int c = 0; c < NC; c++)
@@ -458,7 +466,8 @@ namespace lbx
                         }
                     }
             }
-        }
+        };
+        forEachPass<NPASS>(passBody);
 
         if constexpr (MODE == MODE_JK)
         {
