@@ -166,6 +166,11 @@ namespace lbx
         constexpr int NC = C::template nc<T, SWAP>();
         constexpr int NPASS = ceilDiv(SU, NC);
         constexpr bool F32 = sizeof(T) == 4;
+        // Skipping structural zeros scatters the loads of E (no wide contiguous loads); it pays
+        // where each loaded value feeds several FMAs: lane side with >= 2 columns per pass,
+        // uniform side with >= 4 Hermite rows of X.
+        constexpr bool kSkipL = NC >= 2;
+        constexpr bool kSkipU = HLn >= 4;
         static_assert(!F32 || MODE == MODE_JK, "FP32 form is for J/K only");
         const DPairs &LP = SWAP ? A.ket : A.bra;
         const DPairs &UP = SWAP ? A.bra : A.ket;
@@ -267,7 +272,8 @@ namespace lbx
                             const T r = R[RS.v[h * HU + g]];
 #pragma unroll
                             for (int c = 0; c < NC; c++)
-                                if (pass * NC + c < SU && (!kStaticPass || PU.nz[g][pass * NC + c]))
+                                if (pass * NC + c < SU &&
+                                    (!kSkipU || !kStaticPass || PU.nz[g][pass * NC + c]))
                                     X[h * NC + c] = fma(r, ek[g * SU + pass * NC + c], X[h * NC + c]);
                         }
                 }
@@ -279,7 +285,7 @@ namespace lbx
 #pragma unroll
                     for (int s = 0; s < SL; s++)
                     {
-                        if (!PL.nz[h][s])
+                        if (kSkipL && !PL.nz[h][s])
                             continue;
                         const T e = el[h * SL + s];
 #pragma unroll
