@@ -122,8 +122,9 @@ int lbx_jk_grad(lbx_context *ctx, const double *D, double *grad, double schwarz_
  * variants: 0 one lane per quartet with the bra pairs on the lanes, 1 the same with the ket
  * pairs on the lanes, 2 one wave per quartet on FP64 MFMA, and 3, 4 the forms 0, 1 with R and
  * both transforms in FP32 (the small-bound quartets of LBX_MIXED).  lbx_tune times every
- * variant of every class pair on this density and threshold (on the leading ~2M quartets of
- * each class pair, largest bounds first) and keeps the fastest per quartet: the fastest FP64 form
+ * variant of every class pair on this density and threshold (on a sample of ~2M quartets of
+ * each class pair: every k-th pair of one side with all its partners) and keeps the fastest per
+ * quartet: the fastest FP64 form
  * for FP64 work and the fastest of all forms for the FP32 tier of mixed builds; without it
  * lbx_jk picks by a work estimate from the primitive-pair counts. */
 int lbx_tune(lbx_context *ctx, const double *D, double schwarz_thresh);

@@ -552,27 +552,52 @@ struct lbx_context : Context
         a.K = K;
         if (sample_quartets > 0)
         {
-            // leading uniform-side positions up to about sample_quartets quartets
-            const long long cap = std::max(1LL, sample_quartets / s.width);
-            long long nq = 0;
-            int q = 0;
-            while (q < t.nket && t.prefix[q] < cap)
+            // every stride-th uniform-side position, about sample_quartets quartets spread over
+            // the whole range of Schwarz bounds
+            auto st = tune_tasks.find(key);
+            if (st == tune_tasks.end())
             {
-                nq += t.phi[q] - t.plo[q];
-                q++;
+                const int stride = static_cast<int>(
+                        std::max(1.0, std::ceil(t.quartets / static_cast<double>(sample_quartets))));
+                TaskTable ts;
+                ts.nket = t.nket;
+                ts.plo = t.plo;
+                ts.phi = t.phi;
+                ts.prefix.resize(t.nket + 1);
+                long long acc = 0;
+                for (int q = 0; q < t.nket; q++)
+                {
+                    if (q % stride != 0)
+                        ts.phi[q] = ts.plo[q];
+                    ts.prefix[q] = acc;
+                    const long long cnt = ts.phi[q] - ts.plo[q];
+                    acc += (cnt + s.width - 1) / s.width;
+                    ts.quartets += static_cast<double>(cnt);
+                }
+                ts.prefix[t.nket] = acc;
+                ts.ntasks = acc;
+                ts.d_prefix = upload(ts.prefix);
+                ts.d_plo = upload(ts.plo);
+                ts.d_phi = upload(ts.phi);
+                st = tune_tasks.emplace(key, std::move(ts)).first;
             }
-            a.ntasks = t.prefix[q];
-            sampled = static_cast<double>(nq);
+            const TaskTable &ts = st->second;
+            a.prefix = ts.d_prefix;
+            a.plo = ts.d_plo;
+            a.phi = ts.d_phi;
+            a.ntasks = ts.ntasks;
+            sampled = ts.quartets;
         }
         s.fn(a, stream);
         LBX_HIP(hipGetLastError());
         return t;
     }
 
-    // when > 0, launchJK runs only the first ~sample_quartets quartets (autotuner timing) and
+    // when > 0, launchJK runs only a sample of ~sample_quartets quartets (autotuner timing) and
     // leaves their count in `sampled`
     long long sample_quartets = 0;
     double sampled = 0;
+    std::map<std::tuple<int, int, double, double, int>, TaskTable> tune_tasks;   // the samples
 
     // gradient task tables per (bra class, ket class, Schwarz threshold)
     std::map<std::tuple<int, int, double>, TaskTable> grad_tasks;
