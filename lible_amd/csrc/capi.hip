@@ -536,17 +536,24 @@ struct lbx_context : Context
     {
         const PairClass &bra = cls[i], &ket = cls[j];
         const KernelSlot &s = slotFor(MODE_JK, bra, ket, var);
+        // the one-wave-per-quartet kernel walks consecutive tasks of one pair of the side whose
+        // E matrices are larger, so those stay in cache across the tasks
+        auto eBytes = [](const PairClass &c)
+        { return c.n_pairs ? double(c.n_ppairs) / c.n_pairs * nHerm(c.la + c.lb) * nSph(c.la) * nSph(c.lb) : 0.0; };
+        const bool braMajor = !s.swap && s.width == 1 && i != j && eBytes(bra) > eBytes(ket);
         const auto key = std::make_tuple(i, j, thresh, thresh_hi, var);
         auto it = jk_tasks.find(key);
         if (it == jk_tasks.end())
             it = jk_tasks
-                         .emplace(key, s.swap ? buildTasks(*this, ket, bra, Pairing::Unique, false, thresh,
-                                                           s.width, false, thresh_hi)
-                                              : buildTasks(*this, bra, ket, Pairing::Unique, i == j, thresh,
-                                                           s.width, true, thresh_hi))
+                         .emplace(key, (s.swap || braMajor)
+                                               ? buildTasks(*this, ket, bra, Pairing::Unique, false, thresh,
+                                                            s.width, false, thresh_hi)
+                                               : buildTasks(*this, bra, ket, Pairing::Unique, i == j, thresh,
+                                                            s.width, true, thresh_hi))
                          .first;
         const TaskTable &t = it->second;
         KArgs a = baseArgs(*this, bra, ket, t, i == j);
+        a.bra_major = braMajor ? 1 : 0;
         a.D = D;
         a.J = J;
         a.K = K;

@@ -49,6 +49,7 @@ namespace lbx
         int nket;
         long long ntasks;
         int same_class;            // bra class == ket class: quartet p == q is its own image
+        int bra_major;             // quartetM: q indexes bra pairs, p ket pairs (classes differ)
         const double *boys;
         // MODE_JK
         const double *D;
@@ -580,8 +581,10 @@ namespace lbx
                 next = A.prefix[q + 1];
             }
             const int p = uniform(A.plo[q] + static_cast<int>(t - base));
-            const int cd = uniform(A.ket.order[q]);
-            const int ab = uniform(A.bra.order[p]);
+            // task tables run over the ket pairs of one bra pair when bra_major is set (the
+            // larger E matrix is then shared by consecutive tasks and stays in cache)
+            const int cd = uniform(A.ket.order[A.bra_major ? p : q]);
+            const int ab = uniform(A.bra.order[A.bra_major ? q : p]);
             const int kpoff = uniform(A.ket.poff[cd]), kcnt = uniform(A.ket.pcnt[cd]);
             const int bpoff = uniform(A.bra.poff[ab]), bcnt = uniform(A.bra.pcnt[ab]);
             const int nrows = bcnt * HP;
