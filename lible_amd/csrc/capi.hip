@@ -145,6 +145,13 @@ namespace lbx
             double quartets = 0, flops = 0;
             // over the quartets: sum of K_bra K_ket, of K_bra, of K_ket (canonical roles)
             double sum_kk = 0, sum_kbra = 0, sum_kket = 0;
+            // lane-block-major enumeration of a 64-wide table (empty unless built)
+            int nblk = 0, gshift = 0;
+            long long btasks = 0;
+            std::vector<long long> bprefix;
+            std::vector<int> bq0;
+            long long *d_bprefix = nullptr;
+            int *d_bq0 = nullptr;
         };
 
         enum class Pairing
@@ -242,6 +249,53 @@ namespace lbx
             t.d_plo = c.upload(t.plo);
             t.d_phi = c.upload(t.phi);
             return t;
+        }
+
+        // Lane-block-major enumeration of a 64-wide table.  Lane block b = positions 64b .. 64b+63,
+        // group g = blocks g G .. g G + G - 1 (G = 2^gshift).  Group g takes the uniform positions
+        // q = bq0[g] .. bq0[g] + n_g - 1, the smallest range holding every q whose [plo, phi)
+        // meets the group, each q with the group's G blocks in turn: G n_g tasks (a task whose
+        // block misses q's range has every lane idle and returns at once).  Only groups with
+        // g % stride == 0 get tasks (the autotuner's samples); returns the quartets they hold.
+        double buildBlockMajor(Context &c, TaskTable &t, int nlane, int stride, int gshift)
+        {
+            const int nblk = (nlane + 63) / 64, gsz = 64 << gshift;
+            const int ng = (nlane + gsz - 1) / gsz;
+            std::vector<int> qmin(ng, std::numeric_limits<int>::max()), qmax(ng, -1);
+            std::vector<double> quart(ng, 0.0);
+            for (int q = 0; q < t.nket; q++)
+            {
+                const int lo = t.plo[q], hi = t.phi[q];
+                if (hi <= lo)
+                    continue;
+                for (int g = lo / gsz; g <= (hi - 1) / gsz; g++)
+                {
+                    qmin[g] = std::min(qmin[g], q);
+                    qmax[g] = q;
+                    quart[g] += std::min(hi, gsz * (g + 1)) - std::max(lo, gsz * g);
+                }
+            }
+            t.nblk = nblk;
+            t.gshift = gshift;
+            t.bprefix.assign(ng + 1, 0);
+            t.bq0.assign(ng, 0);
+            long long acc = 0;
+            double quartets = 0;
+            for (int g = 0; g < ng; g++)
+            {
+                t.bprefix[g] = acc;
+                if (qmax[g] >= 0 && g % stride == 0)
+                {
+                    t.bq0[g] = qmin[g];
+                    acc += static_cast<long long>(qmax[g] - qmin[g] + 1) << gshift;
+                    quartets += quart[g];
+                }
+            }
+            t.bprefix[ng] = acc;
+            t.btasks = acc;
+            t.d_bprefix = c.upload(t.bprefix);
+            t.d_bq0 = c.upload(t.bq0);
+            return quartets;
         }
 
         // Tasks of the batched RI sums: for every uniform-side position q (identity order: a
@@ -551,13 +605,58 @@ struct lbx_context : Context
                                                : buildTasks(*this, bra, ket, Pairing::Unique, i == j, thresh,
                                                             s.width, true, thresh_hi))
                          .first;
+        // The one-lane-per-quartet kernels run their tasks by groups of lane blocks: as many
+        // blocks (a power of two, up to 64) as keep the lane side's E matrices of a group within
+        // about 1 MiB of L2, so each uniform pair's E is read once per group.
+        const bool blockMajor = s.width == 64;
+        const PairClass &lside = s.swap ? ket : bra;
+        const int nlane = lside.n_pairs;
+        int gshift = 0;
+        if (blockMajor)
+        {
+            const double groupBytes = 64.0 * eBytes(lside) * (var >= 3 ? 4 : 8);
+            while (gshift < 6 && groupBytes * (2 << gshift) <= double(1 << 20))
+                gshift++;
+        }
+        if (blockMajor && it->second.nblk == 0 && nlane > 0)
+            buildBlockMajor(*this, it->second, nlane, 1, gshift);
         const TaskTable &t = it->second;
         KArgs a = baseArgs(*this, bra, ket, t, i == j);
         a.bra_major = braMajor ? 1 : 0;
         a.D = D;
         a.J = J;
         a.K = K;
-        if (sample_quartets > 0)
+        if (blockMajor && t.nblk > 0)
+        {
+            a.block_major = 1;
+            a.nblk = t.nblk;
+            a.gshift = t.gshift;
+            a.bprefix = t.d_bprefix;
+            a.bq0 = t.d_bq0;
+            a.ntasks = t.btasks;
+        }
+        if (sample_quartets > 0 && a.block_major)
+        {
+            // every stride-th lane block with all its uniform-side partners
+            auto st = tune_tasks.find(key);
+            if (st == tune_tasks.end())
+            {
+                const int stride = static_cast<int>(
+                        std::max(1.0, std::ceil(t.quartets / static_cast<double>(sample_quartets))));
+                TaskTable ts;
+                ts.nket = t.nket;
+                ts.plo = t.plo;
+                ts.phi = t.phi;
+                ts.quartets = buildBlockMajor(*this, ts, nlane, stride, t.gshift);
+                st = tune_tasks.emplace(key, std::move(ts)).first;
+            }
+            const TaskTable &ts = st->second;
+            a.bprefix = ts.d_bprefix;
+            a.bq0 = ts.d_bq0;
+            a.ntasks = ts.btasks;
+            sampled = ts.quartets;
+        }
+        else if (sample_quartets > 0)
         {
             // every stride-th uniform-side position, about sample_quartets quartets spread over
             // the whole range of Schwarz bounds

@@ -50,6 +50,13 @@ namespace lbx
         long long ntasks;
         int same_class;            // bra class == ket class: quartet p == q is its own image
         int bra_major;             // quartetM: q indexes bra pairs, p ket pairs (classes differ)
+        // quartetV lane-block-major order: lane block b = lane positions 64b .. 64b+63; group g =
+        // blocks g G .. g G + G - 1 (G = 2^gshift).  Group g's tasks pair its uniform positions
+        // q = bq0[g], bq0[g] + 1, ... with its G blocks in turn (bprefix[g+1] - bprefix[g] = G x
+        // the count of those q).
+        int block_major, nblk, gshift;
+        const long long *bprefix;
+        const int *bq0;
         const double *boys;
         // MODE_JK
         const double *D;
@@ -153,8 +160,12 @@ namespace lbx
     // wave-uniform pair u.  SWAP = false: lanes = bra, u = ket; SWAP = true: lanes = ket, u = bra.
     // T: type of R, the E matrices and both transforms (double; float for the reduced-precision
     // J/K quartets, whose Boys values, epilogue and accumulation stay double).
-    template <int LA, int LB, int LC, int LD, int MODE, bool SWAP, typename T>
-    __device__ __forceinline__ void quartetVTask(const KArgs &A, int q, int local, int lane)
+    // jl (J/K only, SL entries per lane): this lane's J contributions to its lane-side pair,
+    // scaled by 2f.  CARRY = false: zeroed here and added to J at the end of the task; CARRY =
+    // true: accumulated into the caller's array, which adds it to J once for a run of tasks that
+    // share the lane-side pairs.
+    template <int LA, int LB, int LC, int LD, int MODE, bool SWAP, typename T, bool CARRY = false>
+    __device__ __forceinline__ void quartetVTask(const KArgs &A, int q, int local, int lane, double *jl)
     {
         using C = Cls<LA, LB, LC, LD>;
         constexpr int L = C::L, HL = C::HL;
@@ -176,11 +187,14 @@ namespace lbx
         const DPairs &LP = SWAP ? A.ket : A.bra;
         const DPairs &UP = SWAP ? A.bra : A.ket;
 
+        // lanes cover sorted positions pbase .. pbase + 63 (pbase = `local`)
         const int plo = uniform(A.plo[q]), phi = uniform(A.phi[q]);
-        const int p = plo + local * 64 + lane;
-        const bool active = p < phi;
+        if (plo >= local + 64 || phi <= local)
+            return;   // no lane has a quartet
+        const int p = local + lane;
+        const bool active = p >= plo && p < phi;
         const int u = uniform(UP.order[q]);
-        const int l = LP.order[active ? p : plo];
+        const int l = LP.order[active ? p : local];
         const int upoff = uniform(UP.poff[u]), ucnt = uniform(UP.pcnt[u]);
         const int lpoff = LP.poff[l], lcnt = active ? LP.pcnt[l] : 0;
         // the ket's E carries the sign (-1)^{t'+u'+v'}
@@ -215,10 +229,10 @@ namespace lbx
         }
         const long long n = A.nbf;
         const double *D = A.D;
-        double jl[SL];
+        if constexpr (MODE == MODE_JK && !CARRY)
 #pragma unroll
-        for (int s = 0; s < SL; s++)
-            jl[s] = 0.0;
+            for (int s = 0; s < SL; s++)
+                jl[s] = 0.0;
         double dmax = 0.0;
         double sabs = 0.0, ssum = 0.0;   // MODE_SUM
 
@@ -306,7 +320,7 @@ namespace lbx
                     if (su >= SU)
                         continue;
                     const int ga = su / SUb, de = su % SUb;
-                    const double du = D[(u0 + ga) * n + u1 + de];
+                    const double du = 2.0 * f * D[(u0 + ga) * n + u1 + de];
                     double ju = 0.0;
 #pragma unroll
                     for (int s = 0; s < SL; s++)
@@ -476,12 +490,12 @@ namespace lbx
         };
         forEachPass<NPASS>(passBody);
 
-        if constexpr (MODE == MODE_JK)
+        if constexpr (MODE == MODE_JK && !CARRY)
         {
             if (active)
 #pragma unroll
                 for (int s = 0; s < SL; s++)
-                    unsafeAtomicAdd(&A.J[(l0 + s / SLb) * n + l1 + s % SLb], 2.0 * f * jl[s]);
+                    unsafeAtomicAdd(&A.J[(l0 + s / SLb) * n + l1 + s % SLb], jl[s]);
         }
         else if constexpr (MODE == MODE_DIAG)
         {
@@ -504,14 +518,72 @@ namespace lbx
         }
     }
 
-    // Each wave takes a contiguous range of tasks (so consecutive tasks share the uniform pair
-    // and the task -> q search runs once per wave).
+    // Uniform-major tables: each wave takes a contiguous range of tasks, which share the uniform
+    // pair q and step through its 64-lane blocks (the task -> q search runs once per wave).
+    // Lane-block-major tables (A.block_major): a task is one uniform pair q against one lane
+    // block of a group of G blocks; a group's tasks are consecutive (q outer, block inner) and
+    // wave w takes tasks w, w + nw, ... with nw the resident waves, so the waves in flight share
+    // one group of lane-side pairs, whose E matrices (the larger per-task traffic) stay in L2,
+    // and each uniform pair's E is read once per group.  With G dividing nw a wave stays on one
+    // lane block within a group, so its J contributions to the lane-side pairs are summed in
+    // registers and added to J once per block.
     template <int LA, int LB, int LC, int LD, int MODE, bool SWAP, typename T>
     __global__ __launch_bounds__(256) void quartetV(const KArgs A)
     {
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
         const long long nw = static_cast<long long>(gridDim.x) * 4;
         const long long w = static_cast<long long>(blockIdx.x) * 4 + wave;
+        constexpr int SLb = nSph(SWAP ? LD : LB);
+        constexpr int SL = nSph(SWAP ? LC : LA) * SLb;
+        double jl[SL];
+        if constexpr (MODE == MODE_JK)
+        {
+            if (A.block_major)
+            {
+                const DPairs &LP = SWAP ? A.ket : A.bra;
+                auto flushJ = [&](int b) {
+                    const int p = 64 * b + lane;
+                    if (p < LP.n)
+                    {
+                        const int l = LP.order[p];
+                        const long long r0 = LP.ao_a[l], r1 = LP.ao_b[l];
+#pragma unroll
+                        for (int s = 0; s < SL; s++)
+                            unsafeAtomicAdd(&A.J[(r0 + s / SLb) * A.nbf + r1 + s % SLb], jl[s]);
+                    }
+                };
+                const int gs = A.gshift;
+                const int ng = (A.nblk + (1 << gs) - 1) >> gs;
+                int g = 0, q0 = 0, b = -1;
+                long long base = 0, next = 0;
+                for (long long t = w; t < A.ntasks; t += nw)
+                {
+                    if (t >= next)
+                    {
+                        g = uniform(findKet(A.bprefix, ng, t));
+                        base = A.bprefix[g];
+                        next = A.bprefix[g + 1];
+                        q0 = uniform(A.bq0[g]);
+                    }
+                    const long long local = t - base;
+                    const int bt = (g << gs) + static_cast<int>(local & ((1LL << gs) - 1));
+                    if (bt != b)
+                    {
+                        if (b >= 0)
+                            flushJ(b);
+#pragma unroll
+                        for (int s = 0; s < SL; s++)
+                            jl[s] = 0.0;
+                        b = bt;
+                    }
+                    quartetVTask<LA, LB, LC, LD, MODE, SWAP, T, true>(A, q0 + static_cast<int>(local >> gs), 64 * b,
+                                                                      lane, jl);
+                }
+                if (b >= 0)
+                    flushJ(b);
+                return;
+            }
+        }
         const long long t0 = A.ntasks * w / nw, t1 = A.ntasks * (w + 1) / nw;
         if (t0 >= t1)
             return;
@@ -525,7 +597,8 @@ namespace lbx
                 base = next;
                 next = A.prefix[q + 1];
             }
-            quartetVTask<LA, LB, LC, LD, MODE, SWAP, T>(A, q, static_cast<int>(t - base), lane);
+            quartetVTask<LA, LB, LC, LD, MODE, SWAP, T>(A, q, uniform(A.plo[q]) + 64 * static_cast<int>(t - base),
+                                                        lane, jl);
         }
     }
 
@@ -867,9 +940,22 @@ namespace lbx
     {
         if (A.ntasks <= 0)
             return;
-        // contiguous task ranges per wave; up to 8192 workgroups of 4 waves
+        // uniform-major: contiguous task ranges per wave, up to 8192 workgroups of 4 waves;
+        // lane-block-major: as many workgroups as are resident at once, so that one sweep of the
+        // waves over the task order keeps them all on the same few lane blocks
+        static const int resident = [] {
+            int per_cu = 0, dev = 0, ncu = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                        &per_cu, reinterpret_cast<const void *>(&quartetV<LA, LB, LC, LD, MODE, SWAP, T>), 256, 0) !=
+                        hipSuccess ||
+                hipGetDevice(&dev) != hipSuccess ||
+                hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+                return 8192;
+            return std::max(1, per_cu) * std::max(1, ncu);
+        }();
+        const long long cap = A.block_major ? std::min(resident, 8192) : 8192;
         long long wg = (A.ntasks + 3) / 4;
-        int grid = static_cast<int>(wg < 8192 ? wg : 8192);
+        int grid = static_cast<int>(wg < cap ? wg : cap);
         hipLaunchKernelGGL((quartetV<LA, LB, LC, LD, MODE, SWAP, T>), dim3(grid), dim3(256), 0, s, A);
     }
 
