@@ -72,6 +72,18 @@ int lbx_eri3(lbx_context *ctx, double *out);
 int lbx_eri2(lbx_context *ctx, double *out);
 int lbx_eri2_diag(lbx_context *ctx, double *out);
 
+/* Dense first-derivative ERI tensors, device outputs, row-major.  Component 3k + x (x, y, z) is
+ * the derivative with respect to coordinate x of the centre of the k-th index's shell, i.e. the
+ * blocks of the reference's ERI4D1 / ERI3D1 / ERI2D1 kernels ([Ax..Dz], [Ax..Cz], [Ax..Bz];
+ * eri_kernels.hpp:113-170, eri_kernel_funs.hpp:350-732) placed at their AO positions:
+ *   eri4d1 (12, nbf, nbf, nbf, nbf)   replaces ERI4D1Kernel::operator()
+ *   eri3d1 (9, nbf, nbf, naux)        replaces ERI3D1Kernel::operator()
+ *   eri2d1 (6, naux, naux)            replaces ERI2D1Kernel::operator()
+ * Small systems only (the sizes above); classes up to (ff|ff), (ff|h), (h|h). */
+int lbx_eri4d1(lbx_context *ctx, double *out);
+int lbx_eri3d1(lbx_context *ctx, double *out);
+int lbx_eri2d1(lbx_context *ctx, double *out);
+
 /* Coulomb and exchange matrices, no reference counterpart (the reference stops at the dense
  * tensor).  D, J, K: device (nbf,nbf) row-major, D symmetric.
  *   J_mn = sum_kl (mn|kl) D_kl,   K_mk = sum_nl (mn|kl) D_nl

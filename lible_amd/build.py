@@ -101,6 +101,7 @@ def build(jobs: int | None = None, verbose: bool = True, incremental: bool = Fal
         (os.path.join(CSRC, "ecoef.hip"), os.path.join(BUILD, "ecoef.o"), []),
         (os.path.join(CSRC, "capi.hip"), os.path.join(BUILD, "capi.o"), []),
         (os.path.join(CSRC, "ints_api.cpp"), os.path.join(BUILD, "ints_api.o"), []),
+        (os.path.join(CSRC, "d1.hip"), os.path.join(BUILD, "d1.o"), []),
     ]
     for u in range(N_UNITS):
         work.append((os.path.join(CSRC, "instances.hip"), os.path.join(BUILD, f"inst_{u}.o"),

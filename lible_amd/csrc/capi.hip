@@ -1152,6 +1152,129 @@ extern "C"
                        });
     }
 
+    // Dense derivative tensors (kind 4, 3, 2): every unique class pair's unique quartets, one
+    // launch per class pair, each writing its blocks' images.
+    namespace
+    {
+        void runD1(Context &c, int kind, double *out)
+        {
+            const long long n = static_cast<long long>(c.nbf), nx = static_cast<long long>(c.nbf_aux);
+            const long long total = kind == 4 ? 12 * n * n * n * n : (kind == 3 ? 9 * n * n * nx : 6 * nx * nx);
+            fill(out, total, 0.0, c.stream);
+            if (kind != 2)
+                for (auto &pc : c.cls)
+                    computeEcoeffsD1(c, pc);
+            std::vector<void *> bufs;
+            auto one = [&](const PairClass &bra, const PairClass &ket, bool same)
+            {
+                if (bra.n_pairs == 0 || ket.n_pairs == 0)
+                    return;
+                std::vector<int> tasks;
+                for (int p = 0; p < bra.n_pairs; p++)
+                    for (int q = 0; q < (same ? p + 1 : ket.n_pairs); q++)
+                    {
+                        tasks.push_back(p);
+                        tasks.push_back(q);
+                    }
+                void *d = nullptr;
+                LBX_HIP(hipMalloc(&d, tasks.size() * sizeof(int)));
+                bufs.push_back(d);
+                LBX_HIP(hipMemcpyAsync(d, tasks.data(), tasks.size() * sizeof(int), hipMemcpyHostToDevice,
+                                       c.stream));
+                D1Args a{};
+                a.bra = bra.view();
+                a.ket = ket.view();
+                a.la = bra.la;
+                a.lb = bra.lb;
+                a.lc = ket.la;
+                a.ld = ket.lb;
+                a.bra_single = bra.single ? 1 : 0;
+                a.ket_single = ket.single ? 1 : 0;
+                a.tasks = static_cast<const int *>(d);
+                a.ntasks = static_cast<long long>(tasks.size() / 2);
+                a.boys = c.d_boys;
+                a.out = out;
+                a.kind = kind;
+                a.nbf = n;
+                a.naux = nx;
+                launchD1(a, c.stream);
+                // the host task list must outlive the asynchronous copy
+                LBX_HIP(hipStreamSynchronize(c.stream));
+            };
+            try
+            {
+                if (kind == 4)
+                {
+                    for (size_t i = 0; i < c.cls.size(); i++)
+                        for (size_t j = 0; j <= i; j++)
+                            one(c.cls[i], c.cls[j], i == j);
+                }
+                else if (kind == 3)
+                {
+                    for (auto &bra : c.cls)
+                        for (auto &ket : c.aux)
+                            one(bra, ket, false);
+                }
+                else
+                {
+                    for (size_t i = 0; i < c.aux.size(); i++)
+                        for (size_t j = 0; j <= i; j++)
+                            one(c.aux[i], c.aux[j], i == j);
+                }
+            }
+            catch (...)
+            {
+                (void)hipStreamSynchronize(c.stream);
+                for (void *d : bufs)
+                    (void)hipFree(d);
+                throw;
+            }
+            LBX_HIP(hipStreamSynchronize(c.stream));
+            for (void *d : bufs)
+                (void)hipFree(d);
+        }
+    }
+
+    int lbx_eri4d1(lbx_context *ctx, double *out)
+    {
+        return guarded([&]
+                       {
+                           if (!ctx || !out)
+                               throw Error(LBX_ERR_INVALID, "lbx_eri4d1: null argument");
+                           std::lock_guard<std::mutex> lk(ctx->mu);
+                           requireSingle(*ctx, "lbx_eri4d1");
+                           runD1(*ctx, 4, out);
+                       });
+    }
+
+    int lbx_eri3d1(lbx_context *ctx, double *out)
+    {
+        return guarded([&]
+                       {
+                           if (!ctx || !out)
+                               throw Error(LBX_ERR_INVALID, "lbx_eri3d1: null argument");
+                           std::lock_guard<std::mutex> lk(ctx->mu);
+                           requireSingle(*ctx, "lbx_eri3d1");
+                           if (ctx->shells_aux.empty())
+                               throw Error(LBX_ERR_STATE, "eri3d1: RI approximation is not enabled");
+                           runD1(*ctx, 3, out);
+                       });
+    }
+
+    int lbx_eri2d1(lbx_context *ctx, double *out)
+    {
+        return guarded([&]
+                       {
+                           if (!ctx || !out)
+                               throw Error(LBX_ERR_INVALID, "lbx_eri2d1: null argument");
+                           std::lock_guard<std::mutex> lk(ctx->mu);
+                           requireSingle(*ctx, "lbx_eri2d1");
+                           if (ctx->shells_aux.empty())
+                               throw Error(LBX_ERR_STATE, "eri2d1: RI approximation is not enabled");
+                           runD1(*ctx, 2, out);
+                       });
+    }
+
     int lbx_jk(lbx_context *ctx, const double *D, double *J, double *K, double schwarz_thresh,
                int precision, double mixed_thresh, lbx_jk_stats *stats)
     {

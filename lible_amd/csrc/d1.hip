@@ -1,0 +1,306 @@
+// Dense first-derivative ERI tensors: the quantities of the reference's ERI4D1 / ERI3D1 /
+// ERI2D1 kernels (eri_kernel_funs.hpp:350-732, eri_kernel_funs.cpp:582-748), written as dense
+// tensors whose component 3k + x is the derivative with respect to coordinate x of the centre of
+// the k-th index's shell.
+//
+// For a quartet (ab|cd) with bra primitive pairs i (exponents a, b, p = a + b) and ket primitive
+// pairs k (c, d, q = c + d), and the Hermite form (ab|cd) = sum_ik Eb_i R_{h+g} Ek_k:
+//   d/dA_x = sum_ik (a/p) Eb_i R_{h+g+e_x} Ek_k + Eb'_i,x R_{h+g} Ek_k
+//   d/dB_x = sum_ik (b/p) Eb_i R_{h+g+e_x} Ek_k - Eb'_i,x R_{h+g} Ek_k
+//   d/dC_x = sum_ik -(c/q) Eb_i R_{h+g+e_x} Ek_k + Eb_i R_{h+g} Ek'_k,x
+//   d/dD_x = sum_ik -(d/q) Eb_i R_{h+g+e_x} Ek_k - Eb_i R_{h+g} Ek'_k,x
+// (E' = dE/d(A - B)_x of the pair, the ket's (-1)^{t'+u'+v'} in Ek and Ek'; d/dP_x of the Hermite
+// Gaussian raises its x index).  Single-shell sides (the auxiliary shells of eri3 / eri2) are an
+// s function of exponent 0 at the shell's centre: E' = 0, a/p = 1, and the dummy centre's
+// derivative vanishes, so the 3- and 2-centre tensors are the 4-centre formulas restricted to
+// the real centres.
+//
+// This is the compatibility path of the derivative kernels (dense tensors are only feasible for
+// small systems; the nuclear gradient of J/K contracts them on the fly in grad.hpp): one kernel
+// for every class pair with the angular momenta at run time, one wave per quartet, Hermite R and
+// the ket transform staged in LDS, the output block of one component accumulated in registers
+// (lanes own output elements), one component per pass over the primitive quartets.
+#include "device_common.hpp"
+#include "lbx_internal.hpp"
+
+namespace lbx
+{
+    namespace
+    {
+        constexpr int kD1MaxL1 = 4 * kMaxLMain + 1;             // highest R level, (ff|ff) + 1
+        constexpr int kD1MaxHerm = nHerm(kD1MaxL1);             // 560
+        constexpr int kD1MaxY = nHerm(2 * kMaxLMain) * 49;      // H(ff) x S(ff)
+        constexpr int kD1MaxOut = 49 * 49;                      // S(ff) x S(ff)
+        constexpr int kD1Acc = ceilDiv(kD1MaxOut, 64);          // output elements per lane
+
+        __device__ __forceinline__ void boysRuntime(double x, const double *__restrict__ tab, int L,
+                                                    double *F)
+        {
+            if (x == 0.0)
+            {
+                F[0] = 1.0;
+                for (int k = 1; k <= L; k++)
+                    F[k] = 1.0 / (2 * k + 1);
+            }
+            else if (x > kBoysLargeX)
+            {
+                const double rx = 1.0 / x;
+                F[0] = 0.5 * sqrt(M_PI * rx);
+                for (int k = 1; k <= L; k++)
+                    F[k] = F[k - 1] * (k - 0.5) * rx;
+            }
+            else
+            {
+                const int iv = static_cast<int>(x * 100.0);
+                const double dx = x - iv * kBoysStep;
+                const double *row = tab + iv * kBoysCols + L;
+                const double m = -dx;
+                double s = row[6] * (1.0 / 720.0);
+                s = fma(s, m, row[5] * (1.0 / 120.0));
+                s = fma(s, m, row[4] * (1.0 / 24.0));
+                s = fma(s, m, row[3] * (1.0 / 6.0));
+                s = fma(s, m, row[2] * 0.5);
+                s = fma(s, m, row[1]);
+                s = fma(s, m, row[0]);
+                F[L] = s;
+                const double ex = exp(-x);
+                const double x2 = 2.0 * x;
+                for (int k = L - 1; k >= 0; k--)
+                    F[k] = fma(x2, F[k + 1], ex) * (1.0 / (2 * k + 1));
+            }
+        }
+
+        // image permutations: image position j holds original index perm[j]
+        __constant__ int kImg4[8][4] = {{0, 1, 2, 3}, {1, 0, 2, 3}, {0, 1, 3, 2}, {1, 0, 3, 2},
+                                        {2, 3, 0, 1}, {3, 2, 0, 1}, {2, 3, 1, 0}, {3, 2, 1, 0}};
+
+        __global__ __launch_bounds__(64) void d1Kernel(const D1Args A)
+        {
+            __shared__ double sR[2][kD1MaxHerm];
+            __shared__ double sY[kD1MaxY];
+            __shared__ double sF[kD1MaxL1 + 1];
+            __shared__ int sTuv[kD1MaxHerm];
+            const int lane = threadIdx.x;
+            const int la = A.la, lb = A.lb, lc = A.lc, ld = A.ld;
+            const int L1 = la + lb + lc + ld + 1;
+            const int Hab = nHerm(la + lb), Hcd = nHerm(lc + ld), H1 = nHerm(L1);
+            const int Sa = nSph(la), Sb = nSph(lb), Sc = nSph(lc), Sd = nSph(ld);
+            const int Sab = Sa * Sb, Scd = Sc * Sd, nOut = Sab * Scd;
+            for (int h = lane; h < H1; h += 64)
+                sTuv[h] = hermTuv(h);
+            __syncthreads();
+
+            // components: kind 4 centres A B C D, kind 3 A B C, kind 2 A C
+            const int ncent = A.kind == 4 ? 4 : (A.kind == 3 ? 3 : 2);
+            const long long n = A.nbf;
+            for (long long task = blockIdx.x; task < A.ntasks; task += gridDim.x)
+            {
+                const int p = A.tasks[2 * task], q = A.tasks[2 * task + 1];
+                const int poff = A.bra.poff[p], pcnt = A.bra.pcnt[p];
+                const int qoff = A.ket.poff[q], qcnt = A.ket.pcnt[q];
+                const int ao[4] = {A.bra.ao_a[p], A.bra.ao_b[p], A.ket.ao_a[q], A.ket.ao_b[q]};
+                for (int comp = 0; comp < 3 * ncent; comp++)
+                {
+                    const int ci = comp / 3, x = comp % 3;
+                    // internal centre: 0 A, 1 B, 2 C, 3 D
+                    const int cent = A.kind == 2 ? (ci == 0 ? 0 : 2) : ci;
+                    const int ex = x == 0, ey = x == 1, ez = x == 2;
+                    double acc[kD1Acc];
+#pragma unroll
+                    for (int r = 0; r < kD1Acc; r++)
+                        acc[r] = 0.0;
+
+                    for (int i = poff; i < poff + pcnt; i++)
+                    {
+                        const double4 P = A.bra.pp[i];
+                        const double rai = A.bra_single ? 1.0 : A.bra.ra[i];
+                        for (int k = qoff; k < qoff + qcnt; k++)
+                        {
+                            const double4 Q = A.ket.pp[k];
+                            const double rck = A.ket_single ? 1.0 : A.ket.ra[k];
+                            const double pq = P.x * Q.x, ps = P.x + Q.x, alpha = pq / ps;
+                            const double X = P.y - Q.y, Y = P.z - Q.z, Z = P.w - Q.w;
+                            if (lane == 0)
+                                boysRuntime(alpha * (X * X + Y * Y + Z * Z), A.boys, L1, sF);
+                            __syncthreads();
+                            // Hermite R: level m from level m + 1, m = L1 .. 0 (HJO 9.9.18-20)
+                            const double fac = kTwoPi52 / (pq * sqrt(ps));
+                            int cur = 0;
+                            if (lane == 0)
+                                sR[0][0] = sF[L1] * fac * pow(-2.0 * alpha, L1);
+                            __syncthreads();
+                            for (int m = L1 - 1; m >= 0; m--)
+                            {
+                                const int nxt = cur ^ 1, top = nHerm(L1 - m);
+                                for (int e = lane; e < top; e += 64)
+                                {
+                                    const int tuv = sTuv[e];
+                                    const int t = tuv & 255, u = (tuv >> 8) & 255, v = tuv >> 16;
+                                    double r;
+                                    if (t > 0)
+                                        r = (t > 1 ? (t - 1) * sR[cur][hermIdx(t - 2, u, v)] : 0.0) +
+                                            X * sR[cur][hermIdx(t - 1, u, v)];
+                                    else if (u > 0)
+                                        r = (u > 1 ? (u - 1) * sR[cur][hermIdx(t, u - 2, v)] : 0.0) +
+                                            Y * sR[cur][hermIdx(t, u - 1, v)];
+                                    else if (v > 0)
+                                        r = (v > 1 ? (v - 1) * sR[cur][hermIdx(t, u, v - 2)] : 0.0) +
+                                            Z * sR[cur][hermIdx(t, u, v - 1)];
+                                    else
+                                        r = sF[m] * fac * pow(-2.0 * alpha, m);
+                                    sR[nxt][e] = r;
+                                }
+                                __syncthreads();
+                                cur = nxt;
+                            }
+                            const double *R = sR[cur];
+
+                            // ket transform: Y = aT sum_g R_{h+g+e_x} Ek_g + a0 sum_g R_{h+g} Ek'_g
+                            double aT, a0 = 0.0;
+                            if (cent == 0)
+                                aT = rai;
+                            else if (cent == 1)
+                                aT = 1.0 - rai;
+                            else if (cent == 2)
+                            {
+                                aT = -rck;
+                                a0 = A.ket_single ? 0.0 : 1.0;
+                            }
+                            else
+                            {
+                                aT = -(1.0 - rck);
+                                a0 = A.ket_single ? 0.0 : -1.0;
+                            }
+                            const double *ek = A.ket.ek + static_cast<size_t>(k) * Hcd * Scd;
+                            const double *e1k =
+                                    a0 != 0.0 ? A.ket.e1k + (static_cast<size_t>(k) * 3 + x) * Hcd * Scd : nullptr;
+                            for (int e = lane; e < Hab * Scd; e += 64)
+                            {
+                                const int h = e / Scd, s = e - h * Scd;
+                                const int th = sTuv[h];
+                                const int t = th & 255, u = (th >> 8) & 255, v = th >> 16;
+                                double y = 0.0;
+                                for (int g = 0; g < Hcd; g++)
+                                {
+                                    const int tg = sTuv[g];
+                                    const int t2 = t + (tg & 255), u2 = u + ((tg >> 8) & 255), v2 = v + (tg >> 16);
+                                    double w = aT * R[hermIdx(t2 + ex, u2 + ey, v2 + ez)] * ek[g * Scd + s];
+                                    if (e1k)
+                                        w = fma(a0 * R[hermIdx(t2, u2, v2)], e1k[g * Scd + s], w);
+                                    y += w;
+                                }
+                                sY[e] = y;
+                            }
+                            __syncthreads();
+                            // bra transform into this lane's output elements
+                            const double *eb = A.bra.eb + static_cast<size_t>(i) * Hab * Sab;
+#pragma unroll
+                            for (int r = 0; r < kD1Acc; r++)
+                            {
+                                const int o = lane + 64 * r;
+                                if (o < nOut)
+                                {
+                                    const int sab = o / Scd, s = o - sab * Scd;
+                                    double a = acc[r];
+                                    for (int h = 0; h < Hab; h++)
+                                        a = fma(eb[h * Sab + sab], sY[h * Scd + s], a);
+                                    acc[r] = a;
+                                }
+                            }
+                            __syncthreads();
+                            // the bra pair's own derivative: +- Eb'_x R_{h+g} Ek
+                            if (cent <= 1 && !A.bra_single)
+                            {
+                                for (int e = lane; e < Hab * Scd; e += 64)
+                                {
+                                    const int h = e / Scd, s = e - h * Scd;
+                                    const int th = sTuv[h];
+                                    const int t = th & 255, u = (th >> 8) & 255, v = th >> 16;
+                                    double y = 0.0;
+                                    for (int g = 0; g < Hcd; g++)
+                                    {
+                                        const int tg = sTuv[g];
+                                        y = fma(R[hermIdx(t + (tg & 255), u + ((tg >> 8) & 255), v + (tg >> 16))],
+                                                ek[g * Scd + s], y);
+                                    }
+                                    sY[e] = y;
+                                }
+                                __syncthreads();
+                                const double sg = cent == 0 ? 1.0 : -1.0;
+                                const double *e1b = A.bra.e1b + (static_cast<size_t>(i) * 3 + x) * Hab * Sab;
+#pragma unroll
+                                for (int r = 0; r < kD1Acc; r++)
+                                {
+                                    const int o = lane + 64 * r;
+                                    if (o < nOut)
+                                    {
+                                        const int sab = o / Scd, s = o - sab * Scd;
+                                        double a = 0.0;
+                                        for (int h = 0; h < Hab; h++)
+                                            a = fma(e1b[h * Sab + sab], sY[h * Scd + s], a);
+                                        acc[r] = fma(sg, a, acc[r]);
+                                    }
+                                }
+                                __syncthreads();
+                            }
+                        }
+                    }
+
+                    // scatter this component's block to every image
+#pragma unroll
+                    for (int r = 0; r < kD1Acc; r++)
+                    {
+                        const int o = lane + 64 * r;
+                        if (o >= nOut)
+                            continue;
+                        const int sab = o / Scd, scd = o - sab * Scd;
+                        const int idx[4] = {ao[0] + sab / Sb, ao[1] + sab % Sb, ao[2] + scd / Sd,
+                                            ao[3] + scd % Sd};
+                        const double val = acc[r];
+                        if (A.kind == 4)
+                        {
+                            const long long n4 = n * n * n * n;
+                            for (int im = 0; im < 8; im++)
+                            {
+                                int pos = 0;
+                                for (int j = 0; j < 4; j++)
+                                    if (kImg4[im][j] == cent)
+                                        pos = j;
+                                const long long e = ((static_cast<long long>(idx[kImg4[im][0]]) * n +
+                                                      idx[kImg4[im][1]]) * n + idx[kImg4[im][2]]) * n +
+                                                    idx[kImg4[im][3]];
+                                A.out[(3 * pos + x) * n4 + e] = val;
+                            }
+                        }
+                        else if (A.kind == 3)
+                        {
+                            const long long nx = A.naux, n3 = n * n * nx;
+                            const int pos0 = cent, pos1 = cent == 2 ? 2 : 1 - cent;
+                            A.out[(3 * pos0 + x) * n3 + (static_cast<long long>(idx[0]) * n + idx[1]) * nx + idx[2]] = val;
+                            A.out[(3 * pos1 + x) * n3 + (static_cast<long long>(idx[1]) * n + idx[0]) * nx + idx[2]] = val;
+                        }
+                        else
+                        {
+                            const long long nx = A.naux, n2 = nx * nx;
+                            const int pos = cent == 0 ? 0 : 1;
+                            A.out[(3 * pos + x) * n2 + static_cast<long long>(idx[0]) * nx + idx[2]] = val;
+                            A.out[(3 * (1 - pos) + x) * n2 + static_cast<long long>(idx[2]) * nx + idx[0]] = val;
+                        }
+                    }
+                }
+            }
+        }
+    }
+
+    void launchD1(const D1Args &a, hipStream_t s)
+    {
+        if (a.ntasks <= 0)
+            return;
+        if (a.la + a.lb + a.lc + a.ld + 1 > kD1MaxL1 || nHerm(a.la + a.lb) * nSph(a.lc) * nSph(a.ld) > kD1MaxY ||
+            nSph(a.la) * nSph(a.lb) * nSph(a.lc) * nSph(a.ld) > kD1MaxOut)
+            throw Error(LBX_ERR_INVALID, "derivative tensors: class beyond the kernel's limits");
+        const int grid = static_cast<int>(a.ntasks < 16384 ? a.ntasks : 16384);
+        hipLaunchKernelGGL(d1Kernel, dim3(grid), dim3(64), 0, s, a);
+        LBX_HIP(hipGetLastError());
+    }
+}

@@ -173,4 +173,19 @@ namespace lbx
 
     // schwarz / task tables (capi.cpp)
     void ensureSchwarz(Context &c);
+
+    // d1.hip: dense first-derivative tensors of one class pair
+    struct D1Args
+    {
+        DPairs bra, ket;
+        int la, lb, lc, ld;          // classes; a single-shell side has lb (ld) = 0
+        int bra_single, ket_single;  // single-shell sides (no E', a/p = 1)
+        const int *tasks;            // (bra pair, ket pair) per task, pair indices
+        long long ntasks;
+        const double *boys;
+        double *out;
+        int kind;                    // 4: (12,n,n,n,n); 3: (9,n,n,naux); 2: (6,naux,naux)
+        long long nbf, naux;
+    };
+    void launchD1(const D1Args &a, hipStream_t s);
 }

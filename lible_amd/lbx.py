@@ -20,7 +20,7 @@ LBX_MIXED = 1
 # every symbol include/lbx.h declares
 SYMBOLS = ["lbx_create", "lbx_destroy", "lbx_last_error", "lbx_dims", "lbx_set_stream",
            "lbx_synchronize", "lbx_boys", "lbx_eri4", "lbx_eri4_diag", "lbx_eri3", "lbx_eri2",
-           "lbx_eri2_diag", "lbx_jk", "lbx_create_batch", "lbx_ri_sums", "lbx_jk_grad",
+           "lbx_eri2_diag", "lbx_eri4d1", "lbx_eri3d1", "lbx_eri2d1", "lbx_jk", "lbx_create_batch", "lbx_ri_sums", "lbx_jk_grad",
            "lbx_tune", "lbx_set_variant", "lbx_profile_enable",
            "lbx_profile_read"]
 VARIANTS = ("valu-bra", "valu-ket", "mfma", "valu-bra-f32", "valu-ket-f32")
@@ -75,7 +75,8 @@ def lib():
         L.lbx_set_stream.argtypes = [p, p]
         L.lbx_synchronize.argtypes = [p]
         L.lbx_boys.argtypes = [i, l, p, p, p]
-        for name in ("lbx_eri4", "lbx_eri4_diag", "lbx_eri3", "lbx_eri2", "lbx_eri2_diag"):
+        for name in ("lbx_eri4", "lbx_eri4_diag", "lbx_eri3", "lbx_eri2", "lbx_eri2_diag",
+                     "lbx_eri4d1", "lbx_eri3d1", "lbx_eri2d1"):
             getattr(L, name).argtypes = [p, p]
         L.lbx_jk.argtypes = [p, p, p, p, d, i, d, ctypes.POINTER(JKStats)]
         L.lbx_jk_grad.argtypes = [p, p, p, d, ctypes.POINTER(JKStats)]
@@ -210,6 +211,26 @@ class Context:
     def eri2_diag(self):
         out = self._out((self.nbf_aux,))
         _check(lib().lbx_eri2_diag(self.h, out.data_ptr()))
+        return out
+
+    def eri4d1(self):
+        """(12, nbf, nbf, nbf, nbf): derivatives of (mn|kl) with respect to the centres of the
+        shells of m, n, k, l, x y z each (lbx_eri4d1)."""
+        n = self.nbf
+        out = self._out((12, n, n, n, n))
+        _check(lib().lbx_eri4d1(self.h, out.data_ptr()))
+        return out
+
+    def eri3d1(self):
+        """(9, nbf, nbf, naux) (lbx_eri3d1)."""
+        out = self._out((9, self.nbf, self.nbf, self.nbf_aux))
+        _check(lib().lbx_eri3d1(self.h, out.data_ptr()))
+        return out
+
+    def eri2d1(self):
+        """(6, naux, naux) (lbx_eri2d1)."""
+        out = self._out((6, self.nbf_aux, self.nbf_aux))
+        _check(lib().lbx_eri2d1(self.h, out.data_ptr()))
         return out
 
     def jk_grad(self, D, schwarz: float = 1e-12, grad=None):

@@ -236,6 +236,105 @@ extern "C"
                      });
     }
 
+    // Dense first-derivative tensors from the per-block D1 kernels over every ordered shell pair
+    // (shellPairData(false), the pairs of the reference's deploy tests, tests_ints.cpp:961-1053):
+    // component c = 3 k + x is the derivative with respect to coordinate x of the centre of the
+    // k-th index's shell.
+    //   eri4d1 (12, nbf, nbf, nbf, nbf), eri3d1 (9, nbf, nbf, naux), eri2d1 (6, naux, naux)
+    int orc_eri4d1(void *h, double *out)
+    {
+        return guard([&]
+                     {
+                         const Structure &s = *static_cast<Structure *>(h);
+                         const size_t n = s.nbf, n4 = n * n * n * n;
+                         std::fill(out, out + 12 * n4, 0.0);
+                         auto sp = shellPairData(false, s);
+                         for (const auto &A : sp)
+                             for (const auto &C : sp)
+                             {
+                                 const int na = nSph(A.la), nb = nSph(A.lb), nc = nSph(C.la), nd = nSph(C.lb);
+                                 const size_t blk = size_t(na) * nb * nc * nd;
+                                 ERI4D1Kernel kern(A, C);
+#pragma omp parallel for schedule(dynamic)
+                                 for (size_t p = 0; p < A.n_pairs; p++)
+                                     for (size_t q = 0; q < C.n_pairs; q++)
+                                     {
+                                         auto b = kern(p, q, A, C);
+                                         const size_t oa = A.offsets_sph[2 * p], ob = A.offsets_sph[2 * p + 1];
+                                         const size_t oc = C.offsets_sph[2 * q], od = C.offsets_sph[2 * q + 1];
+                                         for (int c = 0; c < 12; c++)
+                                             for (int ia = 0; ia < na; ia++)
+                                                 for (int ib = 0; ib < nb; ib++)
+                                                     for (int ic = 0; ic < nc; ic++)
+                                                         for (int id = 0; id < nd; id++)
+                                                             out[c * n4 + (((oa + ia) * n + ob + ib) * n + oc + ic) * n + od + id] =
+                                                                     b[c * blk + ((ia * nb + ib) * nc + ic) * nd + id];
+                                     }
+                             }
+                     });
+    }
+
+    int orc_eri3d1(void *h, double *out)
+    {
+        return guard([&]
+                     {
+                         const Structure &s = *static_cast<Structure *>(h);
+                         const size_t n = s.nbf, nx = s.nbf_aux, n3 = n * n * nx;
+                         std::fill(out, out + 9 * n3, 0.0);
+                         auto sp = shellPairData(false, s);
+                         auto sh = shellDataAux(s);
+                         for (const auto &A : sp)
+                             for (const auto &C : sh)
+                             {
+                                 const int na = nSph(A.la), nb = nSph(A.lb), nc = nSph(C.l);
+                                 const size_t blk = size_t(na) * nb * nc;
+                                 ERI3D1Kernel kern(A, C);
+#pragma omp parallel for schedule(dynamic)
+                                 for (size_t p = 0; p < A.n_pairs; p++)
+                                     for (size_t q = 0; q < C.n_shells; q++)
+                                     {
+                                         auto b = kern(p, q, A, C);
+                                         const size_t oa = A.offsets_sph[2 * p], ob = A.offsets_sph[2 * p + 1];
+                                         const size_t oc = C.offsets_sph[q];
+                                         for (int c = 0; c < 9; c++)
+                                             for (int ia = 0; ia < na; ia++)
+                                                 for (int ib = 0; ib < nb; ib++)
+                                                     for (int ic = 0; ic < nc; ic++)
+                                                         out[c * n3 + ((oa + ia) * n + ob + ib) * nx + oc + ic] =
+                                                                 b[c * blk + (ia * nb + ib) * nc + ic];
+                                     }
+                             }
+                     });
+    }
+
+    int orc_eri2d1(void *h, double *out)
+    {
+        return guard([&]
+                     {
+                         const Structure &s = *static_cast<Structure *>(h);
+                         const size_t nx = s.nbf_aux, n2 = nx * nx;
+                         std::fill(out, out + 6 * n2, 0.0);
+                         auto sh = shellDataAux(s);
+                         for (const auto &A : sh)
+                             for (const auto &B : sh)
+                             {
+                                 const int na = nSph(A.l), nb = nSph(B.l);
+                                 const size_t blk = size_t(na) * nb;
+                                 ERI2D1Kernel kern(A, B);
+                                 for (size_t p = 0; p < A.n_shells; p++)
+                                     for (size_t q = 0; q < B.n_shells; q++)
+                                     {
+                                         auto b = kern(p, q, A, B);
+                                         const size_t oa = A.offsets_sph[p], ob = B.offsets_sph[q];
+                                         for (int c = 0; c < 6; c++)
+                                             for (int ia = 0; ia < na; ia++)
+                                                 for (int ib = 0; ib < nb; ib++)
+                                                     out[c * n2 + (oa + ia) * nx + ob + ib] = b[c * blk + ia * nb + ib];
+                                     }
+                             }
+                     });
+    }
+
     // Sum of |values| over every kernel call of the reference's "deploy" tests
     // (tests_ints.cpp:873-1053).  which: 0 ERI4, 1 ERI3, 2 ERI2, 3 ERI4D1, 4 ERI3D1, 5 ERI2D1.
     int orc_deploy_sum(void *h, int which, int use_symm, double *sum)

@@ -39,6 +39,8 @@ def lib():
         L.orc_structure_norms.argtypes = [_p, _p]
         for name in ("orc_eri4", "orc_eri4_diag", "orc_eri3", "orc_eri2", "orc_eri2_diag"):
             getattr(L, name).argtypes = [_p, _p]
+        for name in ("orc_eri4d1", "orc_eri3d1", "orc_eri2d1"):
+            getattr(L, name).argtypes = [_p, _p]
         L.orc_deploy_sum.argtypes = [_p, _i, _i, _p]
         L.orc_jk.argtypes = [_p, _p, _p, _p, _d, _i, _p]
         L.orc_jk_grad.argtypes = [_p, _p, _p, _d]
@@ -199,6 +201,25 @@ class Structure:
     def eri2_diag(self) -> np.ndarray:
         out = np.zeros(self.nbf_aux)
         _check(lib().orc_eri2_diag(self.h, _ptr(out)))
+        return out
+
+    def eri4d1(self) -> np.ndarray:
+        """(12, nbf, nbf, nbf, nbf): d/dR of the centre of each index's shell, x y z."""
+        n = self.nbf
+        out = np.zeros((12, n, n, n, n))
+        _check(lib().orc_eri4d1(self.h, _ptr(out)))
+        return out
+
+    def eri3d1(self) -> np.ndarray:
+        """(9, nbf, nbf, naux)."""
+        out = np.zeros((9, self.nbf, self.nbf, self.nbf_aux))
+        _check(lib().orc_eri3d1(self.h, _ptr(out)))
+        return out
+
+    def eri2d1(self) -> np.ndarray:
+        """(6, naux, naux)."""
+        out = np.zeros((6, self.nbf_aux, self.nbf_aux))
+        _check(lib().orc_eri2d1(self.h, _ptr(out)))
         return out
 
     def deploy_sum(self, which: str, use_symm: bool) -> float:

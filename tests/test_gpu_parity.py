@@ -115,6 +115,55 @@ def test_eri2_diag(lbx, oracle):
     assert abs(np.abs(got).sum() - 830.788976485982) <= REL * 830.79     # :149
 
 
+def unique_pair_mask(shells) -> np.ndarray:
+    """(nbf, nbf) mask of the AO pairs whose shell pair is in shellPairData(true): l_a > l_b, or
+    l_a == l_b with shell b not after shell a (shell_pair_data.cpp:73-202)."""
+    l = np.asarray(shells.l)
+    sh = np.repeat(np.arange(len(l)), 2 * l + 1)
+    la, lb = l[sh][:, None], l[sh][None, :]
+    return (la > lb) | ((la == lb) & (sh[:, None] >= sh[None, :]))
+
+
+def test_eri4d1_o3(lbx, oracle):
+    m = mol("def2-svp", O3)
+    got = lbx.Context(m).eri4d1().cpu().numpy()
+    ref = oracle.Structure(m).eri4d1()
+    close(got, ref)
+    # deployERI4D1Kernel (tests_ints.cpp:961-990): sum of |values| over unique pairs x pairs
+    mk = unique_pair_mask(m.shells)
+    kat = np.abs(got).sum(axis=0)[mk][:, mk].sum()
+    assert abs(kat - 30546.780073140635) <= REL * 30546.78, kat
+    # translational invariance: the four centres' derivatives sum to zero
+    assert np.abs(got[0:3] + got[3:6] + got[6:9] + got[9:12]).max() <= 1e-12 * np.abs(got).max()
+
+
+def test_eri3d1_eri2d1_o3(lbx, oracle):
+    m = mol("def2-svp", O3, "def2-universal-jkfit")
+    ctx = lbx.Context(m)
+    s = oracle.Structure(m)
+    g3 = ctx.eri3d1().cpu().numpy()
+    close(g3, s.eri3d1())
+    assert abs(np.abs(g3).sum() - 78466.360848033844) <= REL * 78466.36   # tests_ints.cpp:992-1022
+    g2 = ctx.eri2d1().cpu().numpy()
+    close(g2, s.eri2d1())
+    assert abs(np.abs(g2).sum() - 39949.218421684251) <= REL * 39949.22   # tests_ints.cpp:1024-1053
+
+
+def test_d1_f_and_g_shells(lbx, oracle):
+    # f main shells (up to (ff|ff), R to order 13) and g auxiliary shells
+    main = [(0, 1, 3), (1, 1, 2), (2, 1, 1), (3, 1, 1)]
+    aux = [(0, 1, 2), (1, 1, 1), (2, 1, 1), (3, 1, 1), (4, 1, 1)]
+    m = B.synthetic_molecule(31, 2, 3.0, main, aux)
+    ctx = lbx.Context(m)
+    s = oracle.Structure(m)
+    # Elements that cancel to ~0 carry rounding noise of the order-13 Hermite sums: up to ~4e-14
+    # of the largest element here (both sides FP64, different summation orders), hence a 1e-13
+    # floor; a wrong term or sign moves elements by O(1) of the scale.
+    close(ctx.eri4d1().cpu().numpy(), s.eri4d1(), floor=1e-13)
+    close(ctx.eri3d1().cpu().numpy(), s.eri3d1(), floor=1e-13)
+    close(ctx.eri2d1().cpu().numpy(), s.eri2d1(), floor=1e-13)
+
+
 CLASS_PAIRS = [(la, lb, lc, ld) for la in range(4) for lb in range(la + 1)
                for lc in range(4) for ld in range(lc + 1)
                if la * (la + 1) // 2 + lb >= lc * (lc + 1) // 2 + ld]
