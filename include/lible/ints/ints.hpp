@@ -19,6 +19,11 @@ struct lbx_context;
 
 namespace lible::ints
 {
+    namespace detail
+    {
+        struct Tensors;   // a structure's dense ERI tensors, computed on the GPU on first use
+    }
+
     /// Angular momentum, exponents and raw contraction coefficients of one shell
     /// (shell.hpp:10-19).
     struct BasisShell
@@ -111,6 +116,18 @@ namespace lible::ints
 
         /// The lbx context of this structure (created on the current device on first use).
         lbx_context *context() const;
+        /// The dense tensors behind the kernel functors (shared by copies of the Structure).
+        std::shared_ptr<detail::Tensors> tensors() const;
+        /// Normalised shells in AO order: l, atom, AO offset, centre (bohr), exponents, raw
+        /// coefficients, primitive norms, spherical norms (constructShells, shell.cpp:89-135).
+        struct ShellInfo
+        {
+            int l;
+            size_t atom, ofs_sph, ofs_cart;
+            std::array<double, 3> xyz;
+            std::vector<double> exps, coeffs, norms_prim, norms;
+        };
+        const std::vector<ShellInfo> &shells(bool aux) const;
 
     private:
         void init(basis_atoms_t main, const basis_atoms_t *aux);
@@ -122,6 +139,138 @@ namespace lible::ints
         std::vector<std::array<double, 3>> coords_, coords_ghost_;   // bohr
         basis_atoms_t basis_, basis_aux_;                  // real atoms, then ghost atoms
         mutable std::shared_ptr<lbx_context> ctx_;
+        mutable std::shared_ptr<detail::Tensors> tensors_;
+        mutable std::shared_ptr<std::vector<ShellInfo>> shells_, shells_aux_;
+    };
+
+    /// Contiguous data of the shells of one angular momentum (shell_pair_data.hpp:7-43): the
+    /// reference's fields, in its order and meaning.
+    struct ShellData
+    {
+        int l_{};
+        size_t n_shells_{};
+        size_t n_primitives_{};
+        std::vector<double> coeffs_;   // contraction coefficients times primitive norms
+        std::vector<double> coords_;
+        std::vector<double> exps_;
+        std::vector<double> norms_;
+        std::vector<size_t> atomic_idxs_;
+        std::vector<size_t> cdepths_;
+        std::vector<size_t> coffsets_;
+        std::vector<size_t> offsets_ecoeffs_;
+        std::vector<size_t> offsets_norms_;
+        std::vector<size_t> offsets_sph_;
+        std::vector<size_t> shell_idxs_;
+        /// (not in the reference) the tensors the kernel functors read
+        std::shared_ptr<detail::Tensors> tensors_;
+    };
+
+    /// Contiguous data of the screened shell pairs of one class (shell_pair_data.hpp:45-119),
+    /// primitive pairs kept where |N_a d_a N_b d_b exp(-ab/(a+b) R_AB^2)| >= primitives_thrs_.
+    struct ShellPairData
+    {
+        bool uses_symm_{};
+        double primitives_thrs_{};
+        int la_{};
+        int lb_{};
+        size_t n_pairs_{};
+        size_t n_pairs_total_{};
+        size_t n_ppairs_{};
+        size_t n_ppairs_total_{};
+        std::vector<double> exps_;     // (a, b) per primitive pair
+        std::vector<double> coeffs_;   // (N_a d_a, N_b d_b) per primitive pair
+        std::vector<double> coords_;   // (A, B) per pair
+        std::vector<double> norms_;    // spherical norms of a, then b, per pair
+        std::vector<size_t> nrs_ppairs_;
+        std::vector<size_t> offsets_primitives_;
+        std::vector<size_t> offsets_sph_;   // (a, b) per pair
+        std::vector<size_t> offsets_cart_;  // (a, b) per pair
+        std::vector<size_t> offsets_norms_;
+        std::vector<size_t> offsets_ecoeffs_;
+        std::vector<size_t> offsets_ecoeffs_deriv1_;
+        std::vector<size_t> atomic_idxs_;   // (a, b) per pair
+        std::vector<size_t> shell_idxs_;    // (a, b) per pair
+        /// (not in the reference) the tensors the kernel functors read
+        std::shared_ptr<detail::Tensors> tensors_;
+
+        std::pair<int, int> getLPair() const { return {la_, lb_}; }
+    };
+
+    /// Shell-pair data of every class of the structure (shell_pair_data.cpp:294-309): with
+    /// use_symm the classes la >= lb and, within la == lb, pairs b <= a; otherwise every
+    /// (la, lb) and every ordered pair.
+    std::vector<ShellPairData> shellPairData(bool use_symm, const Structure &structure);
+    /// Shell data of the auxiliary basis, one entry per l = 0 .. max l (shell_pair_data.cpp).
+    std::vector<ShellData> shellDataAux(const Structure &structure);
+
+    /// Per-block kernels of the reference (eri_kernels.hpp:60-170), same constructors and call
+    /// operators.  Behind them the whole structure's tensor is evaluated once on the GPU (on the
+    /// first construction of a kernel of that kind) and each call returns its block, so they
+    /// suit the small systems the dense tensors fit; construction is not thread-safe, the call
+    /// operators are.  The data must come from shellPairData() / shellDataAux().
+    class ERI4Kernel
+    {
+    public:
+        ERI4Kernel(const ShellPairData &sp_data_ab, const ShellPairData &sp_data_cd);
+        vec4d operator()(size_t ipair_ab, size_t ipair_cd, const ShellPairData &sp_data_ab,
+                         const ShellPairData &sp_data_cd) const;
+    private:
+        std::shared_ptr<detail::Tensors> t_;
+    };
+
+    class ERI3Kernel
+    {
+    public:
+        ERI3Kernel(const ShellPairData &sp_data_ab, const ShellData &sh_data_c);
+        vec3d operator()(size_t ipair_ab, size_t ishell_c, const ShellPairData &sp_data_ab,
+                         const ShellData &sh_data_c) const;
+    private:
+        std::shared_ptr<detail::Tensors> t_;
+    };
+
+    class ERI2Kernel
+    {
+    public:
+        ERI2Kernel(const ShellData &sh_data_a, const ShellData &sh_data_b);
+        vec2d operator()(size_t ishell_a, size_t ishell_b, const ShellData &sh_data_a,
+                         const ShellData &sh_data_b) const;
+    private:
+        std::shared_ptr<detail::Tensors> t_;
+    };
+
+    /// 12 blocks: derivatives with respect to A, B, C, D (x, y, z each).
+    class ERI4D1Kernel
+    {
+    public:
+        ERI4D1Kernel(const ShellPairData &sp_data_ab, const ShellPairData &sp_data_cd);
+        std::array<vec4d, 12> operator()(size_t ipair_ab, size_t ipair_cd,
+                                         const ShellPairData &sp_data_ab,
+                                         const ShellPairData &sp_data_cd) const;
+    private:
+        std::shared_ptr<detail::Tensors> t_;
+    };
+
+    /// 9 blocks: A, B, C.
+    class ERI3D1Kernel
+    {
+    public:
+        ERI3D1Kernel(const ShellPairData &sp_data_ab, const ShellData &sh_data_c);
+        std::array<vec3d, 9> operator()(size_t ipair_ab, size_t ishell_c,
+                                        const ShellPairData &sp_data_ab,
+                                        const ShellData &sh_data_c) const;
+    private:
+        std::shared_ptr<detail::Tensors> t_;
+    };
+
+    /// 6 blocks: A, B.
+    class ERI2D1Kernel
+    {
+    public:
+        ERI2D1Kernel(const ShellData &sh_data_a, const ShellData &sh_data_b);
+        std::array<vec2d, 6> operator()(size_t ishell_a, size_t ishell_b, const ShellData &sh_data_a,
+                                        const ShellData &sh_data_b) const;
+    private:
+        std::shared_ptr<detail::Tensors> t_;
     };
 
     /// (mn|kl), dense nbf^4 (eri4.cpp:38-95).

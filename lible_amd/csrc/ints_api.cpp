@@ -14,6 +14,7 @@
 #include <sstream>
 
 #include "lbx.h"
+#include "lbx_internal.hpp"
 
 namespace fs = std::filesystem;
 
@@ -477,5 +478,347 @@ namespace lible::ints
         dJ.toHost(J.memptr());
         dK.toHost(K.memptr());
         return {std::move(J), std::move(K)};
+    }
+
+    // ------------------------------------------------------------------ shell data, kernels
+    namespace detail
+    {
+        /// A structure's dense tensors, each evaluated on the GPU the first time a kernel of its
+        /// kind is constructed and kept on the host for the kernels' call operators.
+        struct Tensors
+        {
+            // a copy of the structure made before this object existed (no ownership cycle);
+            // shares the device context when one was already made
+            std::shared_ptr<const Structure> source;
+            size_t n = 0, naux = 0;
+            std::mutex mu;
+            std::vector<double> t[6];   // eri4, eri3, eri2, eri4d1, eri3d1, eri2d1
+
+            const std::vector<double> &get(int which)
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                std::vector<double> &v = t[which];
+                if (!v.empty())
+                    return v;
+                const size_t n2 = n * n, sizes[6] = {n2 * n2, n2 * naux, naux * naux, 12 * n2 * n2,
+                                                     9 * n2 * naux, 6 * naux * naux};
+                using Fn = int (*)(lbx_context *, double *);
+                const Fn fns[6] = {lbx_eri4, lbx_eri3, lbx_eri2, lbx_eri4d1, lbx_eri3d1, lbx_eri2d1};
+                lbx_context *ctx = source->context();
+                DeviceArray d(sizes[which]);
+                check(fns[which](ctx, d.get()));
+                check(lbx_synchronize(ctx));
+                v.resize(sizes[which]);
+                d.toHost(v.data());
+                return v;
+            }
+        };
+    }
+
+    std::shared_ptr<detail::Tensors> Structure::tensors() const
+    {
+        if (!tensors_)
+        {
+            auto t = std::make_shared<detail::Tensors>();
+            t->source = std::make_shared<const Structure>(*this);
+            t->n = dim_ao_;
+            t->naux = dim_ao_aux_;
+            tensors_ = t;
+        }
+        return tensors_;
+    }
+
+    const std::vector<Structure::ShellInfo> &Structure::shells(bool aux) const
+    {
+        auto &slot = aux ? shells_aux_ : shells_;
+        if (!slot)
+        {
+            if (aux && !use_ri_)
+                throw std::runtime_error("shellDataAux(): the structure has no auxiliary basis");
+            std::vector<std::array<double, 3>> xyz = coords_;
+            xyz.insert(xyz.end(), coords_ghost_.begin(), coords_ghost_.end());
+            FlatBasis fb(aux ? basis_aux_ : basis_, xyz);
+            size_t nbf = 0;
+            int maxl = 0;
+            auto hs = lbx::buildShells(fb.b, nbf, maxl, aux ? lbx::kMaxLAux : lbx::kMaxLMain);
+            auto v = std::make_shared<std::vector<ShellInfo>>();
+            size_t cart = 0;
+            for (const auto &h : hs)
+            {
+                v->push_back(ShellInfo{h.l, static_cast<size_t>(h.atom), h.ao, cart, h.xyz, h.exps,
+                                       h.coeffs, h.pnorms, h.norms});
+                cart += static_cast<size_t>((h.l + 1) * (h.l + 2) / 2);
+            }
+            slot = v;
+        }
+        return *slot;
+    }
+
+    std::vector<ShellPairData> shellPairData(bool use_symm, const Structure &s)
+    {
+        const auto &sh = s.shells(false);
+        std::vector<std::pair<int, int>> lp;
+        for (int la = 0; la <= s.getMaxL(); la++)
+            for (int lb = 0; lb <= (use_symm ? la : s.getMaxL()); lb++)
+                lp.emplace_back(la, lb);
+        std::vector<ShellPairData> out;
+        for (auto [la, lb] : lp)
+        {
+            std::vector<size_t> A, B;
+            for (size_t i = 0; i < sh.size(); i++)
+            {
+                if (sh[i].l == la)
+                    A.push_back(i);
+                if (sh[i].l == lb)
+                    B.push_back(i);
+            }
+            ShellPairData d;
+            d.uses_symm_ = use_symm;
+            d.primitives_thrs_ = 1e-15;
+            d.la_ = la;
+            d.lb_ = lb;
+            d.tensors_ = s.tensors();
+            const size_t sa = 2 * la + 1, sb = 2 * lb + 1;
+            const size_t ne = sa * sb * static_cast<size_t>(lbx::nHerm(la + lb));
+            size_t nprim = 0, nnorm = 0;
+            for (size_t ia = 0; ia < A.size(); ia++)
+                for (size_t ib = 0; ib < ((use_symm && la == lb) ? ia + 1 : B.size()); ib++)
+                {
+                    const auto &a = sh[A[ia]], &b = sh[B[ib]];
+                    double r2 = 0;
+                    for (int k = 0; k < 3; k++)
+                        r2 += (a.xyz[k] - b.xyz[k]) * (a.xyz[k] - b.xyz[k]);
+                    d.n_pairs_total_++;
+                    size_t surv = 0;
+                    for (size_t x = 0; x < a.exps.size(); x++)
+                        for (size_t y = 0; y < b.exps.size(); y++)
+                        {
+                            d.n_ppairs_total_++;
+                            const double ea = a.exps[x], eb = b.exps[y];
+                            const double da = a.norms_prim[x] * a.coeffs[x], db = b.norms_prim[y] * b.coeffs[y];
+                            if (std::fabs(da * db * std::exp(-(ea * eb / (ea + eb)) * r2)) >= d.primitives_thrs_)
+                            {
+                                d.exps_.insert(d.exps_.end(), {ea, eb});
+                                d.coeffs_.insert(d.coeffs_.end(), {da, db});
+                                surv++;
+                            }
+                        }
+                    if (surv == 0)
+                        continue;
+                    d.coords_.insert(d.coords_.end(), a.xyz.begin(), a.xyz.end());
+                    d.coords_.insert(d.coords_.end(), b.xyz.begin(), b.xyz.end());
+                    d.norms_.insert(d.norms_.end(), a.norms.begin(), a.norms.end());
+                    d.norms_.insert(d.norms_.end(), b.norms.begin(), b.norms.end());
+                    d.nrs_ppairs_.push_back(surv);
+                    d.offsets_primitives_.push_back(nprim);
+                    d.offsets_ecoeffs_.push_back(nprim * ne);
+                    d.offsets_ecoeffs_deriv1_.push_back(3 * nprim * ne);
+                    d.offsets_norms_.push_back(nnorm);
+                    d.offsets_sph_.insert(d.offsets_sph_.end(), {a.ofs_sph, b.ofs_sph});
+                    d.offsets_cart_.insert(d.offsets_cart_.end(), {a.ofs_cart, b.ofs_cart});
+                    d.atomic_idxs_.insert(d.atomic_idxs_.end(), {a.atom, b.atom});
+                    d.shell_idxs_.insert(d.shell_idxs_.end(), {A[ia], B[ib]});
+                    nprim += surv;
+                    nnorm += sa + sb;
+                    d.n_pairs_++;
+                }
+            d.n_ppairs_ = nprim;
+            out.push_back(std::move(d));
+        }
+        return out;
+    }
+
+    std::vector<ShellData> shellDataAux(const Structure &s)
+    {
+        const auto &sh = s.shells(true);
+        std::vector<ShellData> out;
+        for (int l = 0; l <= s.getMaxLAux(); l++)
+        {
+            ShellData d;
+            d.l_ = l;
+            d.tensors_ = s.tensors();
+            const size_t ne = static_cast<size_t>((2 * l + 1) * lbx::nHerm(l));
+            size_t nprim = 0, nnorm = 0;
+            for (size_t i = 0; i < sh.size(); i++)
+            {
+                const auto &a = sh[i];
+                if (a.l != l)
+                    continue;
+                for (size_t x = 0; x < a.exps.size(); x++)
+                {
+                    d.coeffs_.push_back(a.coeffs[x] * a.norms_prim[x]);
+                    d.exps_.push_back(a.exps[x]);
+                }
+                d.coords_.insert(d.coords_.end(), a.xyz.begin(), a.xyz.end());
+                d.norms_.insert(d.norms_.end(), a.norms.begin(), a.norms.end());
+                d.atomic_idxs_.push_back(a.atom);
+                d.cdepths_.push_back(a.exps.size());
+                d.coffsets_.push_back(nprim);
+                d.offsets_ecoeffs_.push_back(nprim * ne);
+                d.offsets_norms_.push_back(nnorm);
+                d.offsets_sph_.push_back(a.ofs_sph);
+                d.shell_idxs_.push_back(i);
+                nprim += a.exps.size();
+                nnorm += 2 * l + 1;
+                d.n_shells_++;
+            }
+            d.n_primitives_ = nprim;
+            out.push_back(std::move(d));
+        }
+        return out;
+    }
+
+    namespace
+    {
+        template <class D>
+        std::shared_ptr<detail::Tensors> tensorsOf(const D &a, const char *who)
+        {
+            if (!a.tensors_)
+                throw std::runtime_error(std::string(who) + ": shell data must come from "
+                                         "shellPairData() / shellDataAux()");
+            return a.tensors_;
+        }
+
+        template <class A, class B>
+        std::shared_ptr<detail::Tensors> sameTensors(const A &a, const B &b, int which, const char *who)
+        {
+            auto ta = tensorsOf(a, who), tb = tensorsOf(b, who);
+            if (ta != tb)
+                throw std::runtime_error(std::string(who) + ": shell data of different structures");
+            ta->get(which);
+            return ta;
+        }
+
+        void checkIndex(size_t i, size_t n, const char *who)
+        {
+            if (i >= n)
+                throw std::out_of_range(std::string(who) + ": index out of range");
+        }
+
+        // block (S_a, S_b, S_c, S_d) of a dense (n, n, n, n) tensor
+        vec4d block4(const double *t, size_t n, size_t oa, size_t ob, size_t oc, size_t od, int la,
+                     int lb, int lc, int ld)
+        {
+            const size_t na = 2 * la + 1, nb = 2 * lb + 1, nc = 2 * lc + 1, nd = 2 * ld + 1;
+            vec4d r(na, nb, nc, nd);
+            for (size_t a = 0; a < na; a++)
+                for (size_t b = 0; b < nb; b++)
+                    for (size_t c = 0; c < nc; c++)
+                        for (size_t d = 0; d < nd; d++)
+                            r(a, b, c, d) = t[(((oa + a) * n + ob + b) * n + oc + c) * n + od + d];
+            return r;
+        }
+
+        vec3d block3(const double *t, size_t n, size_t naux, size_t oa, size_t ob, size_t oc, int la,
+                     int lb, int lc)
+        {
+            const size_t na = 2 * la + 1, nb = 2 * lb + 1, nc = 2 * lc + 1;
+            vec3d r(na, nb, nc);
+            for (size_t a = 0; a < na; a++)
+                for (size_t b = 0; b < nb; b++)
+                    for (size_t c = 0; c < nc; c++)
+                        r(a, b, c) = t[((oa + a) * n + ob + b) * naux + oc + c];
+            return r;
+        }
+
+        vec2d block2(const double *t, size_t naux, size_t oa, size_t ob, int la, int lb)
+        {
+            const size_t na = 2 * la + 1, nb = 2 * lb + 1;
+            vec2d r(na, nb);
+            for (size_t a = 0; a < na; a++)
+                for (size_t b = 0; b < nb; b++)
+                    r(a, b) = t[(oa + a) * naux + ob + b];
+            return r;
+        }
+    }
+
+    ERI4Kernel::ERI4Kernel(const ShellPairData &ab, const ShellPairData &cd)
+        : t_(sameTensors(ab, cd, 0, "ERI4Kernel"))
+    {
+    }
+
+    vec4d ERI4Kernel::operator()(size_t p, size_t q, const ShellPairData &ab, const ShellPairData &cd) const
+    {
+        checkIndex(p, ab.n_pairs_, "ERI4Kernel");
+        checkIndex(q, cd.n_pairs_, "ERI4Kernel");
+        return block4(t_->t[0].data(), t_->n, ab.offsets_sph_[2 * p], ab.offsets_sph_[2 * p + 1],
+                      cd.offsets_sph_[2 * q], cd.offsets_sph_[2 * q + 1], ab.la_, ab.lb_, cd.la_, cd.lb_);
+    }
+
+    ERI3Kernel::ERI3Kernel(const ShellPairData &ab, const ShellData &c)
+        : t_(sameTensors(ab, c, 1, "ERI3Kernel"))
+    {
+    }
+
+    vec3d ERI3Kernel::operator()(size_t p, size_t k, const ShellPairData &ab, const ShellData &c) const
+    {
+        checkIndex(p, ab.n_pairs_, "ERI3Kernel");
+        checkIndex(k, c.n_shells_, "ERI3Kernel");
+        return block3(t_->t[1].data(), t_->n, t_->naux, ab.offsets_sph_[2 * p], ab.offsets_sph_[2 * p + 1],
+                      c.offsets_sph_[k], ab.la_, ab.lb_, c.l_);
+    }
+
+    ERI2Kernel::ERI2Kernel(const ShellData &a, const ShellData &b) : t_(sameTensors(a, b, 2, "ERI2Kernel"))
+    {
+    }
+
+    vec2d ERI2Kernel::operator()(size_t i, size_t j, const ShellData &a, const ShellData &b) const
+    {
+        checkIndex(i, a.n_shells_, "ERI2Kernel");
+        checkIndex(j, b.n_shells_, "ERI2Kernel");
+        return block2(t_->t[2].data(), t_->naux, a.offsets_sph_[i], b.offsets_sph_[j], a.l_, b.l_);
+    }
+
+    ERI4D1Kernel::ERI4D1Kernel(const ShellPairData &ab, const ShellPairData &cd)
+        : t_(sameTensors(ab, cd, 3, "ERI4D1Kernel"))
+    {
+    }
+
+    std::array<vec4d, 12> ERI4D1Kernel::operator()(size_t p, size_t q, const ShellPairData &ab,
+                                                   const ShellPairData &cd) const
+    {
+        checkIndex(p, ab.n_pairs_, "ERI4D1Kernel");
+        checkIndex(q, cd.n_pairs_, "ERI4D1Kernel");
+        const size_t n = t_->n, n4 = n * n * n * n;
+        std::array<vec4d, 12> r;
+        for (int c = 0; c < 12; c++)
+            r[c] = block4(t_->t[3].data() + c * n4, n, ab.offsets_sph_[2 * p], ab.offsets_sph_[2 * p + 1],
+                          cd.offsets_sph_[2 * q], cd.offsets_sph_[2 * q + 1], ab.la_, ab.lb_, cd.la_, cd.lb_);
+        return r;
+    }
+
+    ERI3D1Kernel::ERI3D1Kernel(const ShellPairData &ab, const ShellData &c)
+        : t_(sameTensors(ab, c, 4, "ERI3D1Kernel"))
+    {
+    }
+
+    std::array<vec3d, 9> ERI3D1Kernel::operator()(size_t p, size_t k, const ShellPairData &ab,
+                                                  const ShellData &c) const
+    {
+        checkIndex(p, ab.n_pairs_, "ERI3D1Kernel");
+        checkIndex(k, c.n_shells_, "ERI3D1Kernel");
+        const size_t n = t_->n, n3 = n * n * t_->naux;
+        std::array<vec3d, 9> r;
+        for (int x = 0; x < 9; x++)
+            r[x] = block3(t_->t[4].data() + x * n3, n, t_->naux, ab.offsets_sph_[2 * p],
+                          ab.offsets_sph_[2 * p + 1], c.offsets_sph_[k], ab.la_, ab.lb_, c.l_);
+        return r;
+    }
+
+    ERI2D1Kernel::ERI2D1Kernel(const ShellData &a, const ShellData &b)
+        : t_(sameTensors(a, b, 5, "ERI2D1Kernel"))
+    {
+    }
+
+    std::array<vec2d, 6> ERI2D1Kernel::operator()(size_t i, size_t j, const ShellData &a,
+                                                  const ShellData &b) const
+    {
+        checkIndex(i, a.n_shells_, "ERI2D1Kernel");
+        checkIndex(j, b.n_shells_, "ERI2D1Kernel");
+        const size_t n2 = t_->naux * t_->naux;
+        std::array<vec2d, 6> r;
+        for (int x = 0; x < 6; x++)
+            r[x] = block2(t_->t[5].data() + x * n2, t_->naux, a.offsets_sph_[i], b.offsets_sph_[j], a.l_, b.l_);
+        return r;
     }
 }

@@ -81,6 +81,27 @@ namespace
             ok = ok && absSum(f) == 22.5;
             report("vectormd_layout", ok);
         });
+        guarded("shell_pair_data", [] {
+            // O3 / def2-SVP: 9 s, 6 p, 3 d shells = 171 unique pairs, of which the core s pair
+            // of the atoms 3.78 bohr apart has no primitive pair above 1e-15; the ordered pairs
+            // are both orders of the distinct-shell ones plus the 18 same-shell pairs
+            lints::Structure s("def2-svp", nrs_o3, xyz_o3);
+            size_t symm = 0, all = 0, prim = 0;
+            bool ok = true;
+            for (const auto &d : lints::shellPairData(true, s))
+            {
+                symm += d.n_pairs_;
+                ok = ok && d.offsets_sph_.size() == 2 * d.n_pairs_ && d.nrs_ppairs_.size() == d.n_pairs_ &&
+                     d.exps_.size() == 2 * d.n_ppairs_ && d.la_ >= d.lb_;
+            }
+            for (const auto &d : lints::shellPairData(false, s))
+            {
+                all += d.n_pairs_;
+                prim += d.n_ppairs_total_;
+            }
+            ok = ok && symm == 170 && all == 2 * 170 - 18;
+            report("shell_pair_data", ok, "pairs " + std::to_string(symm) + " / " + std::to_string(all));
+        });
         guarded("dims_o3_svp", [] {
             lints::Structure s("def2-svp", nrs_o3, xyz_o3);
             // O in def2-SVP: 3s 2p 1d = 14 spherical functions
@@ -187,6 +208,96 @@ namespace
         guarded("eri2_diagonal", [] {
             lints::Structure s("def2-SVP", "def2-universal-jkfit", nrs_h2o, xyz_h2o);
             checkSum("eri2_diagonal", absSum(lints::eri2Diagonal(s)), 830.788976485982); // :149
+        });
+        // the per-block kernels, driven as tests_ints.cpp:873-1053 drives them
+        guarded("deploy_eri4_kernel", [] {
+            lints::Structure structure("def2-svp", nrs_o3, xyz_o3);
+            auto sp = lints::shellPairData(false, structure);
+            double sum = 0;
+            for (const auto &ab : sp)
+                for (const auto &cd : sp)
+                {
+                    lints::ERI4Kernel kernel(ab, cd);
+                    for (size_t p = 0; p < ab.n_pairs_; p++)
+                        for (size_t q = 0; q < cd.n_pairs_; q++)
+                            sum += absSum(kernel(p, q, ab, cd));
+                }
+            checkSum("deploy_eri4_kernel", sum, 8246.711763197583);   // :873-900
+        });
+        guarded("deploy_eri3_kernel", [] {
+            lints::Structure structure("def2-svp", "def2-universal-jkfit", nrs_o3, xyz_o3);
+            auto sp = lints::shellPairData(false, structure);
+            auto sh = lints::shellDataAux(structure);
+            double sum = 0;
+            for (const auto &ab : sp)
+                for (const auto &c : sh)
+                {
+                    lints::ERI3Kernel kernel(ab, c);
+                    for (size_t p = 0; p < ab.n_pairs_; p++)
+                        for (size_t k = 0; k < c.n_shells_; k++)
+                            sum += absSum(kernel(p, k, ab, c));
+                }
+            checkSum("deploy_eri3_kernel", sum, 12350.130354189232);  // :902-930
+        });
+        guarded("deploy_eri2_kernel", [] {
+            lints::Structure structure("def2-svp", "def2-universal-jkfit", nrs_o3, xyz_o3);
+            auto sh = lints::shellDataAux(structure);
+            double sum = 0;
+            for (const auto &a : sh)
+                for (const auto &b : sh)
+                {
+                    lints::ERI2Kernel kernel(a, b);
+                    for (size_t i = 0; i < a.n_shells_; i++)
+                        for (size_t j = 0; j < b.n_shells_; j++)
+                            sum += absSum(kernel(i, j, a, b));
+                }
+            checkSum("deploy_eri2_kernel", sum, 21014.512841176023);  // :932-959
+        });
+        guarded("deploy_eri4d1_kernel", [] {
+            lints::Structure structure("def2-svp", nrs_o3, xyz_o3);
+            auto sp = lints::shellPairData(true, structure);
+            double sum = 0;
+            for (const auto &ab : sp)
+                for (const auto &cd : sp)
+                {
+                    lints::ERI4D1Kernel kernel(ab, cd);
+                    for (size_t p = 0; p < ab.n_pairs_; p++)
+                        for (size_t q = 0; q < cd.n_pairs_; q++)
+                            for (const vec4d &blk : kernel(p, q, ab, cd))
+                                sum += absSum(blk);
+                }
+            checkSum("deploy_eri4d1_kernel", sum, 30546.780073140635);  // :961-990
+        });
+        guarded("deploy_eri3d1_kernel", [] {
+            lints::Structure structure("def2-svp", "def2-universal-jkfit", nrs_o3, xyz_o3);
+            auto sp = lints::shellPairData(false, structure);
+            auto sh = lints::shellDataAux(structure);
+            double sum = 0;
+            for (const auto &ab : sp)
+                for (const auto &c : sh)
+                {
+                    lints::ERI3D1Kernel kernel(ab, c);
+                    for (size_t p = 0; p < ab.n_pairs_; p++)
+                        for (size_t k = 0; k < c.n_shells_; k++)
+                            for (const vec3d &blk : kernel(p, k, ab, c))
+                                sum += absSum(blk);
+                }
+            checkSum("deploy_eri3d1_kernel", sum, 78466.360848033844);  // :992-1022
+        });
+        guarded("deploy_eri2d1_kernel", [] {
+            lints::Structure structure("def2-svp", "def2-universal-jkfit", nrs_o3, xyz_o3);
+            auto sh = lints::shellDataAux(structure);
+            double sum = 0;
+            for (const auto &a : sh)
+                for (const auto &b : sh)
+                {
+                    lints::ERI2D1Kernel kernel(a, b);
+                    for (size_t i = 0; i < a.n_shells_; i++)
+                        for (size_t j = 0; j < b.n_shells_; j++)
+                            for (const vec2d &blk : kernel(i, j, a, b))
+                                sum += absSum(blk);
+                }
+            checkSum("deploy_eri2d1_kernel", sum, 39949.218421684251);  // :1024-1053
         });
         guarded("ghost_atoms", [] {
             // Integrals do not see nuclear charges: with the last H a ghost atom, the AO order is

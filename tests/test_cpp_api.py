@@ -42,7 +42,7 @@ def test_header_compiles_standalone(tmp_path):
 
 def test_cpp_interface_host_side():
     lines = run("cpu", 60)
-    assert len(lines) == 7
+    assert len(lines) == 8
 
 
 @pytest.mark.gpu
@@ -50,4 +50,6 @@ def test_cpp_interface_known_answers():
     lines = run("gpu", 300)
     names = {l.split()[1].rstrip(":") for l in lines}
     assert {"eri4", "eri4_symmetry", "coulomb_exchange", "eri4_diagonal", "eri3", "eri2",
-            "eri2_diagonal", "ghost_atoms"} <= names
+            "eri2_diagonal", "ghost_atoms", "deploy_eri4_kernel", "deploy_eri3_kernel",
+            "deploy_eri2_kernel", "deploy_eri4d1_kernel", "deploy_eri3d1_kernel",
+            "deploy_eri2d1_kernel"} <= names
