@@ -177,6 +177,13 @@ def bench_gradient(a):
         cx.close()
         log(f"[bench] finite-difference energy {len(es)} done")
     fd = (es[0] - es[1]) / (2 * h)
+    # mixed precision: quartets with Schwarz bound below MIXED_SPLIT in FP32
+    gm = torch.empty_like(g)
+    ctx.jk_grad(D, schwarz=SCHWARZ, grad=gm, precision=lbx.LBX_MIXED, mixed_thresh=MIXED_SPLIT)
+    log("[bench] mixed gradient first call done")
+    mixed_s, (_, stm) = timed(lambda: ctx.jk_grad(D, schwarz=SCHWARZ, grad=gm, precision=lbx.LBX_MIXED,
+                                                  mixed_thresh=MIXED_SPLIT), a.steps, "mixed gradient")
+    gmh = gm.cpu().numpy()
     tflops = st["flops"] / step_s / 1e12
     res = {
         "metric": "nuclear gradient of the 2e energy: shell quartets/s",
@@ -191,6 +198,11 @@ def bench_gradient(a):
         "fd_check": {"atom": int(ia), "axis": int(x), "analytic": float(gh[ia, x]), "central_diff": fd,
                      "abs_diff": abs(float(gh[ia, x]) - fd)},
         "max_abs_translation_sum": float(np.abs(gh.sum(axis=0)).max()),
+        "mixed": {"split": MIXED_SPLIT, "ms_per_step": mixed_s * 1e3,
+                  "fp32_share": stm["quartets_fp32"] / max(stm["quartets"], 1.0),
+                  "speedup_vs_fp64": step_s / mixed_s,
+                  "max_abs_dgrad": float(np.abs(gmh - gh).max()),
+                  "max_abs_grad": float(np.abs(gh).max())},
     }
     print(json.dumps(res), flush=True)
     return 0

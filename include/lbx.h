@@ -126,9 +126,11 @@ int lbx_ri_sums(lbx_context *ctx, double *out, lbx_ri_stats *stats);
  *   grad[3 A + x] = sum over unique quartets with Q_ab Q_cd >= schwarz_thresh of
  *                   f (4 D_mn D_kl - D_mk D_nl - D_ml D_nk) d(mn|kl)/dR_{A,x}
  * with f the permutational weight.  D: device (nbf, nbf) symmetric; grad: device 3 natoms
- * (atom-major, x y z), Eh/bohr.  FP64. */
+ * (atom-major, x y z), Eh/bohr.  precision LBX_FP64, or LBX_MIXED: quartets with bound below
+ * mixed_thresh in FP32 arithmetic (R, E, E', transforms) with FP64 Boys values and sums; stats
+ * count the quartets and those in FP32. */
 int lbx_jk_grad(lbx_context *ctx, const double *D, double *grad, double schwarz_thresh,
-                lbx_jk_stats *stats);
+                int precision, double mixed_thresh, lbx_jk_stats *stats);
 
 /* Autotuner (no reference counterpart).  Each class pair (la lb|lc ld) has up to five kernel
  * variants: 0 one lane per quartet with the bra pairs on the lanes, 1 the same with the ket

@@ -452,6 +452,23 @@ namespace lbx
             }
         }
 
+        // FP32 copies of the derivative E matrices (mixed-precision gradient)
+        void ensureF32D1(Context &c)
+        {
+            for (auto &pc : c.cls)
+            {
+                if (pc.d_e1b32 || pc.n_ppairs == 0 || !pc.d_e1b)
+                    continue;
+                const long long n = 3 * static_cast<long long>(pc.eCount());
+                pc.d_e1b32 = c.alloc<float>(n);
+                pc.d_e1k32 = c.alloc<float>(n);
+                const int grid = static_cast<int>(std::min<long long>((n + 255) / 256, 65536));
+                hipLaunchKernelGGL(toFloatKernel, dim3(grid), dim3(256), 0, c.stream, pc.d_e1b, pc.d_e1b32, n);
+                hipLaunchKernelGGL(toFloatKernel, dim3(grid), dim3(256), 0, c.stream, pc.d_e1k, pc.d_e1k32, n);
+                LBX_HIP(hipGetLastError());
+            }
+        }
+
         const KernelSlot &slotFor(int mode, const PairClass &bra, const PairClass &ket, int variant)
         {
             const KernelSlot &s = kernelSlot(mode, bra.la, bra.lb, ket.la, ket.lb, variant);
@@ -706,7 +723,7 @@ struct lbx_context : Context
     std::map<std::tuple<int, int, double, double, int>, TaskTable> tune_tasks;   // the samples
 
     // gradient task tables per (bra class, ket class, Schwarz threshold)
-    std::map<std::tuple<int, int, double>, TaskTable> grad_tasks;
+    std::map<std::tuple<int, int, double, double>, TaskTable> grad_tasks;
 
     // batched RI sums: the whole launch sequence captured once into a graph (per output buffer)
     hipGraph_t ri_graph = nullptr;
@@ -773,21 +790,23 @@ struct lbx_context : Context
         }
     }
 
-    // Queues the gradient kernel of class pair (i >= j); returns its task table.
-    const TaskTable &launchGrad(int i, int j, const double *D, double *grad, double thresh)
+    // Queues the gradient kernel of class pair (i >= j) over the quartets with
+    // thresh <= Q_ab Q_cd < thresh_hi, in FP32 arithmetic when fp32; returns its task table.
+    const TaskTable &launchGrad(int i, int j, const double *D, double *grad, double thresh,
+                                double thresh_hi = kNoUpper, bool fp32 = false)
     {
         const PairClass &ci = cls[i], &cj = cls[j];
         const bool lane_i = gradLaneFirst(ci.la, ci.lb, cj.la, cj.lb);
         const PairClass &lane = lane_i ? ci : cj, &unif = lane_i ? cj : ci;
         const GradSlot &s = gradSlot(lane.la, lane.lb, unif.la, unif.lb);
-        if (!s.fn)
+        if (!s.fn || !s.fn32)
             throw Error(LBX_ERR_INVALID, "no gradient kernel for this class pair");
-        const auto key = std::make_tuple(i, j, thresh);
+        const auto key = std::make_tuple(i, j, thresh, thresh_hi);
         auto it = grad_tasks.find(key);
         if (it == grad_tasks.end())
             it = grad_tasks
                          .emplace(key, buildTasks(*this, lane, unif, Pairing::Unique, i == j, thresh, s.width,
-                                                  lane_i))
+                                                  lane_i, thresh_hi))
                          .first;
         const TaskTable &t = it->second;
         GArgs a{};
@@ -803,7 +822,7 @@ struct lbx_context : Context
         a.D = D;
         a.nbf = static_cast<long long>(nbf);
         a.grad = grad;
-        s.fn(a, stream);
+        (fp32 ? s.fn32 : s.fn)(a, stream);
         LBX_HIP(hipGetLastError());
         return t;
     }
@@ -1416,7 +1435,7 @@ extern "C"
     }
 
     int lbx_jk_grad(lbx_context *ctx, const double *D, double *grad, double schwarz_thresh,
-                    lbx_jk_stats *stats)
+                    int precision, double mixed_thresh, lbx_jk_stats *stats)
     {
         return guarded([&]
                        {
@@ -1424,30 +1443,49 @@ extern "C"
                                throw Error(LBX_ERR_INVALID, "lbx_jk_grad: null argument");
                            if (!(schwarz_thresh >= 0))
                                throw Error(LBX_ERR_INVALID, "lbx_jk_grad: negative Schwarz threshold");
+                           if (precision != LBX_FP64 && precision != LBX_MIXED)
+                               throw Error(LBX_ERR_INVALID, "lbx_jk_grad: precision must be LBX_FP64 or LBX_MIXED");
+                           if (precision == LBX_MIXED && !(mixed_thresh >= 0))
+                               throw Error(LBX_ERR_INVALID, "lbx_jk_grad: negative mixed-precision threshold");
                            std::lock_guard<std::mutex> lk(ctx->mu);
                            Context &c = *ctx;
                            requireSingle(c, "lbx_jk_grad");
                            ensureSchwarz(c);
                            for (auto &pc : c.cls)
                                computeEcoeffsD1(c, pc);
+                           const bool mixed = precision == LBX_MIXED && mixed_thresh > schwarz_thresh;
+                           if (mixed)
+                           {
+                               ensureF32(c);
+                               ensureF32D1(c);
+                           }
                            fill(grad, 3LL * c.natoms, 0.0, c.stream);
-                           double nq = 0, fl = 0;
+                           double nq = 0, fl = 0, nq32 = 0;
                            for (int i = 0; i < static_cast<int>(c.cls.size()); i++)
                                for (int j = 0; j <= i; j++)
                                {
                                    const PairClass &bra = c.cls[i], &ket = c.cls[j];
                                    if (bra.n_pairs == 0 || ket.n_pairs == 0)
                                        continue;
-                                   const TaskTable &t = ctx->launchGrad(i, j, D, grad, schwarz_thresh);
+                                   const TaskTable &t = ctx->launchGrad(i, j, D, grad,
+                                                                        mixed ? mixed_thresh : schwarz_thresh);
                                    nq += t.quartets;
                                    fl += gradFlops(t, bra.la, bra.lb, ket.la, ket.lb);
+                                   if (mixed)
+                                   {
+                                       const TaskTable &t32 = ctx->launchGrad(i, j, D, grad, schwarz_thresh,
+                                                                              mixed_thresh, true);
+                                       nq += t32.quartets;
+                                       nq32 += t32.quartets;
+                                       fl += gradFlops(t32, bra.la, bra.lb, ket.la, ket.lb);
+                                   }
                                }
                            LBX_HIP(hipStreamSynchronize(c.stream));
                            if (stats)
                            {
                                stats->quartets = nq;
                                stats->flops = fl;
-                               stats->quartets_fp32 = 0;
+                               stats->quartets_fp32 = nq32;
                            }
                        });
     }

@@ -46,7 +46,8 @@ namespace lbx
     using GradLauncher = void (*)(const GArgs &, hipStream_t);
     struct GradSlot
     {
-        GradLauncher fn = nullptr;
+        GradLauncher fn = nullptr;     // FP64
+        GradLauncher fn32 = nullptr;   // R, E, E' and the transforms in FP32, sums in FP64
         int width = 0;   // lane-side pairs per task
     };
     // (la lb) lane side, (lc ld) uniform side
@@ -105,13 +106,17 @@ namespace lbx
         return x;
     }
 
-    template <int LA, int LB, int LC, int LD>
+    // T: type of R, the E and E' matrices and the transforms (double, or float for the
+    // small-bound quartets of the mixed-precision gradient); Boys values, density weights and the
+    // gradient sums stay double.
+    template <int LA, int LB, int LC, int LD, typename T>
     __global__ __launch_bounds__(64) void gradKernel(const GArgs A)
     {
         using C = GCls<LA, LB, LC, LD>;
         constexpr int L1 = C::L1, HP = C::HP, HQ = C::HQ, HL1 = C::HL1, SP = C::SP, SQ = C::SQ,
                       SB = C::SB, SD = C::SD, G = C::G, LPS = C::LPS, NH = C::NH, NC = C::NC;
-        __shared__ double sR[2][G][HL1];
+        constexpr bool F32 = sizeof(T) == 4;
+        __shared__ T sR[2][G][HL1];
         __shared__ double sW[G][SP * SQ];
         const int lane = threadIdx.x;
         const int slot = lane / LPS, jl = lane % LPS;
@@ -189,26 +194,30 @@ namespace lbx
                     const bool valid = i < lcnt;
                     const int ip = lpoff + (valid ? i : 0);
                     // Y[h][c] = sum_ab Eb_i[h][ab] W[ab][cd0 + c]
-                    double Y[NH][NC], X[NH][NC];
-                    const double *eb = LP.eb + static_cast<size_t>(ip) * HP * SP;
+                    T Y[NH][NC], X[NH][NC];
+                    const T *eb;
+                    if constexpr (F32)
+                        eb = LP.eb32 + static_cast<size_t>(ip) * HP * SP;
+                    else
+                        eb = LP.eb + static_cast<size_t>(ip) * HP * SP;
 #pragma unroll
                     for (int r = 0; r < NH; r++)
 #pragma unroll
                         for (int c = 0; c < NC; c++)
                         {
-                            Y[r][c] = 0.0;
-                            X[r][c] = 0.0;
+                            Y[r][c] = T(0);
+                            X[r][c] = T(0);
                         }
                     for (int ab = 0; ab < SP; ab++)
                     {
-                        double w[NC];
+                        T w[NC];
 #pragma unroll
                         for (int c = 0; c < NC; c++)
-                            w[c] = cd0 + c < SQ ? sW[slot][ab * SQ + cd0 + c] : 0.0;
+                            w[c] = cd0 + c < SQ ? T(sW[slot][ab * SQ + cd0 + c]) : T(0);
 #pragma unroll
                         for (int r = 0; r < NH; r++)
                         {
-                            const double e = hv[r] ? eb[(jl + r * LPS) * SP + ab] : 0.0;
+                            const T e = hv[r] ? eb[(jl + r * LPS) * SP + ab] : T(0);
 #pragma unroll
                             for (int c = 0; c < NC; c++)
                                 Y[r][c] = fma(e, w[c], Y[r][c]);
@@ -250,50 +259,61 @@ namespace lbx
                         }
                         // levels n = L1 .. 0 of R^{(n)}_{tuv}, |tuv| <= L1 - n, ping-pong in LDS
                         if (jl == 0)
-                            sR[L1 & 1][slot][0] = init[L1];
+                            sR[L1 & 1][slot][0] = T(init[L1]);
                         __syncthreads();
 #pragma unroll
                         for (int lev = L1 - 1; lev >= 0; lev--)
                         {
                             const int nh = nHerm(L1 - lev);
-                            const double *in = sR[(lev + 1) & 1][slot];
-                            double *out = sR[lev & 1][slot];
+                            const T *in = sR[(lev + 1) & 1][slot];
+                            T *out = sR[lev & 1][slot];
                             for (int h = jl; h < nh; h += LPS)
                             {
                                 if (h == 0)
                                 {
-                                    out[0] = init[lev];
+                                    out[0] = T(init[lev]);
                                     continue;
                                 }
                                 const unsigned rt = kRecTab.v[h];
                                 const int s1 = rt & 2047, s2 = (rt >> 11) & 2047, ax = (rt >> 22) & 3,
                                           cm = rt >> 24;
-                                const double xa = ax == 0 ? PX : (ax == 1 ? PY : PZ);
-                                out[h] = fma(xa, in[s1], cm * in[s2]);
+                                const T xa = T(ax == 0 ? PX : (ax == 1 ? PY : PZ));
+                                out[h] = fma(xa, in[s1], T(cm) * in[s2]);
                             }
                             __syncthreads();
                         }
-                        const double *R = sR[0][slot];
+                        const T *R = sR[0][slot];
                         if (valid)
                         {
                             const size_t kk = static_cast<size_t>(upoff + k);
-                            const cptr<double> ek = constView(UP.ek + kk * HQ * SQ + cd0);
-                            const cptr<double> e1 = constView(UP.e1k + kk * 3 * HQ * SQ + cd0);
+                            const T *ekb, *e1b;
+                            if constexpr (F32)
+                            {
+                                ekb = UP.ek32;
+                                e1b = UP.e1k32;
+                            }
+                            else
+                            {
+                                ekb = UP.ek;
+                                e1b = UP.e1k;
+                            }
+                            const cptr<T> ek = constView(ekb + kk * HQ * SQ + cd0);
+                            const cptr<T> e1 = constView(e1b + kk * 3 * HQ * SQ + cd0);
                             double tk[3] = {0, 0, 0};
 #pragma unroll
                             for (int g = 0; g < HQ; g++)
                             {
                                 const int tuvg = hermTuv(g);
                                 const int tg = tuvg & 255, ug = (tuvg >> 8) & 255, vg = tuvg >> 16;
-                                double ekv[NC], e1v[3][NC];
+                                T ekv[NC], e1v[3][NC];
 #pragma unroll
                                 for (int c = 0; c < NC; c++)
                                 {
                                     const bool in = cd0 + c < SQ;
-                                    ekv[c] = in ? ek[g * SQ + c] : 0.0;
+                                    ekv[c] = in ? ek[g * SQ + c] : T(0);
 #pragma unroll
                                     for (int x = 0; x < 3; x++)
-                                        e1v[x][c] = in ? e1[(x * HQ + g) * SQ + c] : 0.0;
+                                        e1v[x][c] = in ? e1[(x * HQ + g) * SQ + c] : T(0);
                                 }
 #pragma unroll
                                 for (int r = 0; r < NH; r++)
@@ -301,11 +321,11 @@ namespace lbx
                                     if (!hv[r])
                                         continue;
                                     const int tt = th[r] + tg, uu = uh[r] + ug, vv = vh[r] + vg;
-                                    const double r0 = R[hermIdx(tt, uu, vv)];
-                                    const double rx = R[hermIdx(tt + 1, uu, vv)];
-                                    const double ry = R[hermIdx(tt, uu + 1, vv)];
-                                    const double rz = R[hermIdx(tt, uu, vv + 1)];
-                                    double z = 0.0, zx = 0.0, zy = 0.0, zz = 0.0;
+                                    const T r0 = R[hermIdx(tt, uu, vv)];
+                                    const T rx = R[hermIdx(tt + 1, uu, vv)];
+                                    const T ry = R[hermIdx(tt, uu + 1, vv)];
+                                    const T rz = R[hermIdx(tt, uu, vv + 1)];
+                                    T z = T(0), zx = T(0), zy = T(0), zz = T(0);
 #pragma unroll
                                     for (int c = 0; c < NC; c++)
                                     {
@@ -315,12 +335,12 @@ namespace lbx
                                         zz = fma(Y[r][c], e1v[2][c], zz);
                                         X[r][c] = fma(r0, ekv[c], X[r][c]);
                                     }
-                                    tk[0] = fma(z, rx, tk[0]);
-                                    tk[1] = fma(z, ry, tk[1]);
-                                    tk[2] = fma(z, rz, tk[2]);
-                                    gC[0] = fma(r0, zx, gC[0]);
-                                    gC[1] = fma(r0, zy, gC[1]);
-                                    gC[2] = fma(r0, zz, gC[2]);
+                                    tk[0] = fma(double(z), double(rx), tk[0]);
+                                    tk[1] = fma(double(z), double(ry), tk[1]);
+                                    tk[2] = fma(double(z), double(rz), tk[2]);
+                                    gC[0] = fma(double(r0), double(zx), gC[0]);
+                                    gC[1] = fma(double(r0), double(zy), gC[1]);
+                                    gC[2] = fma(double(r0), double(zz), gC[2]);
                                 }
                             }
                             const double rc = constView(UP.ra)[upoff + k];   // c / q
@@ -343,27 +363,31 @@ namespace lbx
                             gB[x] = fma(1.0 - ra, Ts[x], gB[x]);
                         }
                         // U_x = sum_ab sum_c W[ab][cd0 + c] sum_h Eb'_x[h][ab] X[h][c]
-                        const double *e1 = LP.e1b + static_cast<size_t>(ip) * 3 * HP * SP;
+                        const T *e1;
+                        if constexpr (F32)
+                            e1 = LP.e1b32 + static_cast<size_t>(ip) * 3 * HP * SP;
+                        else
+                            e1 = LP.e1b + static_cast<size_t>(ip) * 3 * HP * SP;
                         double ux[3] = {0, 0, 0};
                         for (int ab = 0; ab < SP; ab++)
                         {
-                            double w[NC];
+                            T w[NC];
 #pragma unroll
                             for (int c = 0; c < NC; c++)
-                                w[c] = cd0 + c < SQ ? sW[slot][ab * SQ + cd0 + c] : 0.0;
+                                w[c] = cd0 + c < SQ ? T(sW[slot][ab * SQ + cd0 + c]) : T(0);
 #pragma unroll
                             for (int r = 0; r < NH; r++)
                             {
                                 if (!hv[r])
                                     continue;
-                                double s = 0.0;
+                                T s = T(0);
 #pragma unroll
                                 for (int c = 0; c < NC; c++)
                                     s = fma(w[c], X[r][c], s);
                                 const int h = jl + r * LPS;
 #pragma unroll
                                 for (int x = 0; x < 3; x++)
-                                    ux[x] = fma(e1[(x * HP + h) * SP + ab], s, ux[x]);
+                                    ux[x] = fma(double(e1[(x * HP + h) * SP + ab]), double(s), ux[x]);
                             }
                         }
 #pragma unroll
@@ -398,20 +422,21 @@ namespace lbx
         }
     }
 
-    template <int LA, int LB, int LC, int LD>
+    template <int LA, int LB, int LC, int LD, typename T>
     void launchGrad(const GArgs &A, hipStream_t s)
     {
         if (A.ntasks <= 0)
             return;
         const int grid = static_cast<int>(A.ntasks < 16384 ? A.ntasks : 16384);
-        hipLaunchKernelGGL((gradKernel<LA, LB, LC, LD>), dim3(grid), dim3(64), 0, s, A);
+        hipLaunchKernelGGL((gradKernel<LA, LB, LC, LD, T>), dim3(grid), dim3(64), 0, s, A);
     }
 
     template <int LA, int LB, int LC, int LD>
     void registerGrad()
     {
         GradSlot &s = gradSlot(LA, LB, LC, LD);
-        s.fn = &launchGrad<LA, LB, LC, LD>;
+        s.fn = &launchGrad<LA, LB, LC, LD, double>;
+        s.fn32 = &launchGrad<LA, LB, LC, LD, float>;
         s.width = GCls<LA, LB, LC, LD>::G;
     }
 }

@@ -59,6 +59,7 @@ namespace lbx
         const double *ra;             // a / (a + b) per primitive pair
         const double *e1b, *e1k;      // d eb / d(A - B)_x, d ek / d(A - B)_x: [ppair][x][tuv][mu nu]
         const int *mol;               // molecule of each pair (batched contexts; may be null)
+        const float *e1b32, *e1k32;   // e1b, e1k rounded to FP32 (mixed-precision gradient)
     };
 
     // Per primitive pair input of the E-matrix kernels.
@@ -95,6 +96,7 @@ namespace lbx
         int *d_atom_a = nullptr, *d_atom_b = nullptr;
         double *d_ra = nullptr, *d_e1b = nullptr, *d_e1k = nullptr;
         int *d_mol = nullptr;
+        float *d_e1b32 = nullptr, *d_e1k32 = nullptr;   // made on first mixed gradient
 
         // doubles in each of d_eb, d_ek
         size_t eCount() const
@@ -106,7 +108,7 @@ namespace lbx
         {
             return DPairs{d_ao_a, d_ao_b, d_sh_a, d_sh_b, d_poff, d_pcnt, d_pp, d_eb, d_ek,
                           d_order, d_q, n_pairs, d_eb32, d_ek32, d_atom_a, d_atom_b, d_ra,
-                          d_e1b, d_e1k, d_mol};
+                          d_e1b, d_e1k, d_mol, d_e1b32, d_e1k32};
         }
     };
 

@@ -79,7 +79,7 @@ def lib():
                      "lbx_eri4d1", "lbx_eri3d1", "lbx_eri2d1"):
             getattr(L, name).argtypes = [p, p]
         L.lbx_jk.argtypes = [p, p, p, p, d, i, d, ctypes.POINTER(JKStats)]
-        L.lbx_jk_grad.argtypes = [p, p, p, d, ctypes.POINTER(JKStats)]
+        L.lbx_jk_grad.argtypes = [p, p, p, d, i, d, ctypes.POINTER(JKStats)]
         L.lbx_create_batch.argtypes = [ctypes.POINTER(p), i, ctypes.POINTER(_Basis),
                                        ctypes.POINTER(_Basis)]
         L.lbx_ri_sums.argtypes = [p, p, ctypes.POINTER(RIStats)]
@@ -233,16 +233,19 @@ class Context:
         _check(lib().lbx_eri2d1(self.h, out.data_ptr()))
         return out
 
-    def jk_grad(self, D, schwarz: float = 1e-12, grad=None):
-        """Nuclear gradient (natoms, 3) of E = 1/2 tr(DJ) - 1/4 tr(DK) at fixed D (lbx_jk_grad)."""
+    def jk_grad(self, D, schwarz: float = 1e-12, grad=None, precision: int = LBX_FP64,
+                mixed_thresh: float = 1e-5):
+        """Nuclear gradient (natoms, 3) of E = 1/2 tr(DJ) - 1/4 tr(DK) at fixed D (lbx_jk_grad);
+        LBX_MIXED evaluates the quartets with Schwarz bound below mixed_thresh in FP32."""
         torch = _torch()
         D = D.to(device=self.device, dtype=torch.float64).contiguous()
         if D.shape != (self.nbf, self.nbf):
             raise LbxError(f"density must be ({self.nbf}, {self.nbf})")
         g = self._out((self.natoms, 3)) if grad is None else grad
         st = JKStats()
-        _check(lib().lbx_jk_grad(self.h, D.data_ptr(), g.data_ptr(), schwarz, ctypes.byref(st)))
-        return g, {"quartets": st.quartets, "flops": st.flops}
+        _check(lib().lbx_jk_grad(self.h, D.data_ptr(), g.data_ptr(), schwarz, int(precision),
+                                 float(mixed_thresh), ctypes.byref(st)))
+        return g, {"quartets": st.quartets, "flops": st.flops, "quartets_fp32": st.quartets_fp32}
 
     def jk(self, D, schwarz: float = 1e-12, precision: int = LBX_FP64, mixed_thresh: float = 1e-5,
            J=None, K=None):

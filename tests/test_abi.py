@@ -39,6 +39,6 @@ def test_null_arguments_rejected():
     assert L.lbx_boys(99, 10, None, None, None) == -1
     assert L.lbx_eri4(None, None) == -1
     assert L.lbx_jk(None, None, None, None, 1e-12, 0, 1e-5, None) == -1
-    assert L.lbx_jk_grad(None, None, None, 1e-12, None) == -1
+    assert L.lbx_jk_grad(None, None, None, 1e-12, 0, 0.0, None) == -1
     assert L.lbx_create_batch(None, 0, None, None) == -1
     assert L.lbx_ri_sums(None, None, None) == -1

@@ -286,6 +286,29 @@ def test_jk_grad(lbx, oracle, thresh):
     assert st["quartets"] > 0 and st["flops"] > 0
 
 
+def test_jk_grad_mixed(lbx, oracle):
+    """Mixed-precision gradient: quartets with Schwarz bound below 1e-5 in FP32 arithmetic.  The
+    FP32 quartets carry relative rounding ~1e-7 of contributions bounded by ~1e-5 times the
+    derivative factors, so the gradient moves by far less than the 1e-7 Eh/bohr finite-difference
+    bar; with the split at 0 the mixed path is the FP64 one (up to the order of the atomic
+    additions)."""
+    import torch
+    m = B.synthetic_molecule(2025, 3, 4.5, B.RECIPE_MAIN)
+    ctx = lbx.Context(m)
+    D = B.density_from_orbitals(6, ctx.nbf, 15, 0.1)
+    Dt = torch.tensor(D, device="cuda")
+    gr = oracle.Structure(m).jk_grad(D, thresh=1e-12)
+    g64, _ = ctx.jk_grad(Dt, schwarz=1e-12)
+    g0, s0 = ctx.jk_grad(Dt, schwarz=1e-12, precision=lbx.LBX_MIXED, mixed_thresh=0.0)
+    assert s0["quartets_fp32"] == 0
+    assert (g0 - g64).abs().max().item() <= 1e-13 * g64.abs().max().item()
+    gm, sm = ctx.jk_grad(Dt, schwarz=1e-12, precision=lbx.LBX_MIXED, mixed_thresh=1e-5)
+    assert 0 < sm["quartets_fp32"] < sm["quartets"]
+    err = np.abs(gm.cpu().numpy() - gr).max()
+    assert err <= 1e-7, err
+    assert err > 0.0   # the FP32 tier did run
+
+
 def test_jk_grad_finite_difference(lbx):
     """The GPU gradient against central differences (h = 1e-4 bohr) of the GPU energy."""
     import torch
