@@ -97,6 +97,14 @@ def bench_ri_batch(a):
         ctx.ri_sums(out)
         log(f"[bench] warm-up step {k} done")
     step_s, (_, st) = timed(lambda: ctx.ri_sums(out), a.steps, "ri")
+    ref = out.cpu().numpy().copy()
+    # mixed precision: triplets / doublets with Schwarz bound below MIXED_SPLIT in FP32
+    outm = torch.empty_like(out)
+    ctx.ri_sums(outm, precision=lbx.LBX_MIXED, mixed_thresh=MIXED_SPLIT)
+    log("[bench] mixed first call (bounds, FP32 E matrices, graph) done")
+    mixed_s, (_, stm) = timed(lambda: ctx.ri_sums(outm, precision=lbx.LBX_MIXED,
+                                                  mixed_thresh=MIXED_SPLIT), a.steps, "ri mixed")
+    got = outm.cpu().numpy()
     tflops = st["flops"] / step_s / 1e12
     res = {
         "metric": "batched RI eri3 (mn|P) shell triplets/s (+ eri2 doublets/s)",
@@ -113,6 +121,14 @@ def bench_ri_batch(a):
         "total_frac_fp64_peak": tflops / FP64_PEAK_TFLOPS,
         "setup_s": t_setup, "first_call_s": t_first,
         "checksum_eri3_abs_total": float(out[:, 0].sum()),
+        "mixed": {"split": MIXED_SPLIT, "ms_per_step": mixed_s * 1e3,
+                  "triplets_per_s": stm["triplets"] / mixed_s,
+                  "fp32_share_triplets": stm["triplets_fp32"] / max(stm["triplets"], 1.0),
+                  "fp32_share_doublets": stm["doublets_fp32"] / max(stm["doublets"], 1.0),
+                  "speedup_vs_fp64": step_s / mixed_s,
+                  "max_rel_err_abs_sums": float((abs(got[:, [0, 2]] - ref[:, [0, 2]]) /
+                                                 abs(ref[:, [0, 2]])).max()),
+                  "max_abs_err_sums": float(abs(got[:, [1, 3]] - ref[:, [1, 3]]).max())},
     }
     if not a.no_cpu_baseline:
         # the oracle's dense eri3 + eri2 of the first small molecules, ~cpu-seconds of work

@@ -112,13 +112,19 @@ int lbx_jk(lbx_context *ctx, const double *D, double *J, double *K, double schwa
  * whole launch sequence is captured into a HIP graph on the first call (per out pointer) and
  * replayed afterwards.  Also valid on an ordinary context with an auxiliary basis (nmol = 1). */
 typedef struct lbx_ri_stats {
-    double triplets;   /* (ab|P) shell triplets evaluated */
-    double doublets;   /* (P|Q) shell doublets evaluated */
-    double flops;      /* algorithmic FLOPs */
+    double triplets;        /* (ab|P) shell triplets evaluated */
+    double doublets;        /* (P|Q) shell doublets evaluated */
+    double flops;           /* algorithmic FLOPs */
+    double triplets_fp32;   /* of which in FP32 (LBX_MIXED) */
+    double doublets_fp32;
 } lbx_ri_stats;
 int lbx_create_batch(lbx_context **ctx, int nmol, const lbx_basis *main_bases,
                      const lbx_basis *aux_bases);
-int lbx_ri_sums(lbx_context *ctx, double *out, lbx_ri_stats *stats);
+/* precision LBX_FP64, or LBX_MIXED: the triplets (doublets) whose Schwarz bound Q_ab Q_P
+ * (Q_P Q_Q), Q = sqrt(max |(xx|xx)|), is below mixed_thresh in FP32 arithmetic with FP64 Boys
+ * values and sums; nothing is screened out.  One graph per (out, precision split). */
+int lbx_ri_sums(lbx_context *ctx, double *out, int precision, double mixed_thresh,
+                lbx_ri_stats *stats);
 
 /* Nuclear gradient of E = 1/2 tr(DJ) - 1/4 tr(DK) for a fixed density, no reference counterpart
  * (the reference supplies the derivative integrals, ERI4D1Kernel, eri_kernels.hpp:113-131 /

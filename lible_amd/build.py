@@ -23,7 +23,7 @@ INCLUDE = os.path.join(ROOT, "include")
 SELFTEST_SRC = os.path.join(ROOT, "tests", "cpp", "ints_selftest.cpp")
 SELFTEST = os.path.join(HERE, "bin", "ints_selftest")
 
-N_UNITS = 22   # csrc/instances.hip: 0..11 FP64 forms and dense outputs, 12..21 FP32 J/K forms
+N_UNITS = 24   # csrc/instances.hip: 0..11 FP64 forms and dense outputs, 12..23 FP32 forms
 N_GRAD_UNITS = 10   # csrc/grad_instances.hip: one per lane-side main pair class
 
 FLAGS = [

@@ -298,12 +298,27 @@ namespace lbx
             return quartets;
         }
 
-        // Tasks of the batched RI sums: for every uniform-side position q (identity order: a
-        // batched context is never Schwarz-sorted) the lane-side pairs of q's molecule, and with
-        // `same` only lane positions p >= q.  No screening.
+        // Tasks of the batched RI sums: for every uniform-side position q the lane-side pairs of
+        // q's molecule (each molecule's positions are contiguous), with `same` only lane
+        // positions p >= q, and with bounds (both sides sorted by Q within each molecule) only
+        // those with lo <= Q_p Q_q < hi.  No screening below: every triplet of the tensors.
         TaskTable buildTasksMol(Context &c, const PairClass &lane, const PairClass &unif, bool same,
-                                int width)
+                                int width, double lo_bound = 0.0, double hi_bound = kNoUpper)
         {
+            const bool bounded = lo_bound > 0.0 || hi_bound < kNoUpper;
+            auto qOf = [](const PairClass &pc, int pos) { return pc.Q.empty() ? 1.0 : pc.Q[pc.order[pos]]; };
+            // first position in [a, b) whose bound with qk falls below thr (Q descending there)
+            auto firstBelowIn = [&](int a, int b, double qk, double thr) {
+                while (a < b)
+                {
+                    const int m = (a + b) / 2;
+                    if (qOf(lane, m) * qk >= thr)
+                        a = m + 1;
+                    else
+                        b = m;
+                }
+                return a;
+            };
             TaskTable t;
             t.nket = unif.n_pairs;
             t.plo.resize(t.nket);
@@ -326,6 +341,15 @@ namespace lbx
             {
                 const int m = unif.mol[unif.order[q]];
                 int a = hi[m] > 0 ? lo[m] : 0, b = hi[m];
+                if (bounded && b > a)
+                {
+                    const double qk = qOf(unif, q);
+                    const int a0 = a, b0 = b;
+                    if (hi_bound < kNoUpper)
+                        a = firstBelowIn(a0, b0, qk, hi_bound);
+                    if (lo_bound > 0.0)
+                        b = firstBelowIn(a0, b0, qk, lo_bound);
+                }
                 if (same)
                     a = std::max(a, q);
                 b = std::max(a, b);
@@ -452,6 +476,23 @@ namespace lbx
             }
         }
 
+        // FP32 copies of the auxiliary shells' E matrices (mixed-precision RI sums)
+        void ensureF32Aux(Context &c)
+        {
+            for (auto &pc : c.aux)
+            {
+                if (pc.d_eb32 || pc.n_ppairs == 0)
+                    continue;
+                const long long n = static_cast<long long>(pc.eCount());
+                pc.d_eb32 = c.alloc<float>(n);
+                pc.d_ek32 = c.alloc<float>(n);
+                const int grid = static_cast<int>(std::min<long long>((n + 255) / 256, 65536));
+                hipLaunchKernelGGL(toFloatKernel, dim3(grid), dim3(256), 0, c.stream, pc.d_eb, pc.d_eb32, n);
+                hipLaunchKernelGGL(toFloatKernel, dim3(grid), dim3(256), 0, c.stream, pc.d_ek, pc.d_ek32, n);
+                LBX_HIP(hipGetLastError());
+            }
+        }
+
         // FP32 copies of the derivative E matrices (mixed-precision gradient)
         void ensureF32D1(Context &c)
         {
@@ -543,32 +584,49 @@ namespace lbx
 
     // Schwarz factors Q_ab = sqrt(max |(ab|ab)|) per pair of every main class; pairs sorted by
     // descending Q so that each ket's surviving bra pairs form a prefix.
+    // Schwarz factors Q = sqrt(max |(ab|ab)|) of one class (kind 4: pairs, kind 2: single
+    // auxiliary shells) and the sorted order: by molecule, then Q descending (one molecule in an
+    // ordinary context), so each molecule's pairs keep their contiguous range of positions.
+    void sortByBound(Context &c, PairClass &pc, int kind)
+    {
+        if (pc.n_pairs == 0)
+            return;
+        KArgs extra{};
+        extra.qout = pc.d_q;
+        extra.kind = kind;
+        run(c, MODE_DIAG, pc, pc, Pairing::Diagonal, true, 0.0, extra);
+        pc.Q.resize(pc.n_pairs);
+        LBX_HIP(hipMemcpy(pc.Q.data(), pc.d_q, pc.n_pairs * sizeof(double), hipMemcpyDeviceToHost));
+        std::vector<int> order(pc.n_pairs);
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+            return pc.mol[x] != pc.mol[y] ? pc.mol[x] < pc.mol[y] : pc.Q[x] > pc.Q[y];
+        });
+        std::vector<double> qs(pc.n_pairs);
+        for (int i = 0; i < pc.n_pairs; i++)
+            qs[i] = pc.Q[order[i]];
+        pc.order = order;
+        LBX_HIP(hipMemcpy(pc.d_order, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice));
+        LBX_HIP(hipMemcpy(pc.d_q, qs.data(), qs.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+
     void ensureSchwarz(Context &c)
     {
         if (c.have_schwarz)
             return;
         for (auto &pc : c.cls)
-        {
-            if (pc.n_pairs == 0)
-                continue;
-            KArgs extra{};
-            extra.qout = pc.d_q;
-            extra.kind = 4;
-            run(c, MODE_DIAG, pc, pc, Pairing::Diagonal, true, 0.0, extra);
-            pc.Q.resize(pc.n_pairs);
-            LBX_HIP(hipMemcpy(pc.Q.data(), pc.d_q, pc.n_pairs * sizeof(double), hipMemcpyDeviceToHost));
-            std::vector<int> order(pc.n_pairs);
-            std::iota(order.begin(), order.end(), 0);
-            std::stable_sort(order.begin(), order.end(),
-                             [&](int x, int y) { return pc.Q[x] > pc.Q[y]; });
-            std::vector<double> qs(pc.n_pairs);
-            for (int i = 0; i < pc.n_pairs; i++)
-                qs[i] = pc.Q[order[i]];
-            pc.order = order;
-            LBX_HIP(hipMemcpy(pc.d_order, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice));
-            LBX_HIP(hipMemcpy(pc.d_q, qs.data(), qs.size() * sizeof(double), hipMemcpyHostToDevice));
-        }
+            sortByBound(c, pc, 4);
         c.have_schwarz = true;
+    }
+
+    // the same for the auxiliary shells (bounds of the RI sums' precision split)
+    void ensureSchwarzAux(Context &c)
+    {
+        if (c.have_schwarz_aux)
+            return;
+        for (auto &pc : c.aux)
+            sortByBound(c, pc, 2);
+        c.have_schwarz_aux = true;
     }
 }
 
@@ -729,7 +787,8 @@ struct lbx_context : Context
     hipGraph_t ri_graph = nullptr;
     hipGraphExec_t ri_exec = nullptr;
     double *ri_out = nullptr;
-    double ri_triplets = 0, ri_doublets = 0, ri_flops = 0;
+    double ri_triplets = 0, ri_doublets = 0, ri_flops = 0, ri_triplets32 = 0, ri_doublets32 = 0;
+    double ri_split = 0.0;   // precision split the prepared launches were built for (0: FP64)
 
     void dropRiGraph()
     {
@@ -743,38 +802,65 @@ struct lbx_context : Context
     }
 
     // Kernel launches of the RI sums: every eri3 (pair class | aux l) and eri2
-    // (aux l | aux l' <= l) class pair, with task tables built once (device memory cannot be
-    // allocated while a stream is being captured).
+    // (aux l | aux l' <= l) class pair, with task tables built once per precision split (device
+    // memory cannot be allocated while a stream is being captured).  split > 0: the triplets and
+    // doublets with bound Q_ab Q_P below it in the FP32 form where the class has one.
     std::vector<std::pair<Launcher, KArgs>> ri_launches;
-    bool ri_prepared = false;
+    std::map<double, std::vector<std::pair<Launcher, KArgs>>> ri_prepared;
+    std::map<double, std::array<double, 5>> ri_counts;   // triplets, doublets, flops, FP32 ones
 
-    void prepareRiSums()
+    void prepareRiSums(double split)
     {
-        if (ri_prepared)
-            return;
-        ri_triplets = ri_doublets = ri_flops = 0;
-        auto add = [&](const PairClass &lane, const PairClass &unif, bool same, int kind)
+        auto it = ri_prepared.find(split);
+        if (it == ri_prepared.end())
         {
-            if (lane.n_pairs == 0 || unif.n_pairs == 0)
-                return;
-            int var = VAR_VALU_BRA;
-            if (!kernelSlot(MODE_SUM, lane.la, lane.lb, unif.la, unif.lb, var).fn)
-                var = VAR_MFMA;
-            const KernelSlot &ks = slotFor(MODE_SUM, lane, unif, var);
-            TaskTable t = buildTasksMol(*this, lane, unif, same, ks.width);
-            KArgs a = baseArgs(*this, lane, unif, t, same);
-            a.kind = kind;
-            ri_launches.emplace_back(ks.fn, a);
-            (kind == 3 ? ri_triplets : ri_doublets) += t.quartets;
-            ri_flops += t.flops;
-        };
-        for (auto &pc : cls)
-            for (auto &ax : aux)
-                add(pc, ax, false, 3);
-        for (int la = 0; la < static_cast<int>(aux.size()); la++)
-            for (int lc = 0; lc <= la; lc++)
-                add(aux[la], aux[lc], la == lc, 2);
-        ri_prepared = true;
+            std::vector<std::pair<Launcher, KArgs>> launches;
+            std::array<double, 5> n{0, 0, 0, 0, 0};
+            auto add = [&](const PairClass &lane, const PairClass &unif, bool same, int kind)
+            {
+                if (lane.n_pairs == 0 || unif.n_pairs == 0)
+                    return;
+                int var = VAR_VALU_BRA;
+                if (!kernelSlot(MODE_SUM, lane.la, lane.lb, unif.la, unif.lb, var).fn)
+                    var = VAR_MFMA;
+                const KernelSlot &ks = slotFor(MODE_SUM, lane, unif, var);
+                const KernelSlot &k32 = kernelSlot(MODE_SUM, lane.la, lane.lb, unif.la, unif.lb, VAR_VALU_BRA32);
+                const bool tiers = split > 0.0 && k32.fn;
+                auto one = [&](const KernelSlot &k, double lo, double hi, bool fp32)
+                {
+                    TaskTable t = buildTasksMol(*this, lane, unif, same, k.width, lo, hi);
+                    KArgs a = baseArgs(*this, lane, unif, t, same);
+                    a.kind = kind;
+                    launches.emplace_back(k.fn, a);
+                    n[kind == 3 ? 0 : 1] += t.quartets;
+                    n[2] += t.flops;
+                    if (fp32)
+                        n[kind == 3 ? 3 : 4] += t.quartets;
+                };
+                if (tiers)
+                {
+                    one(ks, split, kNoUpper, false);
+                    one(k32, 0.0, split, true);
+                }
+                else
+                    one(ks, 0.0, kNoUpper, false);
+            };
+            for (auto &pc : cls)
+                for (auto &ax : aux)
+                    add(pc, ax, false, 3);
+            for (int la = 0; la < static_cast<int>(aux.size()); la++)
+                for (int lc = 0; lc <= la; lc++)
+                    add(aux[la], aux[lc], la == lc, 2);
+            it = ri_prepared.emplace(split, std::move(launches)).first;
+            ri_counts[split] = n;
+        }
+        ri_launches = it->second;
+        const auto &n = ri_counts[split];
+        ri_triplets = n[0];
+        ri_doublets = n[1];
+        ri_flops = n[2];
+        ri_triplets32 = n[3];
+        ri_doublets32 = n[4];
     }
 
     // fill + launches on stream s (the capture stream)
@@ -1378,20 +1464,34 @@ extern "C"
                        });
     }
 
-    int lbx_ri_sums(lbx_context *ctx, double *out, lbx_ri_stats *stats)
+    int lbx_ri_sums(lbx_context *ctx, double *out, int precision, double mixed_thresh,
+                    lbx_ri_stats *stats)
     {
         return guarded([&]
                        {
                            if (!ctx || !out)
                                throw Error(LBX_ERR_INVALID, "lbx_ri_sums: null argument");
+                           if (precision != LBX_FP64 && precision != LBX_MIXED)
+                               throw Error(LBX_ERR_INVALID, "lbx_ri_sums: precision must be LBX_FP64 or LBX_MIXED");
+                           if (precision == LBX_MIXED && !(mixed_thresh >= 0))
+                               throw Error(LBX_ERR_INVALID, "lbx_ri_sums: negative mixed-precision threshold");
                            std::lock_guard<std::mutex> lk(ctx->mu);
                            Context &c = *ctx;
                            if (c.shells_aux.empty())
                                throw Error(LBX_ERR_STATE, "lbx_ri_sums: the context has no auxiliary basis");
-                           if (!ctx->ri_exec || ctx->ri_out != out)
+                           const double split = precision == LBX_MIXED ? mixed_thresh : 0.0;
+                           if (!ctx->ri_exec || ctx->ri_out != out || ctx->ri_split != split)
                            {
                                ctx->dropRiGraph();
-                               ctx->prepareRiSums();
+                               if (split > 0.0)
+                               {
+                                   ensureSchwarz(c);
+                                   ensureSchwarzAux(c);
+                                   ensureF32(c);
+                                   ensureF32Aux(c);
+                               }
+                               ctx->prepareRiSums(split);
+                               ctx->ri_split = split;
                                // capture on a private stream (default priority), replay on the
                                // context stream
                                hipStream_t cs = nullptr;
@@ -1430,6 +1530,8 @@ extern "C"
                                stats->triplets = ctx->ri_triplets;
                                stats->doublets = ctx->ri_doublets;
                                stats->flops = ctx->ri_flops;
+                               stats->triplets_fp32 = ctx->ri_triplets32;
+                               stats->doublets_fp32 = ctx->ri_doublets32;
                            }
                        });
     }

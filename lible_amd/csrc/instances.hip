@@ -3,7 +3,7 @@
 //          forms) and dense for every ket class <= bra class, the (ab|ab) diagonal, and eri3
 //          (ab|c) against every auxiliary shell l = 0..5 not already covered;
 //   10, 11 bra = single auxiliary shell l = 4, 5: eri2 (a|b) for b <= l and the (a|a) diagonal;
-//   12..21 bra = main pair class LBX_UNIT - 12: the FP32 J/K forms.
+//   12..23 the FP32 forms of units 0..11: J/K and the batched RI sums.
 // Splitting the instantiations lets the build compile the units in parallel.
 #include "quartet.hpp"
 
@@ -54,8 +54,8 @@ namespace lbx
             if constexpr (!kF32Unit && kPairUnit < 10 && !(LC <= 3 && pairClassIdx(LC, 0) <= CI))
                 registerQuartet<PA, PB, LC, 0, MODE_DENSE, 1>();
             // batched RI sums: eri3 (PA PB | LC), and eri2 (PA | LC) when PB = 0
-            if constexpr (!kF32Unit && kPairUnit < 10)
-                registerQuartet<PA, PB, LC, 0, MODE_SUM, 1>();
+            if constexpr (kPairUnit < 10)
+                registerQuartet<PA, PB, LC, 0, MODE_SUM, kF32Unit ? 2 : 1>();
         }
 
         template <int LC>
@@ -66,6 +66,8 @@ namespace lbx
                 registerQuartet<PA, 0, LC, 0, MODE_DENSE, 1>();
                 registerQuartet<PA, 0, LC, 0, MODE_SUM, 1>();
             }
+            if constexpr (kF32Unit && kPairUnit >= 10 && LC <= PA)
+                registerQuartet<PA, 0, LC, 0, MODE_SUM, 2>();
         }
 
         struct Registrar

@@ -124,7 +124,7 @@ namespace lbx
         std::vector<PairClass> cls;      // index pairClassIdx(la, lb)
         std::vector<PairClass> aux;      // index l
         double *d_boys = nullptr;        // Boys table kBoysRows x kBoysCols
-        bool have_schwarz = false;
+        bool have_schwarz = false, have_schwarz_aux = false;
         // batched contexts: molecule of each atom (atoms grouped by molecule); pairs only form
         // within a molecule.  Empty for an ordinary (one-molecule) context.
         std::vector<int> atom_mol;

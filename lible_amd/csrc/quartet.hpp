@@ -183,7 +183,7 @@ namespace lbx
         // uniform side with >= 4 Hermite rows of X.
         constexpr bool kSkipL = NC >= 2;
         constexpr bool kSkipU = HLn >= 4;
-        static_assert(!F32 || MODE == MODE_JK, "FP32 form is for J/K only");
+        static_assert(!F32 || MODE == MODE_JK || MODE == MODE_SUM, "FP32 forms: J/K and RI sums");
         const DPairs &LP = SWAP ? A.ket : A.bra;
         const DPairs &UP = SWAP ? A.bra : A.ket;
 
@@ -974,7 +974,7 @@ namespace lbx
     {
         using C = Cls<LA, LB, LC, LD>;
         constexpr bool same = LA == LC && LB == LD;
-        if constexpr (MODE == MODE_JK && C::NC0f > 0)
+        if constexpr ((MODE == MODE_JK || MODE == MODE_SUM) && C::NC0f > 0)
         {
             KernelSlot &s = kernelSlot(MODE, LA, LB, LC, LD, VAR_VALU_BRA32);
             s.fn = &launchValu<LA, LB, LC, LD, MODE, false, float>;

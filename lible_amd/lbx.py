@@ -44,7 +44,8 @@ class JKStats(ctypes.Structure):
 
 class RIStats(ctypes.Structure):
     _fields_ = [("triplets", ctypes.c_double), ("doublets", ctypes.c_double),
-                ("flops", ctypes.c_double)]
+                ("flops", ctypes.c_double), ("triplets_fp32", ctypes.c_double),
+                ("doublets_fp32", ctypes.c_double)]
 
 
 class ClassProfile(ctypes.Structure):
@@ -82,7 +83,7 @@ def lib():
         L.lbx_jk_grad.argtypes = [p, p, p, d, i, d, ctypes.POINTER(JKStats)]
         L.lbx_create_batch.argtypes = [ctypes.POINTER(p), i, ctypes.POINTER(_Basis),
                                        ctypes.POINTER(_Basis)]
-        L.lbx_ri_sums.argtypes = [p, p, ctypes.POINTER(RIStats)]
+        L.lbx_ri_sums.argtypes = [p, p, i, d, ctypes.POINTER(RIStats)]
         L.lbx_tune.argtypes = [p, p, d]
         L.lbx_set_variant.argtypes = [p, i, i, i, i, i]
         L.lbx_profile_enable.argtypes = [p, i]
@@ -155,13 +156,16 @@ class Context:
         self.nbf, self.nbf_aux, self.natoms = nbf.value, nbf_aux.value, nat.value
         self.device = torch.device("cuda", torch.cuda.current_device())
 
-    def ri_sums(self, out=None):
+    def ri_sums(self, out=None, precision: int = LBX_FP64, mixed_thresh: float = 1e-5):
         """Per molecule: sum |eri3|, sum eri3, sum |eri2|, sum eri2 over the dense tensors
-        (lbx_ri_sums) -> ((nmol, 4) tensor, stats)."""
+        (lbx_ri_sums) -> ((nmol, 4) tensor, stats); LBX_MIXED evaluates the triplets and
+        doublets with Schwarz bound below mixed_thresh in FP32."""
         o = self._out((self.nmol, 4)) if out is None else out
         st = RIStats()
-        _check(lib().lbx_ri_sums(self.h, o.data_ptr(), ctypes.byref(st)))
-        return o, {"triplets": st.triplets, "doublets": st.doublets, "flops": st.flops}
+        _check(lib().lbx_ri_sums(self.h, o.data_ptr(), int(precision), float(mixed_thresh),
+                                 ctypes.byref(st)))
+        return o, {"triplets": st.triplets, "doublets": st.doublets, "flops": st.flops,
+                   "triplets_fp32": st.triplets_fp32, "doublets_fp32": st.doublets_fp32}
 
     def _basis(self, coords, sh, natoms) -> _Basis:
         arrs = [np.ascontiguousarray(coords, np.float64), np.ascontiguousarray(sh.atom, np.int32),

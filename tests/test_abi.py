@@ -41,4 +41,4 @@ def test_null_arguments_rejected():
     assert L.lbx_jk(None, None, None, None, 1e-12, 0, 1e-5, None) == -1
     assert L.lbx_jk_grad(None, None, None, 1e-12, 0, 0.0, None) == -1
     assert L.lbx_create_batch(None, 0, None, None) == -1
-    assert L.lbx_ri_sums(None, None, None) == -1
+    assert L.lbx_ri_sums(None, None, 0, 0.0, None) == -1

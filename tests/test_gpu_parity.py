@@ -270,6 +270,27 @@ def test_ri_sums_batch(lbx, oracle):
         ctx.eri3()
 
 
+def test_ri_sums_mixed(lbx, oracle):
+    """Mixed-precision RI sums: triplets and doublets with Schwarz bound below the split in FP32.
+    Their values are bounded by the split and carry ~1e-7 relative rounding, so the sums of |x|
+    (10^2-10^4) move by far less than 1e-8 relative; the FP64 sums stay those of the dense
+    tensors before and after the bound ordering the mixed call sets up."""
+    mols = [B.synthetic_molecule(400 + k, 3 + 2 * k, 4.0 + k, B.RECIPE_MAIN, B.RECIPE_AUX)
+            for k in range(4)]
+    ctx = lbx.Context.batch(mols)
+    ref = np.array([dense_sums(oracle.Structure(m).eri3(), oracle.Structure(m).eri2()) for m in mols])
+    f64, _ = ctx.ri_sums()
+    close(f64.cpu().numpy(), ref)
+    mixed, st = ctx.ri_sums(precision=lbx.LBX_MIXED, mixed_thresh=1e-5)
+    assert 0 < st["triplets_fp32"] < st["triplets"] and st["doublets_fp32"] < st["doublets"]
+    got = mixed.cpu().numpy()
+    rel = np.abs(got[:, [0, 2]] - ref[:, [0, 2]]) / np.abs(ref[:, [0, 2]])
+    assert rel.max() <= 1e-8, rel.max()
+    assert np.abs(got[:, [1, 3]] - ref[:, [1, 3]]).max() <= 1e-6
+    again, _ = ctx.ri_sums()
+    close(again.cpu().numpy(), ref)
+
+
 @pytest.mark.parametrize("thresh", [0.0, 1e-12])
 def test_jk_grad(lbx, oracle, thresh):
     """Nuclear gradient of E = 1/2 tr(DJ) - 1/4 tr(DK) against the oracle's contraction of its
