@@ -61,6 +61,19 @@ def cpu_baseline(mol, D, gpu_stats, target_s: float):
     }
 
 
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter runs
+    (profiles/jk_c2_traffic.json: FETCH_SIZE and WRITE_SIZE passes, each scaled by the same
+    counter's reading for a copy of known size), or None when they cover another kernel."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "jk_c2_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    return t.get("traffic_bytes") if t.get("kernel") == kernel else None
+
+
 def timed(fn, steps: int, label: str):
     import torch
     torch.cuda.synchronize()
@@ -235,6 +248,9 @@ def main(argv=None):
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--mixed-steps", type=int, default=1,
                     help="timed mixed-precision builds after the FP64 ones (0: skip)")
+    ap.add_argument("--pmc-calibration", action="store_true",
+                    help="end with a 1 GiB device-to-device copy, the reference reading for "
+                         "rocprofv3's FETCH_SIZE / WRITE_SIZE counters")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
@@ -366,7 +382,8 @@ def main(argv=None):
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": dom_tflops / FP64_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": measured_traffic("%s<%d,%d,%d,%d> J/K (%s)" % (
+                "quartetM" if dom["kernel"] == "mfma" else "quartetV", *dom["class"], dom["kernel"])),
             "kernel_ms": dom_ms,
             "kernel_share": dom["ms"] / max(sum(p["ms"] for p in prof), 1e-12),
         },
@@ -385,6 +402,12 @@ def main(argv=None):
         log("[bench] cpu baseline (oracle sample)")
         out["cpu_baseline"] = cpu_baseline(mol, Dh, st, a.cpu_seconds)
     print(json.dumps(out), flush=True)
+    if a.pmc_calibration:
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("[bench] calibration copy done: 1 GiB read, 1 GiB written")
     return 0
 
 
