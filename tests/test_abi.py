@@ -42,3 +42,5 @@ def test_null_arguments_rejected():
     assert L.lbx_jk_grad(None, None, None, 1e-12, 0, 0.0, None) == -1
     assert L.lbx_create_batch(None, 0, None, None) == -1
     assert L.lbx_ri_sums(None, None, 0, 0.0, None) == -1
+    for name in ("lbx_eri4d1", "lbx_eri3d1", "lbx_eri2d1"):
+        assert getattr(L, name)(None, None) == -1
