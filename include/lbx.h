@@ -138,10 +138,10 @@ int lbx_ri_sums(lbx_context *ctx, double *out, int precision, double mixed_thres
 int lbx_jk_grad(lbx_context *ctx, const double *D, double *grad, double schwarz_thresh,
                 int precision, double mixed_thresh, lbx_jk_stats *stats);
 
-/* Autotuner (no reference counterpart).  Each class pair (la lb|lc ld) has up to five kernel
+/* Autotuner (no reference counterpart).  Each class pair (la lb|lc ld) has up to six kernel
  * variants: 0 one lane per quartet with the bra pairs on the lanes, 1 the same with the ket
- * pairs on the lanes, 2 one wave per quartet on FP64 MFMA, and 3, 4 the forms 0, 1 with R and
- * both transforms in FP32 (the small-bound quartets of LBX_MIXED).  lbx_tune times every
+ * pairs on the lanes, 2 one wave per quartet on FP64 MFMA, and 3, 4, 5 the forms 0, 1, 2 with R
+ * and both transforms in FP32 (5 on the FP32 MFMA; the small-bound quartets of LBX_MIXED).  lbx_tune times every
  * variant of every class pair on this density and threshold (on a sample of ~2M quartets of
  * each class pair: every k-th pair of one side with all its partners) and keeps the fastest per
  * quartet: the fastest FP64 form
@@ -149,7 +149,7 @@ int lbx_jk_grad(lbx_context *ctx, const double *D, double *grad, double schwarz_
  * lbx_jk picks by a work estimate from the primitive-pair counts. */
 int lbx_tune(lbx_context *ctx, const double *D, double schwarz_thresh);
 /* Fixes the variant of class pair (la lb|lc ld), la >= lb, lc >= ld, bra class >= ket class in
- * the order (0,0),(1,0),(1,1),(2,0),...: variants 0-2 for FP64 work, 3-4 for the FP32 tier;
+ * the order (0,0),(1,0),(1,1),(2,0),...: variants 0-2 for FP64 work, 3-5 for the FP32 tier;
  * fails if the class pair has no such variant.  variant -1 restores both defaults. */
 int lbx_set_variant(lbx_context *ctx, int la, int lb, int lc, int ld, int variant);
 
@@ -159,7 +159,7 @@ int lbx_set_variant(lbx_context *ctx, int la, int lb, int lc, int ld, int varian
  * lbx_profile_read returns the accumulated totals since the last enable. */
 typedef struct lbx_class_profile {
     int la, lb, lc, ld;
-    int kernel;            /* variant: 0 bra on lanes, 1 ket on lanes, 2 MFMA, 3 / 4 FP32 0 / 1 */
+    int kernel;            /* variant: 0 bra on lanes, 1 ket on lanes, 2 MFMA, 3 / 4 / 5 FP32 0 / 1 / 2 */
     int calls;
     double ms;             /* summed kernel time */
     double quartets;       /* summed quartets */

@@ -440,13 +440,14 @@ namespace lbx
         }
 
         // Default for the FP32 tier of a mixed-precision J/K: the cheaper FP32 one-lane form by
-        // the same estimate, or -1 when the class has none (its quartets then stay FP64).
+        // the same estimate, else the FP32 MFMA form, or -1 when the class has neither (its
+        // quartets then stay FP64).
         int defaultVariant32(const PairClass &bra, const PairClass &ket)
         {
             const KernelSlot &vb = kernelSlot(MODE_JK, bra.la, bra.lb, ket.la, ket.lb, VAR_VALU_BRA32);
             const KernelSlot &vk = kernelSlot(MODE_JK, bra.la, bra.lb, ket.la, ket.lb, VAR_VALU_KET32);
             if (!vb.fn && !vk.fn)
-                return -1;
+                return kernelSlot(MODE_JK, bra.la, bra.lb, ket.la, ket.lb, VAR_MFMA32).fn ? VAR_MFMA32 : -1;
             double cb = vb.fn ? valuCost(bra, ket, vb.nc) : 1e300;
             double ck = vk.fn ? valuCost(ket, bra, vk.nc) : 1e300;
             return cb <= ck ? VAR_VALU_BRA32 : VAR_VALU_KET32;

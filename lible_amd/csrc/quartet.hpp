@@ -603,17 +603,36 @@ namespace lbx
     }
 
     // ------------------------------------------------------------------------------- quartetM
-    template <int LA, int LB, int LC, int LD, int MODE>
+    typedef float v4f __attribute__((ext_vector_type(4)));
+
+    // 16x16x4 MFMA in FP64 or FP32.  Both take A[j][g] and B[g][j] from lane (g, j) = (lane / 16,
+    // lane % 16); accumulator element r of that lane holds D[4 r + g][j] for FP64 and
+    // D[4 g + r][j] for FP32.
+    __device__ __forceinline__ v4d mfma16(double a, double b, v4d c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    __device__ __forceinline__ v4f mfma16(float a, float b, v4f c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+
+    // T = double, or float (the small-bound quartets of mixed-precision J/K: R, the E matrices
+    // and both transforms in FP32 on the FP32 MFMA; Boys values and the epilogue in FP64).
+    template <int LA, int LB, int LC, int LD, int MODE, typename T = double>
     __global__ __launch_bounds__(64) void quartetM(const KArgs A)
     {
         using C = Cls<LA, LB, LC, LD>;
         constexpr int L = C::L, HP = C::HP, HQ = C::HQ, SP = C::SP, SQ = C::SQ, HL = C::HL;
         constexpr int NQ = C::NQ, NP = C::NP, KQ = C::KQ, MT = C::MT, RC = C::RC, CHP = C::CHP;
         constexpr int SA = C::SA, SB = C::SB, SC = C::SC, SD = C::SD;
+        constexpr bool F32 = sizeof(T) == 4;
+        using Acc = std::conditional_t<F32, v4f, v4d>;
+        static_assert(!F32 || MODE == MODE_JK, "FP32 MFMA form: J/K only");
         static_assert(CHP <= 64, "one lane per primitive pair of a chunk");
-        __shared__ double sR[2][CHP][HL];
+        __shared__ T sR[2][CHP][HL];
         __shared__ double sInit[CHP][L + 1];
-        __shared__ double sPQ[CHP][3];
+        __shared__ T sPQ[CHP][3];
         __shared__ unsigned short sTab[HP][4 * KQ];
         __shared__ double sI[SP * SQ];
         __shared__ double sD[6][49];
@@ -635,7 +654,7 @@ namespace lbx
             sTab[h][k] = static_cast<unsigned short>(v);
         }
         for (int e = lane; e < 2 * CHP * HL; e += 64)
-            (&sR[0][0][0])[e] = 0.0;
+            (&sR[0][0][0])[e] = T(0);
         __syncthreads();
 
         // contiguous task range per workgroup
@@ -662,12 +681,12 @@ namespace lbx
             const int bpoff = uniform(A.bra.poff[ab]), bcnt = uniform(A.bra.pcnt[ab]);
             const int nrows = bcnt * HP;
 
-            v4d acc2[NQ][NP];
+            Acc acc2[NQ][NP];
 #pragma unroll
             for (int a = 0; a < NQ; a++)
 #pragma unroll
                 for (int b = 0; b < NP; b++)
-                    acc2[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
+                    acc2[a][b] = Acc{0, 0, 0, 0};
 
             for (int rowbase = 0; rowbase < nrows; rowbase += RC)
             {
@@ -675,12 +694,12 @@ namespace lbx
                 const int primEnd = min(bcnt, (rowbase + RC - 1) / HP + 1);
                 const int nprim = primEnd - prim0;
 
-                v4d acc1[MT][NQ];
+                Acc acc1[MT][NQ];
 #pragma unroll
                 for (int a = 0; a < MT; a++)
 #pragma unroll
                     for (int b = 0; b < NQ; b++)
-                        acc1[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
+                        acc1[a][b] = Acc{0, 0, 0, 0};
 
                 for (int k = 0; k < kcnt; k++)
                 {
@@ -692,9 +711,9 @@ namespace lbx
                         const double pq = P.x * Q.x, ps = P.x + Q.x;
                         const double alpha = pq / ps;
                         const double X_ = P.y - Q.y, Y_ = P.z - Q.z, Z_ = P.w - Q.w;
-                        const double T = alpha * (X_ * X_ + Y_ * Y_ + Z_ * Z_);
+                        const double Tb = alpha * (X_ * X_ + Y_ * Y_ + Z_ * Z_);
                         double F[L + 1];
-                        boysF<L>(T, A.boys, F);
+                        boysF<L>(Tb, A.boys, F);
                         double sc = kTwoPi52 / (pq * sqrt(ps));
 #pragma unroll
                         for (int m = 0; m <= L; m++)
@@ -702,10 +721,10 @@ namespace lbx
                             sInit[lane][m] = F[m] * sc;
                             sc *= -2.0 * alpha;
                         }
-                        sPQ[lane][0] = X_;
-                        sPQ[lane][1] = Y_;
-                        sPQ[lane][2] = Z_;
-                        sR[L & 1][lane][0] = sInit[lane][L];
+                        sPQ[lane][0] = T(X_);
+                        sPQ[lane][1] = T(Y_);
+                        sPQ[lane][2] = T(Z_);
+                        sR[L & 1][lane][0] = T(sInit[lane][L]);
                     }
                     __syncthreads();
                     // levels n = L-1 .. 0, ping-pong buffers
@@ -713,8 +732,8 @@ namespace lbx
                     for (int n = L - 1; n >= 0; n--)
                     {
                         const int nh = nHerm(L - n) - 1;
-                        const double(*in)[HL] = sR[(n + 1) & 1];
-                        double(*out)[HL] = sR[n & 1];
+                        const T(*in)[HL] = sR[(n + 1) & 1];
+                        T(*out)[HL] = sR[n & 1];
                         const int items = nprim * nh;
                         for (int it = lane; it < items; it += 64)
                         {
@@ -723,24 +742,28 @@ namespace lbx
                             const unsigned r = kRecTab.v[h];
                             const int s1 = r & 2047, s2 = (r >> 11) & 2047, ax = (r >> 22) & 3,
                                       c = (r >> 24);
-                            out[pr][h] = fma(sPQ[pr][ax], in[pr][s1], c * in[pr][s2]);
+                            out[pr][h] = fma(sPQ[pr][ax], in[pr][s1], T(c) * in[pr][s2]);
                         }
                         if (lane < nprim)
-                            out[lane][0] = sInit[lane][n];
+                            out[lane][0] = T(sInit[lane][n]);
                         __syncthreads();
                     }
                     // ket transform on MFMA: acc1[mt][nt] += Rg[rows, k] Ek[k, cols]
-                    const double *ek = A.ket.ek + static_cast<size_t>(kpoff + k) * HQ * SQ;
+                    const T *ek;
+                    if constexpr (F32)
+                        ek = A.ket.ek32 + static_cast<size_t>(kpoff + k) * HQ * SQ;
+                    else
+                        ek = A.ket.ek + static_cast<size_t>(kpoff + k) * HQ * SQ;
 #pragma unroll
                     for (int kt = 0; kt < KQ; kt++)
                     {
                         const int kk = 4 * kt + g;
-                        double b1[NQ];
+                        T b1[NQ];
 #pragma unroll
                         for (int nt = 0; nt < NQ; nt++)
                         {
                             const int col = 16 * nt + j;
-                            b1[nt] = (kk < HQ && col < SQ) ? ek[kk * SQ + col] : 0.0;
+                            b1[nt] = (kk < HQ && col < SQ) ? ek[kk * SQ + col] : T(0);
                         }
 #pragma unroll
                         for (int mt = 0; mt < MT; mt++)
@@ -750,11 +773,10 @@ namespace lbx
                             const int gr = rowbase + 16 * mt + j;
                             const int pr = gr / HP;
                             const int h = gr - pr * HP;
-                            const double a = (pr < bcnt) ? sR[0][pr - prim0][sTab[h][kk]] : 0.0;
+                            const T a = (pr < bcnt) ? sR[0][pr - prim0][sTab[h][kk]] : T(0);
 #pragma unroll
                             for (int nt = 0; nt < NQ; nt++)
-                                acc1[mt][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(
-                                        a, b1[nt], acc1[mt][nt], 0, 0, 0);
+                                acc1[mt][nt] = mfma16(a, b1[nt], acc1[mt][nt]);
                         }
                     }
                     __syncthreads();
@@ -768,27 +790,32 @@ namespace lbx
 #pragma unroll
                     for (int c = 0; c < 4; c++)
                     {
-                        // K-chunk c of X^T = accumulator element c = rows 16 mt + 4 c + (0..3)
-                        const int gr = rowbase + 16 * mt + 4 * c + g;
+                        // K-chunk c of X^T = accumulator element c of every lane: rows
+                        // 16 mt + 4 c + (0..3) in FP64 (element c holds row 4 c + g), rows
+                        // 16 mt + c + 4 (0..3) in FP32 (element c holds row 4 g + c)
+                        const int gr = rowbase + 16 * mt + (F32 ? 4 * g + c : 4 * c + g);
                         const int pr = gr / HP;
                         const int h = gr - pr * HP;
                         const bool rv = pr < bcnt;
-                        const double *eb = A.bra.eb + (static_cast<size_t>(bpoff + pr) * HP + h) * SP;
+                        const T *eb;
+                        if constexpr (F32)
+                            eb = A.bra.eb32 + (static_cast<size_t>(bpoff + pr) * HP + h) * SP;
+                        else
+                            eb = A.bra.eb + (static_cast<size_t>(bpoff + pr) * HP + h) * SP;
 #pragma unroll
                         for (int np = 0; np < NP; np++)
                         {
                             const int col = 16 * np + j;
-                            const double b2 = (rv && col < SP) ? eb[col] : 0.0;
+                            const T b2 = (rv && col < SP) ? eb[col] : T(0);
 #pragma unroll
                             for (int nq = 0; nq < NQ; nq++)
-                                acc2[nq][np] = __builtin_amdgcn_mfma_f64_16x16x4f64(
-                                        acc1[mt][nq][c], b2, acc2[nq][np], 0, 0, 0);
+                                acc2[nq][np] = mfma16(acc1[mt][nq][c], b2, acc2[nq][np]);
                         }
                     }
                 }
             }
-            // accumulator layout of v_mfma_f64_16x16x4f64: lane (g, j), element r holds
-            // D[4 r + g][j]; here D = I^T[16 nq + 4 r + g][16 np + j]
+            // accumulator element r of lane (g, j) holds D[4 r + g][j] (FP64) or D[4 g + r][j]
+            // (FP32); here D = I^T[16 nq + row][16 np + j]
 #pragma unroll
             for (int nq = 0; nq < NQ; nq++)
 #pragma unroll
@@ -796,9 +823,9 @@ namespace lbx
 #pragma unroll
                     for (int r = 0; r < 4; r++)
                     {
-                        const int kl = 16 * nq + 4 * r + g, mn = 16 * np + j;
+                        const int kl = 16 * nq + (F32 ? 4 * g + r : 4 * r + g), mn = 16 * np + j;
                         if (kl < SQ && mn < SP)
-                            sI[mn * SQ + kl] = acc2[nq][np][r];
+                            sI[mn * SQ + kl] = double(acc2[nq][np][r]);
                     }
             __syncthreads();
 
@@ -968,7 +995,17 @@ namespace lbx
         hipLaunchKernelGGL((quartetM<LA, LB, LC, LD, MODE>), dim3(grid), dim3(64), 0, s, A);
     }
 
-    // The FP32 one-lane forms (J/K only).
+    template <int LA, int LB, int LC, int LD>
+    void launchMfma32(const KArgs &A, hipStream_t s)
+    {
+        if (A.ntasks <= 0)
+            return;
+        int grid = static_cast<int>(A.ntasks < 16384 ? A.ntasks : 16384);
+        hipLaunchKernelGGL((quartetM<LA, LB, LC, LD, MODE_JK, float>), dim3(grid), dim3(64), 0, s, A);
+    }
+
+    // The FP32 forms: one lane per quartet (J/K and RI sums), one wave per quartet on the FP32
+    // MFMA (J/K).
     template <int LA, int LB, int LC, int LD, int MODE>
     void registerQuartet32()
     {
@@ -989,6 +1026,13 @@ namespace lbx
             s.width = 64;
             s.swap = true;
             s.nc = C::NC1f;
+        }
+        if constexpr (MODE == MODE_JK)
+        {
+            KernelSlot &s = kernelSlot(MODE, LA, LB, LC, LD, VAR_MFMA32);
+            s.fn = &launchMfma32<LA, LB, LC, LD>;
+            s.width = 1;
+            s.swap = false;
         }
     }
 

@@ -9,9 +9,9 @@ namespace lbx
     using Launcher = void (*)(const KArgs &, hipStream_t);
 
     // Kernel variants of one class: one lane per quartet with the bra pairs on the lanes,
-    // the same with the ket pairs on the lanes, or one wave per quartet on MFMA; and the two
-    // one-lane forms with R and both transforms in FP32 (J/K of the small-bound quartets of
-    // the mixed-precision build; Boys, the epilogue and the accumulation stay FP64).
+    // the same with the ket pairs on the lanes, or one wave per quartet on MFMA; and those
+    // three forms with R and both transforms in FP32 (J/K of the small-bound quartets of the
+    // mixed-precision build; Boys, the epilogue and the accumulation stay FP64).
     enum Variant
     {
         VAR_VALU_BRA = 0,
@@ -20,7 +20,8 @@ namespace lbx
         N_FP64_VARIANTS = 3,
         VAR_VALU_BRA32 = 3,
         VAR_VALU_KET32 = 4,
-        N_VARIANTS = 5
+        VAR_MFMA32 = 5,
+        N_VARIANTS = 6
     };
 
     struct KernelSlot

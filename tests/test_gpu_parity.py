@@ -191,7 +191,7 @@ def test_jk_every_variant(lbx, jk_case, variant):
     assert st["quartets"] == sr["quartets"]
 
 
-@pytest.mark.parametrize("variant", [3, 4])
+@pytest.mark.parametrize("variant", [3, 4, 5])
 def test_jk_fp32_every_variant(lbx, jk_case, variant):
     """Every FP32 form, on every quartet (mixed threshold above every bound).  FP32 R and
     transforms carry ~1e-7 relative rounding per term, so J and K agree with the FP64 oracle to
