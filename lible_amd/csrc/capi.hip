@@ -257,24 +257,53 @@ namespace lbx
         // meets the group, each q with the group's G blocks in turn: G n_g tasks (a task whose
         // block misses q's range has every lane idle and returns at once).  Only groups with
         // g % stride == 0 get tasks (the autotuner's samples); returns the quartets they hold.
-        double buildBlockMajor(Context &c, TaskTable &t, int nlane, int stride, int gshift)
+        // With a shell-ordered lane side (perm: lane position -> Schwarz-order position) a group
+        // holds a subset of the Schwarz-order positions [lowest, highest]: its q range is then
+        // conservative (the kernel tests each lane) and its quartets are counted exactly.
+        double buildBlockMajor(Context &c, TaskTable &t, int nlane, int stride, int gshift,
+                               const std::vector<int> *perm = nullptr)
         {
             const int nblk = (nlane + 63) / 64, gsz = 64 << gshift;
             const int ng = (nlane + gsz - 1) / gsz;
             std::vector<int> qmin(ng, std::numeric_limits<int>::max()), qmax(ng, -1);
             std::vector<double> quart(ng, 0.0);
-            for (int q = 0; q < t.nket; q++)
+            if (perm)
             {
-                const int lo = t.plo[q], hi = t.phi[q];
-                if (hi <= lo)
-                    continue;
-                for (int g = lo / gsz; g <= (hi - 1) / gsz; g++)
+                std::vector<std::vector<int>> pos(ng);
+                for (int p = 0; p < nlane; p++)
+                    pos[p / gsz].push_back((*perm)[p]);
+                for (auto &v : pos)
+                    std::sort(v.begin(), v.end());
+                for (int q = 0; q < t.nket; q++)
                 {
-                    qmin[g] = std::min(qmin[g], q);
-                    qmax[g] = q;
-                    quart[g] += std::min(hi, gsz * (g + 1)) - std::max(lo, gsz * g);
+                    const int lo = t.plo[q], hi = t.phi[q];
+                    if (hi <= lo)
+                        continue;
+                    for (int g = 0; g < ng; g++)
+                    {
+                        const auto &v = pos[g];
+                        if (hi <= v.front() || lo > v.back())
+                            continue;
+                        qmin[g] = std::min(qmin[g], q);
+                        qmax[g] = q;
+                        quart[g] += static_cast<double>(std::lower_bound(v.begin(), v.end(), hi) -
+                                                        std::lower_bound(v.begin(), v.end(), lo));
+                    }
                 }
             }
+            else
+                for (int q = 0; q < t.nket; q++)
+                {
+                    const int lo = t.plo[q], hi = t.phi[q];
+                    if (hi <= lo)
+                        continue;
+                    for (int g = lo / gsz; g <= (hi - 1) / gsz; g++)
+                    {
+                        qmin[g] = std::min(qmin[g], q);
+                        qmax[g] = q;
+                        quart[g] += std::min(hi, gsz * (g + 1)) - std::max(lo, gsz * g);
+                    }
+                }
             t.nblk = nblk;
             t.gshift = gshift;
             t.bprefix.assign(ng + 1, 0);
@@ -659,6 +688,55 @@ struct lbx_context : Context
         return v >= 0 ? v : variantFor(i, j);
     }
 
+    // Lane side of the one-lane J/K kernels, per pair class: Schwarz-order positions regrouped
+    // by shell within half-decade buckets of the Schwarz factor (lane position -> Schwarz-order
+    // position), so that lanes sharing the first shell are adjacent and combine their K updates,
+    // and the range of Schwarz-order positions each 64-lane block holds.
+    struct LanePerm
+    {
+        std::vector<int> perm, bqlo, bqhi;
+        int *d_perm = nullptr, *d_bqlo = nullptr, *d_bqhi = nullptr;
+    };
+    std::map<int, LanePerm> lane_perm;
+
+    const LanePerm &lanePerm(int idx)
+    {
+        auto it = lane_perm.find(idx);
+        if (it != lane_perm.end())
+            return it->second;
+        const PairClass &pc = cls[idx];
+        const int n = pc.n_pairs;
+        std::vector<int> bk(n), perm(n);
+        for (int p = 0; p < n; p++)
+        {
+            const double q = pc.Q.empty() ? 1.0 : pc.Q[pc.order[p]];
+            bk[p] = q > 0 ? static_cast<int>(std::floor(2.0 * std::log10(q))) : -1000000;
+            perm[p] = p;
+        }
+        std::stable_sort(perm.begin(), perm.end(), [&](int x, int y) {
+            if (bk[x] != bk[y])
+                return bk[x] > bk[y];
+            const int ax = pc.sh_a[pc.order[x]], ay = pc.sh_a[pc.order[y]];
+            if (ax != ay)
+                return ax < ay;
+            return pc.sh_b[pc.order[x]] < pc.sh_b[pc.order[y]];
+        });
+        LanePerm lp;
+        const int nblk = (n + 63) / 64;
+        lp.bqlo.assign(nblk, std::numeric_limits<int>::max());
+        lp.bqhi.assign(nblk, 0);
+        for (int p = 0; p < n; p++)
+        {
+            lp.bqlo[p >> 6] = std::min(lp.bqlo[p >> 6], perm[p]);
+            lp.bqhi[p >> 6] = std::max(lp.bqhi[p >> 6], perm[p] + 1);
+        }
+        lp.d_perm = upload(perm);
+        lp.d_bqlo = upload(lp.bqlo);
+        lp.d_bqhi = upload(lp.bqhi);
+        lp.perm = std::move(perm);
+        return lane_perm.emplace(idx, std::move(lp)).first->second;
+    }
+
     // Queues the J/K kernel of class pair (i >= j) with the given variant on the context stream,
     // over the quartets with lo <= Q_ab Q_cd < hi.
     const TaskTable &launchJK(int i, int j, int var, const double *D, double *J, double *K,
@@ -694,8 +772,9 @@ struct lbx_context : Context
             while (gshift < 6 && groupBytes * (2 << gshift) <= double(1 << 20))
                 gshift++;
         }
+        const LanePerm *lp = blockMajor && nlane > 0 ? &lanePerm(s.swap ? j : i) : nullptr;
         if (blockMajor && it->second.nblk == 0 && nlane > 0)
-            buildBlockMajor(*this, it->second, nlane, 1, gshift);
+            buildBlockMajor(*this, it->second, nlane, 1, gshift, &lp->perm);
         const TaskTable &t = it->second;
         KArgs a = baseArgs(*this, bra, ket, t, i == j);
         a.bra_major = braMajor ? 1 : 0;
@@ -707,6 +786,9 @@ struct lbx_context : Context
             a.block_major = 1;
             a.nblk = t.nblk;
             a.gshift = t.gshift;
+            a.lperm = lp->d_perm;
+            a.bqlo = lp->d_bqlo;
+            a.bqhi = lp->d_bqhi;
             a.bprefix = t.d_bprefix;
             a.bq0 = t.d_bq0;
             a.ntasks = t.btasks;
@@ -723,7 +805,7 @@ struct lbx_context : Context
                 ts.nket = t.nket;
                 ts.plo = t.plo;
                 ts.phi = t.phi;
-                ts.quartets = buildBlockMajor(*this, ts, nlane, stride, t.gshift);
+                ts.quartets = buildBlockMajor(*this, ts, nlane, stride, t.gshift, &lp->perm);
                 st = tune_tasks.emplace(key, std::move(ts)).first;
             }
             const TaskTable &ts = st->second;

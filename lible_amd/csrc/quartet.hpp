@@ -57,6 +57,10 @@ namespace lbx
         int block_major, nblk, gshift;
         const long long *bprefix;
         const int *bq0;
+        // lane-block-major J/K: lane position -> Schwarz-order position (the lane side ordered by
+        // shell within Schwarz-factor buckets, so lanes sharing a shell are adjacent), and per
+        // lane block the range [bqlo, bqhi) of Schwarz-order positions it holds; null otherwise
+        const int *lperm, *bqlo, *bqhi;
         const double *boys;
         // MODE_JK
         const double *D;
@@ -155,6 +159,63 @@ namespace lbx
         }
     }
 
+    // Runs of adjacent lanes with equal key (a shell of the lane-side pair): per-lane masks of a
+    // segmented suffix sum (step i adds the lane 2^i above when it is in the same run), the run
+    // head, and whether any lane of the run holds a quartet.
+    struct LaneRuns
+    {
+        unsigned ok;    // bit i: lane + 2^i is in this lane's run
+        bool head;      // first lane of its run
+        bool any;       // (head lanes) some lane of the run is active
+    };
+
+    __device__ __forceinline__ LaneRuns laneRuns(int key, bool active, int lane)
+    {
+        LaneRuns r;
+        const int prev = __shfl_up(key, 1, 64);
+        r.head = lane == 0 || prev != key;
+        // run id = lane index of the run head (prefix max of head ? lane : 0)
+        int id = r.head ? lane : 0;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1)
+        {
+            const int x = __shfl_up(id, o, 64);
+            if (lane >= o)
+                id = max(id, x);
+        }
+        r.ok = 0;
+        int act = active ? 1 : 0;
+#pragma unroll
+        for (int i = 0; i < 6; i++)
+        {
+            const int o = 1 << i;
+            const int xid = __shfl_down(id, o, 64);
+            const int xa = __shfl_down(act, o, 64);
+            const bool same = lane + o < 64 && xid == id;
+            if (same)
+            {
+                r.ok |= 1u << i;
+                act |= xa;
+            }
+        }
+        r.any = act != 0;
+        return r;
+    }
+
+    // Adds v of every lane of a run into one address, by the run's head lane (all lanes call).
+    __device__ __forceinline__ void runAtomicAdd(const LaneRuns &r, double *addr, double v)
+    {
+#pragma unroll
+        for (int i = 0; i < 6; i++)
+        {
+            const double x = __shfl_down(v, 1 << i, 64);
+            if (r.ok & (1u << i))
+                v += x;
+        }
+        if (r.head && r.any)
+            unsafeAtomicAdd(addr, v);
+    }
+
     // ------------------------------------------------------------------------------- quartetV
     // One quartet per lane: lane-side pair l (64 consecutive sorted positions) against the
     // wave-uniform pair u.  SWAP = false: lanes = bra, u = ket; SWAP = true: lanes = ket, u = bra.
@@ -187,14 +248,22 @@ namespace lbx
         const DPairs &LP = SWAP ? A.ket : A.bra;
         const DPairs &UP = SWAP ? A.bra : A.ket;
 
-        // lanes cover sorted positions pbase .. pbase + 63 (pbase = `local`)
+        // lanes cover lane positions pbase .. pbase + 63 (pbase = `local`); p is the lane's
+        // Schwarz-order position (through A.lperm when the lane side is shell-ordered)
         const int plo = uniform(A.plo[q]), phi = uniform(A.phi[q]);
-        if (plo >= local + 64 || phi <= local)
+        const bool permuted = A.lperm != nullptr;
+        // the last group of a lane-block-major table runs G blocks, some past the lane side
+        if (local >= LP.n)
+            return;
+        if (permuted ? (plo >= A.bqhi[local >> 6] || phi <= A.bqlo[local >> 6])
+                     : (plo >= local + 64 || phi <= local))
             return;   // no lane has a quartet
-        const int p = local + lane;
-        const bool active = p >= plo && p < phi;
+        const int pl = local + lane;
+        const bool inside = !permuted || pl < LP.n;
+        const int p = permuted ? A.lperm[inside ? pl : local] : pl;
+        const bool active = inside && p >= plo && p < phi;
         const int u = uniform(UP.order[q]);
-        const int l = LP.order[active ? p : local];
+        const int l = LP.order[active || permuted ? p : local];
         const int upoff = uniform(UP.poff[u]), ucnt = uniform(UP.pcnt[u]);
         const int lpoff = LP.poff[l], lcnt = active ? LP.pcnt[l] : 0;
         // the ket's E carries the sign (-1)^{t'+u'+v'}
@@ -229,6 +298,23 @@ namespace lbx
         }
         const long long n = A.nbf;
         const double *D = A.D;
+        // K updates on the columns of the lane pair's first shell: with a shell-ordered lane side
+        // the lanes of one shell add their sum once (runs of adjacent lanes), else each lane
+        // adds its own
+        LaneRuns runs{};
+        int la0 = l0;
+        if constexpr (MODE == MODE_JK)
+            if (permuted)
+            {
+                la0 = LP.ao_a[l];
+                runs = laneRuns(la0, active, lane);
+            }
+        auto addA = [&](long long row, int al, double v) {
+            if (permuted)
+                runAtomicAdd(runs, &A.K[row * n + la0 + al], v);
+            else if (active)
+                unsafeAtomicAdd(&A.K[row * n + l0 + al], v);
+        };
         if constexpr (MODE == MODE_JK && !CARRY)
 #pragma unroll
             for (int s = 0; s < SL; s++)
@@ -334,7 +420,6 @@ namespace lbx
                     if (lane == 0)
                         unsafeAtomicAdd(&A.J[(u0 + ga) * n + u1 + de], ju);
                 }
-                if (active)
                 {
                     // K[u0+ga][l0+al] += f sum_{be, de} I D[u1+de][l1+be]
                     // K[u0+ga][l1+be] += f sum_{al, de} I D[u1+de][l0+al]
@@ -369,7 +454,7 @@ namespace lbx
                                     s = fma(double(I[(al * SLb + be) * NC + su % NC]),
                                             D[(u1 + de) * n + l1 + be], s);
                                 }
-                            unsafeAtomicAdd(&A.K[(u0 + ga) * n + l0 + al], f * s);
+                            addA(u0 + ga, al, f * s);
                         }
 #pragma unroll
                         for (int be = 0; be < SLb; be++)
@@ -386,7 +471,8 @@ namespace lbx
                                     s = fma(double(I[(al * SLb + be) * NC + su % NC]),
                                             D[(u1 + de) * n + l0 + al], s);
                                 }
-                            unsafeAtomicAdd(&A.K[(u0 + ga) * n + l1 + be], f * s);
+                            if (active)
+                                unsafeAtomicAdd(&A.K[(u0 + ga) * n + l1 + be], f * s);
                         }
                     }
 #pragma unroll
@@ -413,7 +499,7 @@ namespace lbx
                                     s = fma(double(I[(al * SLb + be) * NC + su % NC]),
                                             D[(u0 + ga) * n + l1 + be], s);
                                 }
-                            unsafeAtomicAdd(&A.K[(u1 + de) * n + l0 + al], f * s);
+                            addA(u1 + de, al, f * s);
                         }
 #pragma unroll
                         for (int be = 0; be < SLb; be++)
@@ -430,7 +516,8 @@ namespace lbx
                                     s = fma(double(I[(al * SLb + be) * NC + su % NC]),
                                             D[(u0 + ga) * n + l0 + al], s);
                                 }
-                            unsafeAtomicAdd(&A.K[(u1 + de) * n + l1 + be], f * s);
+                            if (active)
+                                unsafeAtomicAdd(&A.K[(u1 + de) * n + l1 + be], f * s);
                         }
                     }
                 }
@@ -545,7 +632,7 @@ namespace lbx
                     const int p = 64 * b + lane;
                     if (p < LP.n)
                     {
-                        const int l = LP.order[p];
+                        const int l = LP.order[A.lperm ? A.lperm[p] : p];
                         const long long r0 = LP.ao_a[l], r1 = LP.ao_b[l];
 #pragma unroll
                         for (int s = 0; s < SL; s++)
