@@ -13,7 +13,7 @@ for quartets with Schwarz bound < 1e-5) are timed and compared with the FP64 res
 the batched RI integrals of 256 molecules (value = (mn|P) triplets/s); --config c5_gradient
 the nuclear gradient of the 60-atom workload (value = quartets/s of the gradient).
 
-Prints one JSON line.  --gpus N > 1 exits: this build runs on one GPU.
+Prints one JSON line.  --gpus N > 1 exits: the benchmark runs on one GPU.
 """
 from __future__ import annotations
 
@@ -260,8 +260,9 @@ def main(argv=None):
                          "rocprofv3's FETCH_SIZE / WRITE_SIZE counters")
     a = ap.parse_args(argv)
     if a.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
-              "(the multi-GPU design is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: the J/K build runs on one GPU; --gpus {a.gpus} is not supported "
+              "(J and K would shard by bra-pair blocks with one sum all-reduce, not implemented)",
+              file=sys.stderr)
         return 2
 
     import numpy as np
