@@ -370,6 +370,27 @@ def test_jk_tuned(lbx, jk_case):
     close(K.cpu().numpy(), Kr, rel=0.0, floor=1e-5)
 
 
+def test_jk_medium(lbx, oracle):
+    """10 atoms, 300 functions: lane sides of hundreds to thousands of pairs, so the one-lane
+    kernels run many lane blocks in several groups with the lane side ordered by shell inside
+    Schwarz buckets (long runs of shared shells, bucket boundaries inside blocks); default and
+    tuned variant choices against the oracle."""
+    import torch
+    m = B.synthetic_molecule(2031, 10, 8.0, B.RECIPE_MAIN)
+    D = B.density_from_orbitals(9, m.shells.nbf, 30, 0.1)
+    Jr, Kr, sr = oracle.Structure(m).jk(D, thresh=1e-12)
+    ctx = lbx.Context(m)
+    Dt = torch.tensor(D, device="cuda")
+    J, K, st = ctx.jk(Dt, schwarz=1e-12)
+    assert st["quartets"] == sr["quartets"]
+    close(J.cpu().numpy(), Jr)
+    close(K.cpu().numpy(), Kr)
+    ctx.tune(Dt, schwarz=1e-12)
+    J, K, st = ctx.jk(Dt, schwarz=1e-12)
+    close(J.cpu().numpy(), Jr)
+    close(K.cpu().numpy(), Kr)
+
+
 @pytest.mark.parametrize("thresh", [0.0, 1e-12])
 def test_jk_small(lbx, oracle, thresh):
     import torch
