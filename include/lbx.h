@@ -138,18 +138,20 @@ int lbx_ri_sums(lbx_context *ctx, double *out, int precision, double mixed_thres
 int lbx_jk_grad(lbx_context *ctx, const double *D, double *grad, double schwarz_thresh,
                 int precision, double mixed_thresh, lbx_jk_stats *stats);
 
-/* Autotuner (no reference counterpart).  Each class pair (la lb|lc ld) has up to six kernel
+/* Autotuner (no reference counterpart).  Each class pair (la lb|lc ld) has up to eight kernel
  * variants: 0 one lane per quartet with the bra pairs on the lanes, 1 the same with the ket
- * pairs on the lanes, 2 one wave per quartet on FP64 MFMA, and 3, 4, 5 the forms 0, 1, 2 with R
- * and both transforms in FP32 (5 on the FP32 MFMA; the small-bound quartets of LBX_MIXED).  lbx_tune times every
- * variant of every class pair on this density and threshold (on a sample of ~2M quartets of
- * each class pair: every k-th pair of one side with all its partners) and keeps the fastest per
- * quartet: the fastest FP64 form
- * for FP64 work and the fastest of all forms for the FP32 tier of mixed builds; without it
- * lbx_jk picks by a work estimate from the primitive-pair counts. */
+ * pairs on the lanes, 2 one wave per quartet on FP64 MFMA, 6 one workgroup per bra pair and
+ * block of ket pairs (ket transform on the VALU, bra transform on FP64 MFMA with the ket pairs
+ * along N), and 3, 4, 5, 7 the forms 0, 1, 2, 6 with R and both transforms in FP32 (5 and 7 on
+ * the FP32 MFMA; the small-bound quartets of LBX_MIXED).  lbx_tune times every variant of every
+ * class pair on this density and threshold (on a sample of ~2M quartets of each class pair:
+ * every k-th pair of one side with all its partners) and keeps the fastest per quartet: the
+ * fastest FP64 form for FP64 work and the fastest of all forms for the FP32 tier of mixed
+ * builds; without it lbx_jk picks by a work estimate from the primitive-pair counts. */
 int lbx_tune(lbx_context *ctx, const double *D, double schwarz_thresh);
 /* Fixes the variant of class pair (la lb|lc ld), la >= lb, lc >= ld, bra class >= ket class in
- * the order (0,0),(1,0),(1,1),(2,0),...: variants 0-2 for FP64 work, 3-5 for the FP32 tier;
+ * the order (0,0),(1,0),(1,1),(2,0),...: variants 0-2 and 6 for FP64 work, 3-5 and 7 for the
+ * FP32 tier;
  * fails if the class pair has no such variant.  variant -1 restores both defaults. */
 int lbx_set_variant(lbx_context *ctx, int la, int lb, int lc, int ld, int variant);
 
@@ -159,7 +161,8 @@ int lbx_set_variant(lbx_context *ctx, int la, int lb, int lc, int ld, int varian
  * lbx_profile_read returns the accumulated totals since the last enable. */
 typedef struct lbx_class_profile {
     int la, lb, lc, ld;
-    int kernel;            /* variant: 0 bra on lanes, 1 ket on lanes, 2 MFMA, 3 / 4 / 5 FP32 0 / 1 / 2 */
+    int kernel;            /* variant: 0 bra on lanes, 1 ket on lanes, 2 MFMA, 6 block;
+                              3 / 4 / 5 / 7 their FP32 forms */
     int calls;
     double ms;             /* summed kernel time */
     double quartets;       /* summed quartets */
@@ -167,6 +170,13 @@ typedef struct lbx_class_profile {
 } lbx_class_profile;
 int lbx_profile_enable(lbx_context *ctx, int on);
 int lbx_profile_read(lbx_context *ctx, lbx_class_profile *out, int capacity, int *count);
+
+/* Restricts the following lbx_jk calls on this context to one class pair (la lb|lc ld)
+ * (canonical: la >= lb, lc >= ld, bra class >= ket class); all four -1 lifts the restriction.
+ * No reference counterpart: the per-(la,lb,lc,ld) loop of eri4Benchmark (eri4.cpp:97-150) is
+ * the model.  J and K then hold that class pair's contribution only (symmetrised), which gives
+ * per-class-pair error and timing reports. */
+int lbx_set_class_filter(lbx_context *ctx, int la, int lb, int lc, int ld);
 
 #ifdef __cplusplus
 }

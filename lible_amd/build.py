@@ -25,6 +25,7 @@ SELFTEST = os.path.join(HERE, "bin", "ints_selftest")
 
 N_UNITS = 24   # csrc/instances.hip: 0..11 FP64 forms and dense outputs, 12..23 FP32 forms
 N_GRAD_UNITS = 10   # csrc/grad_instances.hip: one per lane-side main pair class
+N_BLOCK_UNITS = 20  # csrc/block_instances.hip: 0..9 FP64 block J/K forms per bra class, 10..19 FP32
 
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950", "-mcode-object-version=5",
@@ -106,6 +107,9 @@ def build(jobs: int | None = None, verbose: bool = True, incremental: bool = Fal
     for u in range(N_UNITS):
         work.append((os.path.join(CSRC, "instances.hip"), os.path.join(BUILD, f"inst_{u}.o"),
                      [f"-DLBX_UNIT={u}"]))
+    for u in range(N_BLOCK_UNITS):
+        work.append((os.path.join(CSRC, "block_instances.hip"), os.path.join(BUILD, f"block_{u}.o"),
+                     [f"-DLBX_BUNIT={u}"]))
     for u in range(N_GRAD_UNITS):
         work.append((os.path.join(CSRC, "grad_instances.hip"), os.path.join(BUILD, f"grad_{u}.o"),
                      [f"-DLBX_GUNIT={u}"]))

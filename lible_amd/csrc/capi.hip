@@ -3,6 +3,7 @@
 #include "grad.hpp"
 #include "quartet.hpp"
 #include "registry.hpp"
+#include "tasks.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -140,6 +141,7 @@ namespace lbx
             int nket = 0;
             std::vector<long long> prefix;
             std::vector<int> plo, phi;
+            tasks::Cuts cuts;   // the Schwarz cuts behind plo/phi (buildTasks)
             long long *d_prefix = nullptr;
             int *d_plo = nullptr, *d_phi = nullptr;
             double quartets = 0, flops = 0;
@@ -161,20 +163,7 @@ namespace lbx
             Diagonal    // p == q only
         };
 
-        // first sorted position p with qb[p] * qk < thr (qb descending)
-        int firstBelow(const std::vector<double> &qb, double qk, double thr)
-        {
-            int a = 0, b = static_cast<int>(qb.size());
-            while (a < b)
-            {
-                const int m = (a + b) / 2;
-                if (qb[m] * qk >= thr)
-                    a = m + 1;
-                else
-                    b = m;
-            }
-            return a;
-        }
+        using tasks::firstBelow;
 
         constexpr double kNoUpper = std::numeric_limits<double>::infinity();
         // quartets per class pair and variant that the autotuner times
@@ -204,6 +193,13 @@ namespace lbx
             const double Hab = nHerm(la + lb), Hcd = nHerm(lc + ld);
             const double Sab = nSph(la) * nSph(lb), Scd = nSph(lc) * nSph(ld);
             const double perKK = quartetFlops(la, lb, lc, ld, 1, 1) - 2 * Sab * Hab * Scd - 12 * Sab * Scd;
+            if (mode != Pairing::Diagonal)
+            {
+                std::vector<double> qk(t.nket);
+                for (int q = 0; q < t.nket; q++)
+                    qk[q] = ket.Q.empty() ? 1.0 : ket.Q[ket.order[q]];
+                t.cuts = tasks::cut(qb, qk, mode == Pairing::Unique && same, thresh, thresh_hi);
+            }
             long long acc = 0;
             for (int q = 0; q < t.nket; q++)
             {
@@ -215,14 +211,8 @@ namespace lbx
                 }
                 else
                 {
-                    lo = (mode == Pairing::Unique && same) ? q : 0;
-                    hi = bra.n_pairs;
-                    const double qk = ket.Q.empty() ? 1.0 : ket.Q[ket.order[q]];
-                    if (thresh > 0)
-                        hi = firstBelow(qb, qk, thresh);
-                    if (thresh_hi < kNoUpper)
-                        lo = std::max(lo, firstBelow(qb, qk, thresh_hi));
-                    hi = std::max(hi, lo);
+                    lo = t.cuts.plo[q];
+                    hi = t.cuts.phi[q];
                 }
                 t.plo[q] = lo;
                 t.phi[q] = hi;
@@ -261,70 +251,28 @@ namespace lbx
         // holds a subset of the Schwarz-order positions [lowest, highest]: its q range is then
         // conservative (the kernel tests each lane) and its quartets are counted exactly.
         double buildBlockMajor(Context &c, TaskTable &t, int nlane, int stride, int gshift,
-                               const std::vector<int> *perm = nullptr)
+                               const std::vector<int> &perm, const std::vector<int> &bqlo,
+                               const std::vector<int> &bqhi)
         {
-            const int nblk = (nlane + 63) / 64, gsz = 64 << gshift;
-            const int ng = (nlane + gsz - 1) / gsz;
-            std::vector<int> qmin(ng, std::numeric_limits<int>::max()), qmax(ng, -1);
-            std::vector<double> quart(ng, 0.0);
-            if (perm)
+            if (t.cuts.plo.size() != static_cast<size_t>(t.nket))
+                throw Error(LBX_ERR_STATE, "lane-block-major table without Schwarz cuts");
+            tasks::BlockMajor b = tasks::blockMajor(t.cuts, nlane, stride, gshift, perm);
+            try
             {
-                std::vector<std::vector<int>> pos(ng);
-                for (int p = 0; p < nlane; p++)
-                    pos[p / gsz].push_back((*perm)[p]);
-                for (auto &v : pos)
-                    std::sort(v.begin(), v.end());
-                for (int q = 0; q < t.nket; q++)
-                {
-                    const int lo = t.plo[q], hi = t.phi[q];
-                    if (hi <= lo)
-                        continue;
-                    for (int g = 0; g < ng; g++)
-                    {
-                        const auto &v = pos[g];
-                        if (hi <= v.front() || lo > v.back())
-                            continue;
-                        qmin[g] = std::min(qmin[g], q);
-                        qmax[g] = q;
-                        quart[g] += static_cast<double>(std::lower_bound(v.begin(), v.end(), hi) -
-                                                        std::lower_bound(v.begin(), v.end(), lo));
-                    }
-                }
+                tasks::checkBlockMajor(b, t.nket, nlane, gshift, perm, bqlo, bqhi);
             }
-            else
-                for (int q = 0; q < t.nket; q++)
-                {
-                    const int lo = t.plo[q], hi = t.phi[q];
-                    if (hi <= lo)
-                        continue;
-                    for (int g = lo / gsz; g <= (hi - 1) / gsz; g++)
-                    {
-                        qmin[g] = std::min(qmin[g], q);
-                        qmax[g] = q;
-                        quart[g] += std::min(hi, gsz * (g + 1)) - std::max(lo, gsz * g);
-                    }
-                }
-            t.nblk = nblk;
+            catch (const std::runtime_error &e)
+            {
+                throw Error(LBX_ERR_STATE, e.what());
+            }
+            t.nblk = b.nblk;
             t.gshift = gshift;
-            t.bprefix.assign(ng + 1, 0);
-            t.bq0.assign(ng, 0);
-            long long acc = 0;
-            double quartets = 0;
-            for (int g = 0; g < ng; g++)
-            {
-                t.bprefix[g] = acc;
-                if (qmax[g] >= 0 && g % stride == 0)
-                {
-                    t.bq0[g] = qmin[g];
-                    acc += static_cast<long long>(qmax[g] - qmin[g] + 1) << gshift;
-                    quartets += quart[g];
-                }
-            }
-            t.bprefix[ng] = acc;
-            t.btasks = acc;
+            t.bprefix = std::move(b.bprefix);
+            t.bq0 = std::move(b.bq0);
+            t.btasks = b.ntasks;
             t.d_bprefix = c.upload(t.bprefix);
             t.d_bq0 = c.upload(t.bq0);
-            return quartets;
+            return b.quartets;
         }
 
         // Tasks of the batched RI sums: for every uniform-side position q the lane-side pairs of
@@ -670,6 +618,8 @@ struct lbx_context : Context
     // kernel variant per class pair chosen by lbx_tune (absent: work-estimate default), for
     // the FP64 build and for the FP32 tier of the mixed build
     std::map<std::pair<int, int>, int> tuned, tuned32;
+    // lbx_set_class_filter: the one class pair (bra class, ket class) lbx_jk evaluates; -1: all
+    int filter_i = -1, filter_j = -1;
 
     int variantFor(int i, int j)
     {
@@ -706,30 +656,19 @@ struct lbx_context : Context
             return it->second;
         const PairClass &pc = cls[idx];
         const int n = pc.n_pairs;
-        std::vector<int> bk(n), perm(n);
+        std::vector<double> qs(n);
+        std::vector<int> sa(n), sb(n);
         for (int p = 0; p < n; p++)
         {
-            const double q = pc.Q.empty() ? 1.0 : pc.Q[pc.order[p]];
-            bk[p] = q > 0 ? static_cast<int>(std::floor(2.0 * std::log10(q))) : -1000000;
-            perm[p] = p;
+            qs[p] = pc.Q.empty() ? 1.0 : pc.Q[pc.order[p]];
+            sa[p] = pc.sh_a[pc.order[p]];
+            sb[p] = pc.sh_b[pc.order[p]];
         }
-        std::stable_sort(perm.begin(), perm.end(), [&](int x, int y) {
-            if (bk[x] != bk[y])
-                return bk[x] > bk[y];
-            const int ax = pc.sh_a[pc.order[x]], ay = pc.sh_a[pc.order[y]];
-            if (ax != ay)
-                return ax < ay;
-            return pc.sh_b[pc.order[x]] < pc.sh_b[pc.order[y]];
-        });
+        tasks::LaneOrder lo = tasks::laneOrder(qs, sa, sb);
         LanePerm lp;
-        const int nblk = (n + 63) / 64;
-        lp.bqlo.assign(nblk, std::numeric_limits<int>::max());
-        lp.bqhi.assign(nblk, 0);
-        for (int p = 0; p < n; p++)
-        {
-            lp.bqlo[p >> 6] = std::min(lp.bqlo[p >> 6], perm[p]);
-            lp.bqhi[p >> 6] = std::max(lp.bqhi[p >> 6], perm[p] + 1);
-        }
+        std::vector<int> perm = std::move(lo.perm);
+        lp.bqlo = std::move(lo.bqlo);
+        lp.bqhi = std::move(lo.bqhi);
         lp.d_perm = upload(perm);
         lp.d_bqlo = upload(lp.bqlo);
         lp.d_bqhi = upload(lp.bqhi);
@@ -748,33 +687,35 @@ struct lbx_context : Context
         // E matrices are larger, so those stay in cache across the tasks
         auto eBytes = [](const PairClass &c)
         { return c.n_pairs ? double(c.n_ppairs) / c.n_pairs * nHerm(c.la + c.lb) * nSph(c.la) * nSph(c.lb) : 0.0; };
-        const bool braMajor = !s.swap && s.width == 1 && i != j && eBytes(bra) > eBytes(ket);
+        const bool braMajor = !s.swap && s.kind == KIND_WAVE && i != j && eBytes(bra) > eBytes(ket);
+        // the block kernel's tasks: per bra position, blocks of s.width consecutive ket positions
+        const bool blockKind = s.kind == KIND_BLOCK;
         const auto key = std::make_tuple(i, j, thresh, thresh_hi, var);
         auto it = jk_tasks.find(key);
         if (it == jk_tasks.end())
             it = jk_tasks
-                         .emplace(key, (s.swap || braMajor)
-                                               ? buildTasks(*this, ket, bra, Pairing::Unique, false, thresh,
-                                                            s.width, false, thresh_hi)
+                         .emplace(key, (s.swap || braMajor || blockKind)
+                                               ? buildTasks(*this, ket, bra, Pairing::Unique, blockKind && i == j,
+                                                            thresh, s.width, false, thresh_hi)
                                                : buildTasks(*this, bra, ket, Pairing::Unique, i == j, thresh,
                                                             s.width, true, thresh_hi))
                          .first;
         // The one-lane-per-quartet kernels run their tasks by groups of lane blocks: as many
         // blocks (a power of two, up to 64) as keep the lane side's E matrices of a group within
         // about 1 MiB of L2, so each uniform pair's E is read once per group.
-        const bool blockMajor = s.width == 64;
+        const bool blockMajor = s.kind == KIND_LANE;
         const PairClass &lside = s.swap ? ket : bra;
         const int nlane = lside.n_pairs;
         int gshift = 0;
         if (blockMajor)
         {
-            const double groupBytes = 64.0 * eBytes(lside) * (var >= 3 ? 4 : 8);
+            const double groupBytes = 64.0 * eBytes(lside) * (isF32Variant(var) ? 4 : 8);
             while (gshift < 6 && groupBytes * (2 << gshift) <= double(1 << 20))
                 gshift++;
         }
         const LanePerm *lp = blockMajor && nlane > 0 ? &lanePerm(s.swap ? j : i) : nullptr;
         if (blockMajor && it->second.nblk == 0 && nlane > 0)
-            buildBlockMajor(*this, it->second, nlane, 1, gshift, &lp->perm);
+            buildBlockMajor(*this, it->second, nlane, 1, gshift, lp->perm, lp->bqlo, lp->bqhi);
         const TaskTable &t = it->second;
         KArgs a = baseArgs(*this, bra, ket, t, i == j);
         a.bra_major = braMajor ? 1 : 0;
@@ -805,7 +746,8 @@ struct lbx_context : Context
                 ts.nket = t.nket;
                 ts.plo = t.plo;
                 ts.phi = t.phi;
-                ts.quartets = buildBlockMajor(*this, ts, nlane, stride, t.gshift, &lp->perm);
+                ts.cuts = t.cuts;
+                ts.quartets = buildBlockMajor(*this, ts, nlane, stride, t.gshift, lp->perm, lp->bqlo, lp->bqhi);
                 st = tune_tasks.emplace(key, std::move(ts)).first;
             }
             const TaskTable &ts = st->second;
@@ -1511,7 +1453,7 @@ extern "C"
                                const TaskTable &t = ctx->launchJK(i, j, var, D, J, K, lo, hi);
                                nq += t.quartets;
                                fl += t.flops;
-                               if (var >= N_FP64_VARIANTS)
+                               if (isF32Variant(var))
                                    nq32 += t.quartets;
                                if (pr)
                                {
@@ -1526,6 +1468,8 @@ extern "C"
                                for (int j = 0; j <= i; j++)
                                {
                                    if (c.cls[i].n_pairs == 0 || c.cls[j].n_pairs == 0)
+                                       continue;
+                                   if (ctx->filter_i >= 0 && (i != ctx->filter_i || j != ctx->filter_j))
                                        continue;
                                    launch(i, j, 0, ctx->variantFor(i, j), split, kNoUpper);
                                    if (mixed)
@@ -1706,7 +1650,7 @@ extern "C"
                                        {
                                            if (!kernelSlot(MODE_JK, bra.la, bra.lb, ket.la, ket.lb, v).fn)
                                                continue;
-                                           if (v >= N_FP64_VARIANTS)
+                                           if (isF32Variant(v))
                                                ensureF32(c);
                                            // first launch builds the task table and loads the code;
                                            // the timed one runs a sample of leading quartets
@@ -1723,7 +1667,7 @@ extern "C"
                                            LBX_HIP(hipEventElapsedTime(&ev, e0, e1));
                                            // time per quartet: the variants' samples differ
                                            const double ms = ev / std::max(ctx->sampled, 1.0);
-                                           if (v < N_FP64_VARIANTS && (best < 0 || ms < best_ms))
+                                           if (!isF32Variant(v) && (best < 0 || ms < best_ms))
                                            {
                                                best = v;
                                                best_ms = ms;
@@ -1771,10 +1715,30 @@ extern "C"
                            }
                            if (variant >= N_VARIANTS || !kernelSlot(MODE_JK, la, lb, lc, ld, variant).fn)
                                throw Error(LBX_ERR_INVALID, "lbx_set_variant: class pair has no such variant");
-                           if (variant >= N_FP64_VARIANTS)
+                           if (isF32Variant(variant))
                                ctx->tuned32[key] = variant;
                            else
                                ctx->tuned[key] = variant;
+                       });
+    }
+
+    int lbx_set_class_filter(lbx_context *ctx, int la, int lb, int lc, int ld)
+    {
+        return guarded([&]
+                       {
+                           if (!ctx)
+                               throw Error(LBX_ERR_INVALID, "lbx_set_class_filter: null context");
+                           std::lock_guard<std::mutex> lk(ctx->mu);
+                           if (la == -1 && lb == -1 && lc == -1 && ld == -1)
+                           {
+                               ctx->filter_i = ctx->filter_j = -1;
+                               return;
+                           }
+                           if (la < lb || lc < ld || la > kMaxLMain || lc > kMaxLMain || lb < 0 || ld < 0 ||
+                               pairClassIdx(la, lb) < pairClassIdx(lc, ld))
+                               throw Error(LBX_ERR_INVALID, "lbx_set_class_filter: not a canonical class pair");
+                           ctx->filter_i = pairClassIdx(la, lb);
+                           ctx->filter_j = pairClassIdx(lc, ld);
                        });
     }
 

@@ -1120,6 +1120,7 @@ namespace lbx
             s.fn = &launchMfma32<LA, LB, LC, LD>;
             s.width = 1;
             s.swap = false;
+            s.kind = KIND_WAVE;
         }
     }
 
@@ -1156,5 +1157,6 @@ namespace lbx
         s.fn = &launchMfma<LA, LB, LC, LD, MODE>;
         s.width = 1;
         s.swap = false;
+        s.kind = KIND_WAVE;
     }
 }

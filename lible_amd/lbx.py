@@ -22,8 +22,9 @@ SYMBOLS = ["lbx_create", "lbx_destroy", "lbx_last_error", "lbx_dims", "lbx_set_s
            "lbx_synchronize", "lbx_boys", "lbx_eri4", "lbx_eri4_diag", "lbx_eri3", "lbx_eri2",
            "lbx_eri2_diag", "lbx_eri4d1", "lbx_eri3d1", "lbx_eri2d1", "lbx_jk", "lbx_create_batch", "lbx_ri_sums", "lbx_jk_grad",
            "lbx_tune", "lbx_set_variant", "lbx_profile_enable",
-           "lbx_profile_read"]
-VARIANTS = ("valu-bra", "valu-ket", "mfma", "valu-bra-f32", "valu-ket-f32", "mfma-f32")
+           "lbx_profile_read", "lbx_set_class_filter"]
+VARIANTS = ("valu-bra", "valu-ket", "mfma", "valu-bra-f32", "valu-ket-f32", "mfma-f32", "block",
+            "block-f32")
 
 
 class LbxError(RuntimeError):
@@ -86,6 +87,7 @@ def lib():
         L.lbx_ri_sums.argtypes = [p, p, i, d, ctypes.POINTER(RIStats)]
         L.lbx_tune.argtypes = [p, p, d]
         L.lbx_set_variant.argtypes = [p, i, i, i, i, i]
+        L.lbx_set_class_filter.argtypes = [p, i, i, i, i]
         L.lbx_profile_enable.argtypes = [p, i]
         L.lbx_profile_read.argtypes = [p, ctypes.POINTER(ClassProfile), i, ctypes.POINTER(i)]
         _LIB = L
@@ -271,6 +273,10 @@ class Context:
         torch = _torch()
         D = D.to(device=self.device, dtype=torch.float64).contiguous()
         _check(lib().lbx_tune(self.h, D.data_ptr(), schwarz))
+
+    def set_class_filter(self, cls: tuple | None):
+        """Restricts the following jk() calls to one class pair (la, lb, lc, ld); None: all."""
+        _check(lib().lbx_set_class_filter(self.h, *(cls if cls is not None else (-1, -1, -1, -1))))
 
     def set_variant(self, cls: tuple, variant: int) -> bool:
         """Fix the kernel variant of a class pair; False if it has no such variant."""

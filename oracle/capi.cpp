@@ -218,6 +218,12 @@ extern "C"
                      });
     }
 
+    // Sums |eri3|, eri3, |eri2|, eri2 over the dense tensors without forming them.
+    int orc_ri_sums(void *h, double *out)
+    {
+        return guard([&] { riSums(*static_cast<Structure *>(h), out); });
+    }
+
     int orc_eri2(void *h, double *out)
     {
         return guard([&]

@@ -554,4 +554,64 @@ namespace orc
         }
         return out;
     }
+
+    // The checksums of eri3() and eri2() without the dense tensors: a block of shell pair (ab)
+    // with a != b appears twice in the dense eri3 tensor ((ab|P) and (ba|P)), a block (P|Q) of
+    // distinct shells twice in eri2 ((P|Q) and (Q|P)); diagonal blocks once.
+    void riSums(const Structure &s, double out[4])
+    {
+        auto shs = shellDataAux(s);
+        auto sp = shellPairData(true, s);
+        double a3 = 0, s3 = 0, a2 = 0, s2 = 0;
+        for (const auto &A : sp)
+            for (const auto &C : shs)
+            {
+                ERI3Kernel kern(A, C);
+#pragma omp parallel for schedule(dynamic) reduction(+ : a3, s3)
+                for (size_t p = 0; p < A.n_pairs; p++)
+                {
+                    const double w = A.shell_idxs[2 * p] == A.shell_idxs[2 * p + 1] ? 1.0 : 2.0;
+                    for (size_t q = 0; q < C.n_shells; q++)
+                    {
+                        auto b = kern(p, q, A, C);
+                        double ba = 0, bs = 0;
+                        for (double v : b)
+                        {
+                            ba += std::fabs(v);
+                            bs += v;
+                        }
+                        a3 += w * ba;
+                        s3 += w * bs;
+                    }
+                }
+            }
+        for (size_t ia = 0; ia < shs.size(); ia++)
+            for (size_t ib = 0; ib <= ia; ib++)
+            {
+                const auto &A = shs[ia], &B = shs[ib];
+                ERI2Kernel kern(A, B);
+#pragma omp parallel for schedule(dynamic) reduction(+ : a2, s2)
+                for (size_t p = 0; p < A.n_shells; p++)
+                {
+                    const size_t bound = (A.l == B.l) ? p + 1 : B.n_shells;
+                    for (size_t q = 0; q < bound; q++)
+                    {
+                        const double w = (A.l == B.l && p == q) ? 1.0 : 2.0;
+                        auto b = kern(p, q, A, B);
+                        double ba = 0, bs = 0;
+                        for (double v : b)
+                        {
+                            ba += std::fabs(v);
+                            bs += v;
+                        }
+                        a2 += w * ba;
+                        s2 += w * bs;
+                    }
+                }
+            }
+        out[0] = a3;
+        out[1] = s3;
+        out[2] = a2;
+        out[3] = s2;
+    }
 }

@@ -200,6 +200,10 @@ namespace orc
     std::vector<double> eri3(const Structure &s);          // nbf*nbf*naux
     std::vector<double> eri2(const Structure &s);          // naux^2
     std::vector<double> eri2Diagonal(const Structure &s);  // naux
+    // Sums of |x| and x over the dense eri3 tensor and over the dense eri2 tensor (out[0..3]),
+    // accumulated block by block over the same enumeration as eri3() / eri2() with each block
+    // weighted by the number of its images in the dense tensor; no tensor is formed.
+    void riSums(const Structure &s, double out[4]);
 
     // ---- J/K, energy and gradient (new functionality; no reference counterpart) ----
     struct JKStats

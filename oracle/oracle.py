@@ -39,6 +39,7 @@ def lib():
         L.orc_structure_norms.argtypes = [_p, _p]
         for name in ("orc_eri4", "orc_eri4_diag", "orc_eri3", "orc_eri2", "orc_eri2_diag"):
             getattr(L, name).argtypes = [_p, _p]
+        L.orc_ri_sums.argtypes = [_p, _p]
         for name in ("orc_eri4d1", "orc_eri3d1", "orc_eri2d1"):
             getattr(L, name).argtypes = [_p, _p]
         L.orc_deploy_sum.argtypes = [_p, _i, _i, _p]
@@ -201,6 +202,12 @@ class Structure:
     def eri2_diag(self) -> np.ndarray:
         out = np.zeros(self.nbf_aux)
         _check(lib().orc_eri2_diag(self.h, _ptr(out)))
+        return out
+
+    def ri_sums(self) -> np.ndarray:
+        """Sums of |x| and x over the dense eri3 and eri2 tensors: [|eri3|, eri3, |eri2|, eri2]."""
+        out = np.zeros(4)
+        _check(lib().orc_ri_sums(self.h, _ptr(out)))
         return out
 
     def eri4d1(self) -> np.ndarray:

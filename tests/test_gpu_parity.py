@@ -177,7 +177,7 @@ def jk_case(oracle):
     return m, D, Jr, Kr, sr
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("variant", [0, 1, 2, 6])
 def test_jk_every_variant(lbx, jk_case, variant):
     """Every kernel variant of every class pair (where it exists) gives the oracle's J and K."""
     import torch
@@ -191,7 +191,7 @@ def test_jk_every_variant(lbx, jk_case, variant):
     assert st["quartets"] == sr["quartets"]
 
 
-@pytest.mark.parametrize("variant", [3, 4, 5])
+@pytest.mark.parametrize("variant", [3, 4, 5, 7])
 def test_jk_fp32_every_variant(lbx, jk_case, variant):
     """Every FP32 form, on every quartet (mixed threshold above every bound).  FP32 R and
     transforms carry ~1e-7 relative rounding per term, so J and K agree with the FP64 oracle to
@@ -370,25 +370,48 @@ def test_jk_tuned(lbx, jk_case):
     close(K.cpu().numpy(), Kr, rel=0.0, floor=1e-5)
 
 
-def test_jk_medium(lbx, oracle):
-    """10 atoms, 300 functions: lane sides of hundreds to thousands of pairs, so the one-lane
-    kernels run many lane blocks in several groups with the lane side ordered by shell inside
-    Schwarz buckets (long runs of shared shells, bucket boundaries inside blocks); default and
-    tuned variant choices against the oracle."""
-    import torch
+@pytest.fixture(scope="module")
+def jk_medium_case(oracle):
     m = B.synthetic_molecule(2031, 10, 8.0, B.RECIPE_MAIN)
     D = B.density_from_orbitals(9, m.shells.nbf, 30, 0.1)
     Jr, Kr, sr = oracle.Structure(m).jk(D, thresh=1e-12)
+    return m, D, Jr, Kr, sr
+
+
+@pytest.mark.parametrize("variant", [None, "tuned", 0, 1, 6])
+def test_jk_medium(lbx, jk_medium_case, variant):
+    """10 atoms, 300 functions: lane sides of hundreds to thousands of pairs, so the one-lane
+    kernels run many lane blocks in several groups with the lane side ordered by shell inside
+    Schwarz buckets (long runs of shared shells, bucket boundaries inside blocks), and the block
+    kernel many partial blocks; the default choice, the tuned choice and each FP64 variant
+    wherever the class pair has it, against the oracle."""
+    import torch
+    m, D, Jr, Kr, sr = jk_medium_case
     ctx = lbx.Context(m)
     Dt = torch.tensor(D, device="cuda")
+    if variant == "tuned":
+        ctx.tune(Dt, schwarz=1e-12)
+    elif variant is not None:
+        assert any([ctx.set_variant(c, variant) for c in CLASS_PAIRS])
     J, K, st = ctx.jk(Dt, schwarz=1e-12)
     assert st["quartets"] == sr["quartets"]
     close(J.cpu().numpy(), Jr)
     close(K.cpu().numpy(), Kr)
-    ctx.tune(Dt, schwarz=1e-12)
-    J, K, st = ctx.jk(Dt, schwarz=1e-12)
-    close(J.cpu().numpy(), Jr)
-    close(K.cpu().numpy(), Kr)
+
+
+@pytest.mark.parametrize("variant", [3, 4, 7])
+def test_jk_medium_fp32(lbx, jk_medium_case, variant):
+    """The FP32 forms on the 300-function system, every quartet in FP32 (mixed threshold above
+    every bound): within 1e-5 of the largest element of J and K."""
+    import torch
+    m, D, Jr, Kr, sr = jk_medium_case
+    ctx = lbx.Context(m)
+    assert any([ctx.set_variant(c, variant) for c in CLASS_PAIRS])
+    J, K, st = ctx.jk(torch.tensor(D, device="cuda"), schwarz=1e-12, precision=lbx.LBX_MIXED,
+                      mixed_thresh=1e30)
+    assert st["quartets"] == sr["quartets"]
+    close(J.cpu().numpy(), Jr, rel=0.0, floor=1e-5)
+    close(K.cpu().numpy(), Kr, rel=0.0, floor=1e-5)
 
 
 @pytest.mark.parametrize("thresh", [0.0, 1e-12])

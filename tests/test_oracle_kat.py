@@ -107,3 +107,26 @@ def test_deploy_ri(oracle, which, ref):
     s = oracle.Structure(mol("def2-svp", O3, "def2-universal-jkfit"))
     val = s.deploy_sum(which, False)
     assert abs(val - ref) <= REL * ref, (val, ref)
+
+
+@pytest.mark.parametrize("main,geom,aux,which,ref", [
+    ("def2-tzvp", O3, "def2-qzvp-rifit", 0, 25854.105851585416531),   # eri3 :730-746
+    ("def2-svp", H2O, "def2-qzvp-rifit", 2, 8801.334703460838),       # eri2 :694-710
+])
+def test_ri_sums_kat(oracle, main, geom, aux, which, ref):
+    """The dense-free checksums of eri3/eri2 (used for the 256-molecule fixtures) against the
+    reference's eri3 and eri2 checksums."""
+    s = oracle.Structure(mol(main, geom, aux))
+    val = s.ri_sums()[which]
+    assert abs(val - ref) <= REL * ref, (val, ref)
+
+
+def test_ri_sums_match_dense(oracle):
+    """All four sums against the dense tensors on a synthetic molecule with f main and g
+    auxiliary shells (signed sums too: a wrong image weight changes them)."""
+    m = B.synthetic_molecule(77, 3, 4.0, B.RECIPE_MAIN, B.RECIPE_AUX)
+    s = oracle.Structure(m)
+    e3, e2 = s.eri3(), s.eri2()
+    ref = np.array([np.abs(e3).sum(), e3.sum(), np.abs(e2).sum(), e2.sum()])
+    got = s.ri_sums()
+    assert np.allclose(got, ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max()), (got, ref)
