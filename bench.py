@@ -81,6 +81,20 @@ def measured_traffic(kernel: str):
     return t.get("traffic_bytes") if t.get("kernel") == kernel else None
 
 
+def energy_difference(D, J, K, Jm, Km) -> float:
+    """E(Jm, Km) - E(J, K) for E = 1/2 tr(DJ) - 1/4 tr(DK), formed from the differences
+    1/2 tr(D (Jm - J)) - 1/4 tr(D (Km - K)) so that it is not lost in the rounding of two
+    energies of ~1e5 Eh (one ulp of such an energy is ~1e-11 Eh)."""
+    return 0.5 * float((D * (Jm - J)).sum()) - 0.25 * float((D * (Km - K)).sum())
+
+
+def kernel_name(prof: dict) -> str:
+    """The template name of a class pair's J/K kernel from its profile record."""
+    kind = prof["kernel"]
+    base = "quartetM" if kind.startswith("mfma") else "quartetB" if kind.startswith("block") else "quartetV"
+    return "%s<%d,%d,%d,%d> J/K (%s)" % (base, *prof["class"], kind)
+
+
 def timed(fn, steps: int, label: str):
     import torch
     torch.cuda.synchronize()
@@ -340,18 +354,17 @@ def main(argv=None):
             log(f"[bench] mixed step {k} done")
         torch.cuda.synchronize()
         m_s = (time.perf_counter() - t0) / a.mixed_steps
-        Em = 0.5 * float((D * Jm).sum()) - 0.25 * float((D * Km).sum())
+        dE = energy_difference(D, J, K, Jm, Km)
         mixed = {
             "split": MIXED_SPLIT,
             "ms_per_step": m_s * 1e3,
             "quartets_per_s": sm["quartets"] / m_s,
             "fp32_share": sm["quartets_fp32"] / max(sm["quartets"], 1),
             "speedup_vs_fp64": step_s / m_s,
-            "delta_E": Em - E,
-            "delta_E_rel": (Em - E) / abs(E),
+            "delta_E": dE,
+            "delta_E_rel": dE / abs(E),
             "max_abs_dJ": float((Jm - J).abs().max()),
             "max_abs_dK": float((Km - K).abs().max()),
-            "energy_2e": Em,
         }
 
     out = {
@@ -383,15 +396,12 @@ def main(argv=None):
         "roofline": {
             # FP64 compute bound: MI355X's 78.6 TFLOP/s FP64 peak is the same for MFMA and VALU
             "bound": "mfma",
-            "kernel": "%s<%d,%d,%d,%d> J/K (%s)" % (
-                "quartetM" if dom["kernel"] == "mfma" else "quartetV", *dom["class"],
-                dom["kernel"]),
+            "kernel": kernel_name(dom),
             "achieved": dom_tflops,
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": dom_tflops / FP64_PEAK_TFLOPS,
-            "traffic": measured_traffic("%s<%d,%d,%d,%d> J/K (%s)" % (
-                "quartetM" if dom["kernel"] == "mfma" else "quartetV", *dom["class"], dom["kernel"])),
+            "traffic": measured_traffic(kernel_name(dom)),
             "kernel_ms": dom_ms,
             "kernel_share": dom["ms"] / max(sum(p["ms"] for p in prof), 1e-12),
         },

@@ -84,6 +84,24 @@ int lbx_eri4d1(lbx_context *ctx, double *out);
 int lbx_eri3d1(lbx_context *ctx, double *out);
 int lbx_eri2d1(lbx_context *ctx, double *out);
 
+/* Blocks of listed shell quartets, triplets or doublets, the per-call unit of the reference's
+ * kernel functors (ERI4Kernel / ERI3Kernel / ERI2Kernel / ERI4D1Kernel / ERI3D1Kernel /
+ * ERI2D1Kernel::operator(), eri_kernels.hpp:60-170, eri_kernel_funs.hpp:104-346, 350-732), for
+ * many entries in one launch.  shells: host, n entries of 4 (a, b, c, d: main-basis shell
+ * indices), 3 (a, b main, c auxiliary) or 2 (a, b auxiliary) shell indices, shells numbered in
+ * AO order from 0 within their basis; every entry of one call has the angular momenta of the
+ * first.  out: device, entry t at out + t * ncomp * Sa*Sb*Sc*Sd (S = 2l + 1 of each shell), row
+ * major in the entry's shell order (the reference's vec4d / vec3d / vec2d), ncomp = 1 for the
+ * integrals and 12 / 9 / 6 for the derivatives (component 3k + x: coordinate x of the centre of
+ * the entry's k-th shell, the reference's [Ax..Dz] order).  Entries whose shell pair has no
+ * primitive pair above the 1e-15 screen (absent from the reference's ShellPairData) are zero. */
+int lbx_eri4_batch(lbx_context *ctx, long n, const int *shells, double *out);
+int lbx_eri3_batch(lbx_context *ctx, long n, const int *shells, double *out);
+int lbx_eri2_batch(lbx_context *ctx, long n, const int *shells, double *out);
+int lbx_eri4d1_batch(lbx_context *ctx, long n, const int *shells, double *out);
+int lbx_eri3d1_batch(lbx_context *ctx, long n, const int *shells, double *out);
+int lbx_eri2d1_batch(lbx_context *ctx, long n, const int *shells, double *out);
+
 /* Coulomb and exchange matrices, no reference counterpart (the reference stops at the dense
  * tensor).  D, J, K: device (nbf,nbf) row-major, D symmetric.
  *   J_mn = sum_kl (mn|kl) D_kl,   K_mk = sum_nl (mn|kl) D_nl

@@ -21,7 +21,8 @@ namespace lible::ints
 {
     namespace detail
     {
-        struct Tensors;   // a structure's dense ERI tensors, computed on the GPU on first use
+        struct Tensors;      // ties shell data and kernel functors to their structure
+        struct BlockCache;   // a kernel functor's blocks, computed on the GPU in chunks
     }
 
     /// Angular momentum, exponents and raw contraction coefficients of one shell
@@ -116,7 +117,8 @@ namespace lible::ints
 
         /// The lbx context of this structure (created on the current device on first use).
         lbx_context *context() const;
-        /// The dense tensors behind the kernel functors (shared by copies of the Structure).
+        /// The link from shell data and kernel functors back to this structure (shared by copies
+        /// of the Structure).
         std::shared_ptr<detail::Tensors> tensors() const;
         /// Normalised shells in AO order: l, atom, AO offset, centre (bohr), exponents, raw
         /// coefficients, primitive norms, spherical norms (constructShells, shell.cpp:89-135).
@@ -204,10 +206,12 @@ namespace lible::ints
     std::vector<ShellData> shellDataAux(const Structure &structure);
 
     /// Per-block kernels of the reference (eri_kernels.hpp:60-170), same constructors and call
-    /// operators.  Behind them the whole structure's tensor is evaluated once on the GPU (on the
-    /// first construction of a kernel of that kind) and each call returns its block, so they
-    /// suit the small systems the dense tensors fit; construction is not thread-safe, the call
-    /// operators are.  The data must come from shellPairData() / shellDataAux().
+    /// operators.  Each computes only its own class pair's blocks: a call evaluates, on the GPU
+    /// (lbx_eri*_batch), the chunk of consecutive first-side indices (bra pairs, or auxiliary
+    /// shells) that holds it against every second-side index, about 32 MiB of blocks, and
+    /// keeps the last 8 chunks on the host, so callers that walk the first side in order make
+    /// one GPU call per chunk.  Construction is not thread-safe, the call operators are.  The
+    /// data must come from shellPairData() / shellDataAux().
     class ERI4Kernel
     {
     public:
@@ -215,7 +219,7 @@ namespace lible::ints
         vec4d operator()(size_t ipair_ab, size_t ipair_cd, const ShellPairData &sp_data_ab,
                          const ShellPairData &sp_data_cd) const;
     private:
-        std::shared_ptr<detail::Tensors> t_;
+        std::shared_ptr<detail::BlockCache> c_;
     };
 
     class ERI3Kernel
@@ -225,7 +229,7 @@ namespace lible::ints
         vec3d operator()(size_t ipair_ab, size_t ishell_c, const ShellPairData &sp_data_ab,
                          const ShellData &sh_data_c) const;
     private:
-        std::shared_ptr<detail::Tensors> t_;
+        std::shared_ptr<detail::BlockCache> c_;
     };
 
     class ERI2Kernel
@@ -235,7 +239,7 @@ namespace lible::ints
         vec2d operator()(size_t ishell_a, size_t ishell_b, const ShellData &sh_data_a,
                          const ShellData &sh_data_b) const;
     private:
-        std::shared_ptr<detail::Tensors> t_;
+        std::shared_ptr<detail::BlockCache> c_;
     };
 
     /// 12 blocks: derivatives with respect to A, B, C, D (x, y, z each).
@@ -247,7 +251,7 @@ namespace lible::ints
                                          const ShellPairData &sp_data_ab,
                                          const ShellPairData &sp_data_cd) const;
     private:
-        std::shared_ptr<detail::Tensors> t_;
+        std::shared_ptr<detail::BlockCache> c_;
     };
 
     /// 9 blocks: A, B, C.
@@ -259,7 +263,7 @@ namespace lible::ints
                                         const ShellPairData &sp_data_ab,
                                         const ShellData &sh_data_c) const;
     private:
-        std::shared_ptr<detail::Tensors> t_;
+        std::shared_ptr<detail::BlockCache> c_;
     };
 
     /// 6 blocks: A, B.
@@ -270,7 +274,7 @@ namespace lible::ints
         std::array<vec2d, 6> operator()(size_t ishell_a, size_t ishell_b, const ShellData &sh_data_a,
                                         const ShellData &sh_data_b) const;
     private:
-        std::shared_ptr<detail::Tensors> t_;
+        std::shared_ptr<detail::BlockCache> c_;
     };
 
     /// (mn|kl), dense nbf^4 (eri4.cpp:38-95).
@@ -288,4 +292,12 @@ namespace lible::ints
     /// quartets (no reference counterpart).  D symmetric nbf x nbf.
     std::pair<vec2d, vec2d> coulombExchange(const Structure &structure, const vec2d &D,
                                             double schwarz_threshold = 1e-12);
+
+    /// The backward of coulombExchange's energy: the nuclear gradient dE/dR (atoms x 3, Eh/bohr,
+    /// real atoms then ghost atoms) of E = 1/2 tr(DJ) - 1/4 tr(DK) at fixed D, over the same
+    /// Schwarz-screened unique quartets (no reference counterpart: the reference supplies the
+    /// derivative integrals, ERI4D1Kernel, eri_kernels.hpp:113-131; here they are contracted
+    /// with the density inside the kernel and never stored).  D symmetric nbf x nbf.
+    vec2d coulombExchangeGradient(const Structure &structure, const vec2d &D,
+                                  double schwarz_threshold = 1e-12);
 }

@@ -13,6 +13,7 @@
 #include <numeric>
 #include <string>
 #include <tuple>
+#include <unordered_map>
 
 namespace lbx
 {
@@ -620,6 +621,8 @@ struct lbx_context : Context
     std::map<std::pair<int, int>, int> tuned, tuned32;
     // lbx_set_class_filter: the one class pair (bra class, ket class) lbx_jk evaluates; -1: all
     int filter_i = -1, filter_j = -1;
+    // lbx_eri*_batch: (shell a, shell b) -> pair index per main pair class
+    std::vector<std::unordered_map<long long, int>> pair_maps;
 
     int variantFor(int i, int j)
     {
@@ -938,6 +941,79 @@ struct lbx_context : Context
         return t;
     }
 
+    // Queues one J/K build on the context stream: J and K cleared, every class pair's kernel
+    // (the FP32 tier of a mixed build below `split`), J and K symmetrised.
+    lbx_jk_stats queueJK(const double *D, double *J, double *K, double schwarz_thresh, double split,
+                         bool mixed, bool prof_on);
+
+    // J/K launch sequences captured into graphs, per (D, J, K, Schwarz threshold, split, mixed);
+    // dropped when the variant choice changes (lbx_tune, lbx_set_variant)
+    using JKKey = std::tuple<const double *, double *, double *, double, double, int>;
+    struct JKGraph
+    {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        lbx_jk_stats stats{};
+    };
+    std::map<JKKey, JKGraph> jk_graphs;
+    static constexpr size_t kMaxJKGraphs = 8;
+
+    void dropJKGraphs()
+    {
+        for (auto &kv : jk_graphs)
+        {
+            if (kv.second.exec)
+                (void)hipGraphExecDestroy(kv.second.exec);
+            if (kv.second.graph)
+                (void)hipGraphDestroy(kv.second.graph);
+        }
+        jk_graphs.clear();
+    }
+
+    // Captures the launch sequence of a build just run eagerly (its task tables exist, so the
+    // capture allocates nothing) on a private stream of default priority.
+    void captureJK(const JKKey &key, const lbx_jk_stats &st)
+    {
+        if (jk_graphs.size() >= kMaxJKGraphs)
+            dropJKGraphs();
+        hipStream_t cs = nullptr;
+        LBX_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+        hipStream_t saved = stream;
+        JKGraph g;
+        g.stats = st;
+        try
+        {
+            stream = cs;
+            LBX_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+            try
+            {
+                queueJK(std::get<0>(key), std::get<1>(key), std::get<2>(key), std::get<3>(key), std::get<4>(key),
+                        std::get<5>(key) != 0, false);
+            }
+            catch (...)
+            {
+                hipGraph_t tmp = nullptr;
+                (void)hipStreamEndCapture(cs, &tmp);
+                if (tmp)
+                    (void)hipGraphDestroy(tmp);
+                throw;
+            }
+            LBX_HIP(hipStreamEndCapture(cs, &g.graph));
+            stream = saved;
+            LBX_HIP(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+        }
+        catch (...)
+        {
+            stream = saved;
+            if (g.graph)
+                (void)hipGraphDestroy(g.graph);
+            (void)hipStreamDestroy(cs);
+            throw;
+        }
+        (void)hipStreamDestroy(cs);
+        jk_graphs.emplace(key, g);
+    }
+
     // per-class timing (lbx_profile_enable)
     struct Prof
     {
@@ -951,6 +1027,7 @@ struct lbx_context : Context
     ~lbx_context()
     {
         dropRiGraph();
+        dropJKGraphs();
         for (auto &kv : prof)
         {
             if (kv.second.start)
@@ -1365,6 +1442,202 @@ extern "C"
         }
     }
 
+    // Blocks of listed shell quartets / triplets / doublets (kind 4, 3, 2) in the caller's
+    // shell order, the integrals themselves (plain) or their first derivatives; every entry of
+    // one call has the same angular momenta.  Entries whose shell pair has no primitive pair
+    // above the 1e-15 screen (the reference's ShellPairData drops such pairs) are zero.
+    namespace
+    {
+        // (shell a, shell b) of a pair class -> pair index, made on first use
+        const std::unordered_map<long long, int> &pairIndex(Context &c, std::vector<std::unordered_map<long long, int>> &maps,
+                                                           int cls_idx)
+        {
+            if (maps.size() < c.cls.size())
+                maps.resize(c.cls.size());
+            auto &m = maps[cls_idx];
+            const PairClass &pc = c.cls[cls_idx];
+            if (m.empty() && pc.n_pairs > 0)
+            {
+                const long long nsh = static_cast<long long>(c.shells.size());
+                for (int p = 0; p < pc.n_pairs; p++)
+                    m.emplace(static_cast<long long>(pc.sh_a[p]) * nsh + pc.sh_b[p], p);
+            }
+            return m;
+        }
+
+        void runBlocks(lbx_context &c, std::vector<std::unordered_map<long long, int>> &maps, int kind,
+                       bool plain, long long nq, const int *sh, double *out, const char *who)
+        {
+            const int per = kind == 4 ? 4 : (kind == 3 ? 3 : 2);
+            const int nsh = static_cast<int>(c.shells.size()), nax = static_cast<int>(c.shells_aux.size());
+            auto isAux = [&](int pos) { return kind == 2 || (kind == 3 && pos == 2); };
+            auto lOf = [&](int pos, int idx) {
+                const int n = isAux(pos) ? nax : nsh;
+                if (idx < 0 || idx >= n)
+                    throw Error(LBX_ERR_INVALID, std::string(who) + ": shell index " + std::to_string(idx) +
+                                                         " out of range");
+                return isAux(pos) ? c.shells_aux[idx].l : c.shells[idx].l;
+            };
+            if (nq == 0)
+                return;
+            int l[4] = {0, 0, 0, 0};
+            for (int k = 0; k < per; k++)
+                l[k] = lOf(k, sh[k]);
+            for (long long t = 0; t < nq; t++)
+                for (int k = 0; k < per; k++)
+                    if (lOf(k, sh[per * t + k]) != l[k])
+                        throw Error(LBX_ERR_INVALID, std::string(who) + ": every entry must have the "
+                                                                        "angular momenta of the first");
+            const int ncomp = plain ? 1 : 3 * (kind == 4 ? 4 : (kind == 3 ? 3 : 2));
+            long long bsize = 1;
+            for (int k = 0; k < per; k++)
+                bsize *= nSph(l[k]);
+            fill(out, nq * ncomp * bsize, 0.0, c.stream);
+            // library pair classes of the two sides
+            PairClass *bra, *ket;
+            int ibra = -1, iket = -1;
+            if (kind == 2)
+            {
+                bra = &c.aux[l[0]];
+                ket = &c.aux[l[1]];
+            }
+            else
+            {
+                ibra = pairClassIdx(std::max(l[0], l[1]), std::min(l[0], l[1]));
+                bra = &c.cls[ibra];
+                if (kind == 4)
+                {
+                    iket = pairClassIdx(std::max(l[2], l[3]), std::min(l[2], l[3]));
+                    ket = &c.cls[iket];
+                }
+                else
+                    ket = &c.aux[l[2]];
+            }
+            // pair of two main shells in the library's order (l_a > l_b, or equal l and a >= b)
+            auto mainPair = [&](int cls_idx, int a, int b, bool &swapped) {
+                swapped = c.shells[a].l < c.shells[b].l || (c.shells[a].l == c.shells[b].l && a < b);
+                const long long key = static_cast<long long>(swapped ? b : a) * nsh + (swapped ? a : b);
+                const auto &m = pairIndex(c, maps, cls_idx);
+                auto it = m.find(key);
+                return it == m.end() ? -1 : it->second;
+            };
+            // single auxiliary shell -> its position in the class of its l
+            auto auxPair = [&](const PairClass &pc, int s) {
+                auto it = std::lower_bound(pc.sh_a.begin(), pc.sh_a.end(), s);
+                return it != pc.sh_a.end() && *it == s ? static_cast<int>(it - pc.sh_a.begin()) : -1;
+            };
+            std::vector<int> tasks;
+            tasks.reserve(4 * nq);
+            for (long long t = 0; t < nq; t++)
+            {
+                const int *e = sh + per * t;
+                bool sab = false, scd = false;
+                int p, q;
+                if (kind == 2)
+                {
+                    p = auxPair(*bra, e[0]);
+                    q = auxPair(*ket, e[1]);
+                }
+                else
+                {
+                    p = mainPair(ibra, e[0], e[1], sab);
+                    q = kind == 4 ? mainPair(iket, e[2], e[3], scd) : auxPair(*ket, e[2]);
+                }
+                if (p < 0 || q < 0)
+                    continue;
+                tasks.insert(tasks.end(), {p, q, (sab ? 1 : 0) | (scd ? 2 : 0), static_cast<int>(t)});
+            }
+            if (tasks.empty())
+                return;
+            if (!plain)
+            {
+                if (kind != 2)
+                    computeEcoeffsD1(c, *bra);
+                if (kind == 4)
+                    computeEcoeffsD1(c, *ket);
+            }
+            void *d = nullptr;
+            LBX_HIP(hipMalloc(&d, tasks.size() * sizeof(int)));
+            try
+            {
+                LBX_HIP(hipMemcpyAsync(d, tasks.data(), tasks.size() * sizeof(int), hipMemcpyHostToDevice,
+                                       c.stream));
+                D1Args a{};
+                a.bra = bra->view();
+                a.ket = ket->view();
+                a.la = bra->la;
+                a.lb = bra->lb;
+                a.lc = ket->la;
+                a.ld = ket->lb;
+                a.bra_single = bra->single ? 1 : 0;
+                a.ket_single = ket->single ? 1 : 0;
+                a.tasks = static_cast<const int *>(d);
+                a.ntasks = static_cast<long long>(tasks.size() / 4);
+                a.boys = c.d_boys;
+                a.out = out;
+                a.kind = kind;
+                a.nbf = static_cast<long long>(c.nbf);
+                a.naux = static_cast<long long>(c.nbf_aux);
+                a.blocks = 1;
+                a.plain = plain ? 1 : 0;
+                launchD1(a, c.stream);
+                LBX_HIP(hipStreamSynchronize(c.stream));
+            }
+            catch (...)
+            {
+                (void)hipStreamSynchronize(c.stream);
+                (void)hipFree(d);
+                throw;
+            }
+            (void)hipFree(d);
+        }
+
+        int blocksEntry(lbx_context *ctx, int kind, bool plain, long nq, const int *shells, double *out,
+                        const char *who)
+        {
+            return guarded([&]
+                           {
+                               if (!ctx || nq < 0 || (nq > 0 && (!shells || !out)))
+                                   throw Error(LBX_ERR_INVALID, std::string(who) + ": null argument or negative count");
+                               std::lock_guard<std::mutex> lk(ctx->mu);
+                               requireSingle(*ctx, who);
+                               if (kind != 4 && ctx->shells_aux.empty())
+                                   throw Error(LBX_ERR_STATE, std::string(who) + ": RI approximation is not enabled");
+                               runBlocks(*ctx, ctx->pair_maps, kind, plain, nq, shells, out, who);
+                           });
+        }
+    }
+
+    int lbx_eri4_batch(lbx_context *ctx, long nq, const int *shells, double *out)
+    {
+        return blocksEntry(ctx, 4, true, nq, shells, out, "lbx_eri4_batch");
+    }
+
+    int lbx_eri3_batch(lbx_context *ctx, long nq, const int *shells, double *out)
+    {
+        return blocksEntry(ctx, 3, true, nq, shells, out, "lbx_eri3_batch");
+    }
+
+    int lbx_eri2_batch(lbx_context *ctx, long nq, const int *shells, double *out)
+    {
+        return blocksEntry(ctx, 2, true, nq, shells, out, "lbx_eri2_batch");
+    }
+
+    int lbx_eri4d1_batch(lbx_context *ctx, long nq, const int *shells, double *out)
+    {
+        return blocksEntry(ctx, 4, false, nq, shells, out, "lbx_eri4d1_batch");
+    }
+
+    int lbx_eri3d1_batch(lbx_context *ctx, long nq, const int *shells, double *out)
+    {
+        return blocksEntry(ctx, 3, false, nq, shells, out, "lbx_eri3d1_batch");
+    }
+
+    int lbx_eri2d1_batch(lbx_context *ctx, long nq, const int *shells, double *out)
+    {
+        return blocksEntry(ctx, 2, false, nq, shells, out, "lbx_eri2d1_batch");
+    }
+
     int lbx_eri4d1(lbx_context *ctx, double *out)
     {
         return guarded([&]
@@ -1405,6 +1678,65 @@ extern "C"
                        });
     }
 
+    lbx_jk_stats lbx_context::queueJK(const double *D, double *J, double *K, double schwarz_thresh,
+                                      double split, bool mixed, bool prof_on)
+    {
+        Context &c = *this;
+        const long long n = static_cast<long long>(c.nbf);
+        fill(J, n * n, 0.0, c.stream);
+        fill(K, n * n, 0.0, c.stream);
+        lbx_jk_stats st{};
+        auto launch = [&](int i, int j, int tier, int var, double lo, double hi)
+        {
+            const PairClass &bra = c.cls[i], &ket = c.cls[j];
+            Prof *pr = nullptr;
+            if (prof_on)
+            {
+                pr = &prof[{i, j, tier}];
+                if (!pr->start)
+                {
+                    LBX_HIP(hipEventCreate(&pr->start));
+                    LBX_HIP(hipEventCreate(&pr->stop));
+                    pr->rec.la = bra.la;
+                    pr->rec.lb = bra.lb;
+                    pr->rec.lc = ket.la;
+                    pr->rec.ld = ket.lb;
+                }
+                pr->rec.kernel = var;
+                LBX_HIP(hipEventRecord(pr->start, c.stream));
+            }
+            const TaskTable &t = launchJK(i, j, var, D, J, K, lo, hi);
+            st.quartets += t.quartets;
+            st.flops += t.flops;
+            if (isF32Variant(var))
+                st.quartets_fp32 += t.quartets;
+            if (pr)
+            {
+                LBX_HIP(hipEventRecord(pr->stop, c.stream));
+                pr->pending = true;
+                pr->rec.calls += 1;
+                pr->rec.quartets += t.quartets;
+                pr->rec.flops += t.flops;
+            }
+        };
+        for (int i = 0; i < static_cast<int>(c.cls.size()); i++)
+            for (int j = 0; j <= i; j++)
+            {
+                if (c.cls[i].n_pairs == 0 || c.cls[j].n_pairs == 0)
+                    continue;
+                if (filter_i >= 0 && (i != filter_i || j != filter_j))
+                    continue;
+                launch(i, j, 0, variantFor(i, j), split, kNoUpper);
+                if (mixed)
+                    launch(i, j, 1, variant32For(i, j), schwarz_thresh, split);
+            }
+        const int gridn = static_cast<int>(std::min<long long>((n * n + 255) / 256, 65536));
+        hipLaunchKernelGGL(symmetrizeKernel, dim3(gridn), dim3(256), 0, c.stream, J, static_cast<int>(n));
+        hipLaunchKernelGGL(symmetrizeKernel, dim3(gridn), dim3(256), 0, c.stream, K, static_cast<int>(n));
+        LBX_HIP(hipGetLastError());
+        return st;
+    }
+
     int lbx_jk(lbx_context *ctx, const double *D, double *J, double *K, double schwarz_thresh,
                int precision, double mixed_thresh, lbx_jk_stats *stats)
     {
@@ -1427,67 +1759,31 @@ extern "C"
                            const double split = mixed ? mixed_thresh : schwarz_thresh;
                            if (mixed)
                                ensureF32(c);
-                           const long long n = static_cast<long long>(c.nbf);
-                           fill(J, n * n, 0.0, c.stream);
-                           fill(K, n * n, 0.0, c.stream);
-                           double nq = 0, fl = 0, nq32 = 0;
-                           auto launch = [&](int i, int j, int tier, int var, double lo, double hi)
+                           // Replays: the launch sequence of a (D, J, K, thresholds) set is
+                           // captured into a HIP graph after its first (eager) build and replayed
+                           // on later calls; per-class timing and the class filter run eagerly.
+                           const bool graphs = !ctx->profiling && ctx->filter_i < 0;
+                           const lbx_context::JKKey gkey{D, J, K, schwarz_thresh, split, mixed ? 1 : 0};
+                           if (graphs)
                            {
-                               const PairClass &bra = c.cls[i], &ket = c.cls[j];
-                               lbx_context::Prof *pr = nullptr;
-                               if (ctx->profiling)
+                               auto it = ctx->jk_graphs.find(gkey);
+                               if (it != ctx->jk_graphs.end())
                                {
-                                   pr = &ctx->prof[{i, j, tier}];
-                                   if (!pr->start)
-                                   {
-                                       LBX_HIP(hipEventCreate(&pr->start));
-                                       LBX_HIP(hipEventCreate(&pr->stop));
-                                       pr->rec.la = bra.la;
-                                       pr->rec.lb = bra.lb;
-                                       pr->rec.lc = ket.la;
-                                       pr->rec.ld = ket.lb;
-                                   }
-                                   pr->rec.kernel = var;
-                                   LBX_HIP(hipEventRecord(pr->start, c.stream));
+                                   LBX_HIP(hipGraphLaunch(it->second.exec, c.stream));
+                                   LBX_HIP(hipStreamSynchronize(c.stream));
+                                   if (stats)
+                                       *stats = it->second.stats;
+                                   return;
                                }
-                               const TaskTable &t = ctx->launchJK(i, j, var, D, J, K, lo, hi);
-                               nq += t.quartets;
-                               fl += t.flops;
-                               if (isF32Variant(var))
-                                   nq32 += t.quartets;
-                               if (pr)
-                               {
-                                   LBX_HIP(hipEventRecord(pr->stop, c.stream));
-                                   pr->pending = true;
-                                   pr->rec.calls += 1;
-                                   pr->rec.quartets += t.quartets;
-                                   pr->rec.flops += t.flops;
-                               }
-                           };
-                           for (int i = 0; i < static_cast<int>(c.cls.size()); i++)
-                               for (int j = 0; j <= i; j++)
-                               {
-                                   if (c.cls[i].n_pairs == 0 || c.cls[j].n_pairs == 0)
-                                       continue;
-                                   if (ctx->filter_i >= 0 && (i != ctx->filter_i || j != ctx->filter_j))
-                                       continue;
-                                   launch(i, j, 0, ctx->variantFor(i, j), split, kNoUpper);
-                                   if (mixed)
-                                       launch(i, j, 1, ctx->variant32For(i, j), schwarz_thresh, split);
-                               }
-                           const int gridn = static_cast<int>(std::min<long long>((n * n + 255) / 256, 65536));
-                           hipLaunchKernelGGL(symmetrizeKernel, dim3(gridn), dim3(256), 0, c.stream, J, static_cast<int>(n));
-                           hipLaunchKernelGGL(symmetrizeKernel, dim3(gridn), dim3(256), 0, c.stream, K, static_cast<int>(n));
-                           LBX_HIP(hipGetLastError());
+                           }
+                           lbx_jk_stats st = ctx->queueJK(D, J, K, schwarz_thresh, split, mixed, ctx->profiling);
                            LBX_HIP(hipStreamSynchronize(c.stream));
                            if (ctx->profiling)
                                ctx->collectProfile();
+                           if (graphs)
+                               ctx->captureJK(gkey, st);
                            if (stats)
-                           {
-                               stats->quartets = nq;
-                               stats->flops = fl;
-                               stats->quartets_fp32 = nq32;
-                           }
+                               *stats = st;
                        });
     }
 
@@ -1678,6 +1974,7 @@ extern "C"
                                                best32_ms = ms;
                                            }
                                        }
+                                       ctx->dropJKGraphs();
                                        if (best >= 0)
                                            ctx->tuned[{static_cast<int>(i), static_cast<int>(j)}] = best;
                                        if (best32 >= 0)
@@ -1707,6 +2004,7 @@ extern "C"
                                throw Error(LBX_ERR_INVALID, "lbx_set_variant: not a canonical class pair");
                            std::lock_guard<std::mutex> lk(ctx->mu);
                            const std::pair<int, int> key{pairClassIdx(la, lb), pairClassIdx(lc, ld)};
+                           ctx->dropJKGraphs();
                            if (variant < 0)
                            {
                                ctx->tuned.erase(key);

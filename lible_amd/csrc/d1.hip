@@ -92,19 +92,21 @@ namespace lbx
 
             // components: kind 4 centres A B C D, kind 3 A B C, kind 2 A C
             const int ncent = A.kind == 4 ? 4 : (A.kind == 3 ? 3 : 2);
+            const int ncomp = A.plain ? 1 : 3 * ncent;
             const long long n = A.nbf;
+            const int ts = A.blocks ? 4 : 2;
             for (long long task = blockIdx.x; task < A.ntasks; task += gridDim.x)
             {
-                const int p = A.tasks[2 * task], q = A.tasks[2 * task + 1];
+                const int p = A.tasks[ts * task], q = A.tasks[ts * task + 1];
                 const int poff = A.bra.poff[p], pcnt = A.bra.pcnt[p];
                 const int qoff = A.ket.poff[q], qcnt = A.ket.pcnt[q];
                 const int ao[4] = {A.bra.ao_a[p], A.bra.ao_b[p], A.ket.ao_a[q], A.ket.ao_b[q]};
-                for (int comp = 0; comp < 3 * ncent; comp++)
+                for (int comp = 0; comp < ncomp; comp++)
                 {
                     const int ci = comp / 3, x = comp % 3;
-                    // internal centre: 0 A, 1 B, 2 C, 3 D
-                    const int cent = A.kind == 2 ? (ci == 0 ? 0 : 2) : ci;
-                    const int ex = x == 0, ey = x == 1, ez = x == 2;
+                    // internal centre: 0 A, 1 B, 2 C, 3 D; -1 for the integrals themselves
+                    const int cent = A.plain ? -1 : (A.kind == 2 ? (ci == 0 ? 0 : 2) : ci);
+                    const int ex = cent >= 0 && x == 0, ey = cent >= 0 && x == 1, ez = cent >= 0 && x == 2;
                     double acc[kD1Acc];
 #pragma unroll
                     for (int r = 0; r < kD1Acc; r++)
@@ -113,11 +115,11 @@ namespace lbx
                     for (int i = poff; i < poff + pcnt; i++)
                     {
                         const double4 P = A.bra.pp[i];
-                        const double rai = A.bra_single ? 1.0 : A.bra.ra[i];
+                        const double rai = (A.bra_single || A.plain) ? 1.0 : A.bra.ra[i];
                         for (int k = qoff; k < qoff + qcnt; k++)
                         {
                             const double4 Q = A.ket.pp[k];
-                            const double rck = A.ket_single ? 1.0 : A.ket.ra[k];
+                            const double rck = (A.ket_single || A.plain) ? 1.0 : A.ket.ra[k];
                             const double pq = P.x * Q.x, ps = P.x + Q.x, alpha = pq / ps;
                             const double X = P.y - Q.y, Y = P.z - Q.z, Z = P.w - Q.w;
                             if (lane == 0)
@@ -157,7 +159,9 @@ namespace lbx
 
                             // ket transform: Y = aT sum_g R_{h+g+e_x} Ek_g + a0 sum_g R_{h+g} Ek'_g
                             double aT, a0 = 0.0;
-                            if (cent == 0)
+                            if (cent < 0)
+                                aT = 1.0;
+                            else if (cent == 0)
                                 aT = rai;
                             else if (cent == 1)
                                 aT = 1.0 - rai;
@@ -209,7 +213,7 @@ namespace lbx
                             }
                             __syncthreads();
                             // the bra pair's own derivative: +- Eb'_x R_{h+g} Ek
-                            if (cent <= 1 && !A.bra_single)
+                            if ((cent == 0 || cent == 1) && !A.bra_single)
                             {
                                 for (int e = lane; e < Hab * Scd; e += 64)
                                 {
@@ -246,6 +250,41 @@ namespace lbx
                         }
                     }
 
+                    if (A.blocks)
+                    {
+                        // the block in the caller's shell order: a <-> b (flag 1), c <-> d (2)
+                        const int flags = A.tasks[4 * task + 2];
+                        const long long slot = A.tasks[4 * task + 3];
+                        const bool swab = flags & 1, swcd = flags & 2;
+                        const int rb = swab ? Sa : Sb, rc = swcd ? Sd : Sc, rd = swcd ? Sc : Sd;
+                        // component of the caller's centre: its shell position k, 3 k + x
+                        int rcomp = 0;
+                        if (cent >= 0)
+                        {
+                            int pos = cent;
+                            if (A.kind == 2)
+                                pos = cent == 0 ? 0 : 1;
+                            else if (cent <= 1)
+                                pos = swab ? 1 - cent : cent;
+                            else if (A.kind == 4)
+                                pos = swcd ? 5 - cent : cent;
+                            rcomp = 3 * pos + x;
+                        }
+                        double *dst = A.out + (slot * ncomp + rcomp) * nOut;
+#pragma unroll
+                        for (int r = 0; r < kD1Acc; r++)
+                        {
+                            const int o = lane + 64 * r;
+                            if (o >= nOut)
+                                continue;
+                            const int sab = o / Scd, scd = o - sab * Scd;
+                            const int i0 = sab / Sb, i1 = sab % Sb, i2 = scd / Sd, i3 = scd % Sd;
+                            const int m0 = swab ? i1 : i0, m1 = swab ? i0 : i1;
+                            const int m2 = swcd ? i3 : i2, m3 = swcd ? i2 : i3;
+                            dst[((m0 * rb + m1) * rc + m2) * rd + m3] = acc[r];
+                        }
+                        continue;
+                    }
                     // scatter this component's block to every image
 #pragma unroll
                     for (int r = 0; r < kD1Acc; r++)

@@ -10,6 +10,7 @@
 #include <cstdlib>
 #include <filesystem>
 #include <fstream>
+#include <list>
 #include <mutex>
 #include <sstream>
 
@@ -480,37 +481,99 @@ namespace lible::ints
         return {std::move(J), std::move(K)};
     }
 
+    vec2d coulombExchangeGradient(const Structure &s, const vec2d &D, double schwarz_threshold)
+    {
+        const size_t n = s.getDimAO();
+        if (D.dim<0>() != n || D.dim<1>() != n)
+            throw std::runtime_error("coulombExchangeGradient(): density must be nbf x nbf");
+        lbx_context *ctx = s.context();
+        long nbf = 0, naux = 0, natoms = 0;
+        check(lbx_dims(ctx, &nbf, &naux, &natoms));
+        DeviceArray dD(n * n), dG(3 * static_cast<size_t>(natoms));
+        dD.fromHost(D.memptr());
+        check(lbx_jk_grad(ctx, dD.get(), dG.get(), schwarz_threshold, LBX_FP64, 0.0, nullptr));
+        check(lbx_synchronize(ctx));
+        vec2d g(static_cast<size_t>(natoms), 3);
+        dG.toHost(g.memptr());
+        return g;
+    }
+
     // ------------------------------------------------------------------ shell data, kernels
     namespace detail
     {
-        /// A structure's dense tensors, each evaluated on the GPU the first time a kernel of its
-        /// kind is constructed and kept on the host for the kernels' call operators.
+        /// Ties shell data and kernel functors to their structure: a copy of the structure made
+        /// before this object existed (no ownership cycle), which shares the device context when
+        /// one was already made.
         struct Tensors
         {
-            // a copy of the structure made before this object existed (no ownership cycle);
-            // shares the device context when one was already made
             std::shared_ptr<const Structure> source;
             size_t n = 0, naux = 0;
-            std::mutex mu;
-            std::vector<double> t[6];   // eri4, eri3, eri2, eri4d1, eri3d1, eri2d1
+        };
 
-            const std::vector<double> &get(int which)
+        /// The blocks of one kernel functor's class pair, computed on the GPU
+        /// (lbx_eri*_batch) in chunks: a run of consecutive first-side indices against every
+        /// second-side index, the most recently used chunks kept on the host.
+        struct BlockCache
+        {
+            std::shared_ptr<Tensors> t;
+            int (*fn)(lbx_context *, long, const int *, double *) = nullptr;
+            int per1 = 0, per2 = 0;          // shells per first-side / second-side index
+            std::vector<int> first, second;  // their shell indices
+            size_t n1 = 0, n2 = 0, block = 0, chunk = 1;
+            std::mutex mu;
+            std::list<std::pair<size_t, std::shared_ptr<const std::vector<double>>>> lru;
+
+            static constexpr size_t kChunkDoubles = size_t(4) << 20;   // 32 MiB per chunk
+            static constexpr size_t kChunks = 8;
+
+            void init(size_t block_doubles)
+            {
+                block = block_doubles;
+                const size_t row = std::max<size_t>(1, n2 * block);
+                chunk = std::max<size_t>(1, kChunkDoubles / row);
+            }
+
+            std::shared_ptr<const std::vector<double>> get(size_t c)
             {
                 std::lock_guard<std::mutex> lk(mu);
-                std::vector<double> &v = t[which];
-                if (!v.empty())
-                    return v;
-                const size_t n2 = n * n, sizes[6] = {n2 * n2, n2 * naux, naux * naux, 12 * n2 * n2,
-                                                     9 * n2 * naux, 6 * naux * naux};
-                using Fn = int (*)(lbx_context *, double *);
-                const Fn fns[6] = {lbx_eri4, lbx_eri3, lbx_eri2, lbx_eri4d1, lbx_eri3d1, lbx_eri2d1};
-                lbx_context *ctx = source->context();
-                DeviceArray d(sizes[which]);
-                check(fns[which](ctx, d.get()));
-                check(lbx_synchronize(ctx));
-                v.resize(sizes[which]);
-                d.toHost(v.data());
+                for (auto it = lru.begin(); it != lru.end(); ++it)
+                    if (it->first == c)
+                    {
+                        lru.splice(lru.begin(), lru, it);
+                        return lru.front().second;
+                    }
+                const size_t lo = c * chunk, hi = std::min(n1, lo + chunk);
+                const size_t ne = (hi - lo) * n2, per = per1 + per2;
+                std::vector<int> sh(ne * per);
+                size_t e = 0;
+                for (size_t i = lo; i < hi; i++)
+                    for (size_t j = 0; j < n2; j++, e++)
+                    {
+                        for (int k = 0; k < per1; k++)
+                            sh[e * per + k] = first[i * per1 + k];
+                        for (int k = 0; k < per2; k++)
+                            sh[e * per + per1 + k] = second[j * per2 + k];
+                    }
+                auto v = std::make_shared<std::vector<double>>(ne * block);
+                if (ne > 0)
+                {
+                    lbx_context *ctx = t->source->context();
+                    DeviceArray d(ne * block);
+                    check(fn(ctx, static_cast<long>(ne), sh.data(), d.get()));
+                    check(lbx_synchronize(ctx));
+                    d.toHost(v->data());
+                }
+                lru.emplace_front(c, v);
+                if (lru.size() > kChunks)
+                    lru.pop_back();
                 return v;
+            }
+
+            // the block of (i, j): `block` doubles
+            std::pair<std::shared_ptr<const std::vector<double>>, size_t> at(size_t i, size_t j)
+            {
+                auto v = get(i / chunk);
+                return {v, ((i % chunk) * n2 + j) * block};
             }
         };
     }
@@ -680,12 +743,11 @@ namespace lible::ints
         }
 
         template <class A, class B>
-        std::shared_ptr<detail::Tensors> sameTensors(const A &a, const B &b, int which, const char *who)
+        std::shared_ptr<detail::Tensors> sameTensors(const A &a, const B &b, const char *who)
         {
             auto ta = tensorsOf(a, who), tb = tensorsOf(b, who);
             if (ta != tb)
                 throw std::runtime_error(std::string(who) + ": shell data of different structures");
-            ta->get(which);
             return ta;
         }
 
@@ -695,45 +757,46 @@ namespace lible::ints
                 throw std::out_of_range(std::string(who) + ": index out of range");
         }
 
-        // block (S_a, S_b, S_c, S_d) of a dense (n, n, n, n) tensor
-        vec4d block4(const double *t, size_t n, size_t oa, size_t ob, size_t oc, size_t od, int la,
-                     int lb, int lc, int ld)
+        std::vector<int> shellList(const std::vector<size_t> &idx)
         {
-            const size_t na = 2 * la + 1, nb = 2 * lb + 1, nc = 2 * lc + 1, nd = 2 * ld + 1;
-            vec4d r(na, nb, nc, nd);
-            for (size_t a = 0; a < na; a++)
-                for (size_t b = 0; b < nb; b++)
-                    for (size_t c = 0; c < nc; c++)
-                        for (size_t d = 0; d < nd; d++)
-                            r(a, b, c, d) = t[(((oa + a) * n + ob + b) * n + oc + c) * n + od + d];
-            return r;
+            return std::vector<int>(idx.begin(), idx.end());
         }
 
-        vec3d block3(const double *t, size_t n, size_t naux, size_t oa, size_t ob, size_t oc, int la,
-                     int lb, int lc)
+        size_t nsph(int l) { return static_cast<size_t>(2 * l + 1); }
+
+        // The functor's cache: first side = bra pairs (or auxiliary shells), second side = ket
+        // pairs (or auxiliary shells); ncomp components per entry.
+        std::shared_ptr<detail::BlockCache> makeCache(std::shared_ptr<detail::Tensors> t,
+                                                      int (*fn)(lbx_context *, long, const int *, double *),
+                                                      std::vector<int> first, int per1, size_t n1,
+                                                      std::vector<int> second, int per2, size_t n2,
+                                                      size_t block)
         {
-            const size_t na = 2 * la + 1, nb = 2 * lb + 1, nc = 2 * lc + 1;
-            vec3d r(na, nb, nc);
-            for (size_t a = 0; a < na; a++)
-                for (size_t b = 0; b < nb; b++)
-                    for (size_t c = 0; c < nc; c++)
-                        r(a, b, c) = t[((oa + a) * n + ob + b) * naux + oc + c];
-            return r;
+            auto c = std::make_shared<detail::BlockCache>();
+            c->t = std::move(t);
+            c->fn = fn;
+            c->first = std::move(first);
+            c->second = std::move(second);
+            c->per1 = per1;
+            c->per2 = per2;
+            c->n1 = n1;
+            c->n2 = n2;
+            c->init(block);
+            return c;
         }
 
-        vec2d block2(const double *t, size_t naux, size_t oa, size_t ob, int la, int lb)
+        template <class V>
+        V copyBlock(V r, const std::pair<std::shared_ptr<const std::vector<double>>, size_t> &b, size_t comp)
         {
-            const size_t na = 2 * la + 1, nb = 2 * lb + 1;
-            vec2d r(na, nb);
-            for (size_t a = 0; a < na; a++)
-                for (size_t b = 0; b < nb; b++)
-                    r(a, b) = t[(oa + a) * naux + ob + b];
+            std::copy_n(b.first->data() + b.second + comp * r.size(), r.size(), r.memptr());
             return r;
         }
     }
 
     ERI4Kernel::ERI4Kernel(const ShellPairData &ab, const ShellPairData &cd)
-        : t_(sameTensors(ab, cd, 0, "ERI4Kernel"))
+        : c_(makeCache(sameTensors(ab, cd, "ERI4Kernel"), lbx_eri4_batch, shellList(ab.shell_idxs_), 2,
+                       ab.n_pairs_, shellList(cd.shell_idxs_), 2, cd.n_pairs_,
+                       nsph(ab.la_) * nsph(ab.lb_) * nsph(cd.la_) * nsph(cd.lb_)))
     {
     }
 
@@ -741,12 +804,13 @@ namespace lible::ints
     {
         checkIndex(p, ab.n_pairs_, "ERI4Kernel");
         checkIndex(q, cd.n_pairs_, "ERI4Kernel");
-        return block4(t_->t[0].data(), t_->n, ab.offsets_sph_[2 * p], ab.offsets_sph_[2 * p + 1],
-                      cd.offsets_sph_[2 * q], cd.offsets_sph_[2 * q + 1], ab.la_, ab.lb_, cd.la_, cd.lb_);
+        return copyBlock(vec4d(nsph(ab.la_), nsph(ab.lb_), nsph(cd.la_), nsph(cd.lb_)), c_->at(p, q), 0);
     }
 
     ERI3Kernel::ERI3Kernel(const ShellPairData &ab, const ShellData &c)
-        : t_(sameTensors(ab, c, 1, "ERI3Kernel"))
+        : c_(makeCache(sameTensors(ab, c, "ERI3Kernel"), lbx_eri3_batch, shellList(ab.shell_idxs_), 2,
+                       ab.n_pairs_, shellList(c.shell_idxs_), 1, c.n_shells_,
+                       nsph(ab.la_) * nsph(ab.lb_) * nsph(c.l_)))
     {
     }
 
@@ -754,11 +818,12 @@ namespace lible::ints
     {
         checkIndex(p, ab.n_pairs_, "ERI3Kernel");
         checkIndex(k, c.n_shells_, "ERI3Kernel");
-        return block3(t_->t[1].data(), t_->n, t_->naux, ab.offsets_sph_[2 * p], ab.offsets_sph_[2 * p + 1],
-                      c.offsets_sph_[k], ab.la_, ab.lb_, c.l_);
+        return copyBlock(vec3d(nsph(ab.la_), nsph(ab.lb_), nsph(c.l_)), c_->at(p, k), 0);
     }
 
-    ERI2Kernel::ERI2Kernel(const ShellData &a, const ShellData &b) : t_(sameTensors(a, b, 2, "ERI2Kernel"))
+    ERI2Kernel::ERI2Kernel(const ShellData &a, const ShellData &b)
+        : c_(makeCache(sameTensors(a, b, "ERI2Kernel"), lbx_eri2_batch, shellList(a.shell_idxs_), 1,
+                       a.n_shells_, shellList(b.shell_idxs_), 1, b.n_shells_, nsph(a.l_) * nsph(b.l_)))
     {
     }
 
@@ -766,11 +831,13 @@ namespace lible::ints
     {
         checkIndex(i, a.n_shells_, "ERI2Kernel");
         checkIndex(j, b.n_shells_, "ERI2Kernel");
-        return block2(t_->t[2].data(), t_->naux, a.offsets_sph_[i], b.offsets_sph_[j], a.l_, b.l_);
+        return copyBlock(vec2d(nsph(a.l_), nsph(b.l_)), c_->at(i, j), 0);
     }
 
     ERI4D1Kernel::ERI4D1Kernel(const ShellPairData &ab, const ShellPairData &cd)
-        : t_(sameTensors(ab, cd, 3, "ERI4D1Kernel"))
+        : c_(makeCache(sameTensors(ab, cd, "ERI4D1Kernel"), lbx_eri4d1_batch, shellList(ab.shell_idxs_), 2,
+                       ab.n_pairs_, shellList(cd.shell_idxs_), 2, cd.n_pairs_,
+                       12 * nsph(ab.la_) * nsph(ab.lb_) * nsph(cd.la_) * nsph(cd.lb_)))
     {
     }
 
@@ -779,16 +846,17 @@ namespace lible::ints
     {
         checkIndex(p, ab.n_pairs_, "ERI4D1Kernel");
         checkIndex(q, cd.n_pairs_, "ERI4D1Kernel");
-        const size_t n = t_->n, n4 = n * n * n * n;
+        const auto b = c_->at(p, q);
         std::array<vec4d, 12> r;
-        for (int c = 0; c < 12; c++)
-            r[c] = block4(t_->t[3].data() + c * n4, n, ab.offsets_sph_[2 * p], ab.offsets_sph_[2 * p + 1],
-                          cd.offsets_sph_[2 * q], cd.offsets_sph_[2 * q + 1], ab.la_, ab.lb_, cd.la_, cd.lb_);
+        for (size_t x = 0; x < 12; x++)
+            r[x] = copyBlock(vec4d(nsph(ab.la_), nsph(ab.lb_), nsph(cd.la_), nsph(cd.lb_)), b, x);
         return r;
     }
 
     ERI3D1Kernel::ERI3D1Kernel(const ShellPairData &ab, const ShellData &c)
-        : t_(sameTensors(ab, c, 4, "ERI3D1Kernel"))
+        : c_(makeCache(sameTensors(ab, c, "ERI3D1Kernel"), lbx_eri3d1_batch, shellList(ab.shell_idxs_), 2,
+                       ab.n_pairs_, shellList(c.shell_idxs_), 1, c.n_shells_,
+                       9 * nsph(ab.la_) * nsph(ab.lb_) * nsph(c.l_)))
     {
     }
 
@@ -797,16 +865,16 @@ namespace lible::ints
     {
         checkIndex(p, ab.n_pairs_, "ERI3D1Kernel");
         checkIndex(k, c.n_shells_, "ERI3D1Kernel");
-        const size_t n = t_->n, n3 = n * n * t_->naux;
+        const auto b = c_->at(p, k);
         std::array<vec3d, 9> r;
-        for (int x = 0; x < 9; x++)
-            r[x] = block3(t_->t[4].data() + x * n3, n, t_->naux, ab.offsets_sph_[2 * p],
-                          ab.offsets_sph_[2 * p + 1], c.offsets_sph_[k], ab.la_, ab.lb_, c.l_);
+        for (size_t x = 0; x < 9; x++)
+            r[x] = copyBlock(vec3d(nsph(ab.la_), nsph(ab.lb_), nsph(c.l_)), b, x);
         return r;
     }
 
     ERI2D1Kernel::ERI2D1Kernel(const ShellData &a, const ShellData &b)
-        : t_(sameTensors(a, b, 5, "ERI2D1Kernel"))
+        : c_(makeCache(sameTensors(a, b, "ERI2D1Kernel"), lbx_eri2d1_batch, shellList(a.shell_idxs_), 1,
+                       a.n_shells_, shellList(b.shell_idxs_), 1, b.n_shells_, 6 * nsph(a.l_) * nsph(b.l_)))
     {
     }
 
@@ -815,10 +883,10 @@ namespace lible::ints
     {
         checkIndex(i, a.n_shells_, "ERI2D1Kernel");
         checkIndex(j, b.n_shells_, "ERI2D1Kernel");
-        const size_t n2 = t_->naux * t_->naux;
+        const auto blk = c_->at(i, j);
         std::array<vec2d, 6> r;
-        for (int x = 0; x < 6; x++)
-            r[x] = block2(t_->t[5].data() + x * n2, t_->naux, a.offsets_sph_[i], b.offsets_sph_[j], a.l_, b.l_);
+        for (size_t x = 0; x < 6; x++)
+            r[x] = copyBlock(vec2d(nsph(a.l_), nsph(b.l_)), blk, x);
         return r;
     }
 }

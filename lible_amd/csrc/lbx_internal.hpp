@@ -188,6 +188,11 @@ namespace lbx
         double *out;
         int kind;                    // 4: (12,n,n,n,n); 3: (9,n,n,naux); 2: (6,naux,naux)
         long long nbf, naux;
+        // Block output (lbx_eri*_batch): tasks are (bra pair, ket pair, flags, slot) and task t
+        // writes its block(s) to out + slot * ncomp * Sa Sb Sc Sd in the caller's shell order
+        // (flags bit 0: the caller's bra shells are the pair's (b, a); bit 1: the ket's (d, c)).
+        int blocks;
+        int plain;                   // the integrals themselves (one component), no derivative
     };
     void launchD1(const D1Args &a, hipStream_t s);
 }

@@ -20,7 +20,9 @@ LBX_MIXED = 1
 # every symbol include/lbx.h declares
 SYMBOLS = ["lbx_create", "lbx_destroy", "lbx_last_error", "lbx_dims", "lbx_set_stream",
            "lbx_synchronize", "lbx_boys", "lbx_eri4", "lbx_eri4_diag", "lbx_eri3", "lbx_eri2",
-           "lbx_eri2_diag", "lbx_eri4d1", "lbx_eri3d1", "lbx_eri2d1", "lbx_jk", "lbx_create_batch", "lbx_ri_sums", "lbx_jk_grad",
+           "lbx_eri2_diag", "lbx_eri4d1", "lbx_eri3d1", "lbx_eri2d1", "lbx_eri4_batch",
+           "lbx_eri3_batch", "lbx_eri2_batch", "lbx_eri4d1_batch", "lbx_eri3d1_batch",
+           "lbx_eri2d1_batch", "lbx_jk", "lbx_create_batch", "lbx_ri_sums", "lbx_jk_grad",
            "lbx_tune", "lbx_set_variant", "lbx_profile_enable",
            "lbx_profile_read", "lbx_set_class_filter"]
 VARIANTS = ("valu-bra", "valu-ket", "mfma", "valu-bra-f32", "valu-ket-f32", "mfma-f32", "block",
@@ -80,6 +82,9 @@ def lib():
         for name in ("lbx_eri4", "lbx_eri4_diag", "lbx_eri3", "lbx_eri2", "lbx_eri2_diag",
                      "lbx_eri4d1", "lbx_eri3d1", "lbx_eri2d1"):
             getattr(L, name).argtypes = [p, p]
+        for name in ("lbx_eri4_batch", "lbx_eri3_batch", "lbx_eri2_batch", "lbx_eri4d1_batch",
+                     "lbx_eri3d1_batch", "lbx_eri2d1_batch"):
+            getattr(L, name).argtypes = [p, l, p, p]
         L.lbx_jk.argtypes = [p, p, p, p, d, i, d, ctypes.POINTER(JKStats)]
         L.lbx_jk_grad.argtypes = [p, p, p, d, i, d, ctypes.POINTER(JKStats)]
         L.lbx_create_batch.argtypes = [ctypes.POINTER(p), i, ctypes.POINTER(_Basis),
@@ -124,6 +129,7 @@ class Context:
         self._keep = []
         main = self._basis(mol.coords, mol.shells, mol.natoms)
         aux = self._basis(mol.coords, mol.aux, mol.natoms) if mol.aux is not None else None
+        self._ls = (list(mol.shells.l), list(mol.aux.l) if mol.aux is not None else None)
         h = ctypes.c_void_p()
         _check(lib().lbx_create(ctypes.byref(h), ctypes.byref(main),
                                 ctypes.byref(aux) if aux is not None else None))
@@ -136,6 +142,7 @@ class Context:
         _torch()
         self = cls.__new__(cls)
         self._keep = []
+        self._ls = (None, None)
         n = len(mols)
         mains, auxs = (_Basis * n)(), (_Basis * n)()
         for k, m in enumerate(mols):
@@ -237,6 +244,33 @@ class Context:
         """(6, naux, naux) (lbx_eri2d1)."""
         out = self._out((6, self.nbf_aux, self.nbf_aux))
         _check(lib().lbx_eri2d1(self.h, out.data_ptr()))
+        return out
+
+    def blocks(self, shells, deriv: bool = False):
+        """Blocks of listed shell quartets (n, 4), triplets (n, 3: main, main, auxiliary) or
+        doublets (n, 2: auxiliary) -> tensor (n, ncomp, S_a, S_b[, S_c[, S_d]]) with ncomp 1 for
+        the integrals, 12 / 9 / 6 for their derivatives (lbx_eri*_batch)."""
+        sh = np.ascontiguousarray(shells, dtype=np.int32)
+        if sh.ndim != 2 or sh.shape[1] not in (2, 3, 4):
+            raise LbxError("shells must be (n, 4), (n, 3) or (n, 2)")
+        per = sh.shape[1]
+        ls_main = np.asarray(self._ls[0]) if self._ls[0] is not None else None
+        ls_aux = np.asarray(self._ls[1]) if self._ls[1] is not None else None
+        dims = []
+        for k in range(per):
+            aux = per == 2 or (per == 3 and k == 2)
+            ls = ls_aux if aux else ls_main
+            if ls is None or sh.shape[0] == 0:
+                dims.append(1)
+            else:
+                idx = int(sh[0, k])
+                if not 0 <= idx < len(ls):
+                    raise LbxError("shell index out of range")
+                dims.append(2 * int(ls[idx]) + 1)
+        ncomp = (12, 9, 6)[4 - per] if deriv else 1
+        out = self._out((sh.shape[0], ncomp, *dims))
+        name = {4: "lbx_eri4", 3: "lbx_eri3", 2: "lbx_eri2"}[per] + ("d1_batch" if deriv else "_batch")
+        _check(getattr(lib(), name)(self.h, sh.shape[0], sh.ctypes.data, out.data_ptr()))
         return out
 
     def jk_grad(self, D, schwarz: float = 1e-12, grad=None, precision: int = LBX_FP64,

@@ -1,6 +1,10 @@
 // Oracle (test infrastructure only): C entry points for ctypes.  Loaded by tests/ and by
 // bench.py's cpu_baseline leg; never by the product.
 #include "oracle.hpp"
+#include <type_traits>
+#include <algorithm>
+#include <memory>
+#include <map>
 
 #include <cmath>
 #include <cstring>
@@ -9,6 +13,15 @@
 #include <string>
 
 using namespace orc;
+
+namespace
+{
+    template <class T>
+    struct Tag
+    {
+        using type = T;
+    };
+}
 
 namespace
 {
@@ -417,6 +430,173 @@ extern "C"
                                  }
                          }
                          *sum = tot;
+                     });
+    }
+
+    // Sums of |x| over the blocks of the second-derivative and spin-orbit kernels, driven as the
+    // reference's deployERI2D2Kernel / deployERI4SOCKernel / deployERI3SOCKernel
+    // (tests_ints.cpp:1055-1148): which 6 eri2d2, 7 eri4soc, 8 eri3soc; all ordered pairs.
+    int orc_deploy_sum2(void *h, int which, double *sum)
+    {
+        return guard([&]
+                     {
+                         const Structure &s = *static_cast<Structure *>(h);
+                         double tot = 0;
+                         if (which == 6)
+                         {
+                             auto sh = shellDataAux(s);
+                             for (auto &A : sh)
+                                 for (auto &B : sh)
+                                 {
+                                     ERI2D2Kernel k(A, B);
+                                     for (size_t p = 0; p < A.n_shells; p++)
+                                         for (size_t q = 0; q < B.n_shells; q++)
+                                             tot += absSum(k(p, q, A, B));
+                                 }
+                         }
+                         else if (which == 7)
+                         {
+                             auto sp = shellPairData(false, s);
+                             for (auto &A : sp)
+                                 for (auto &C : sp)
+                                 {
+                                     ERI4SOCKernel k(A, C);
+                                     for (size_t p = 0; p < A.n_pairs; p++)
+                                         for (size_t q = 0; q < C.n_pairs; q++)
+                                             tot += absSum(k(p, q, A, C));
+                                 }
+                         }
+                         else if (which == 8)
+                         {
+                             auto sp = shellPairData(false, s);
+                             auto sh = shellDataAux(s);
+                             for (auto &A : sp)
+                                 for (auto &C : sh)
+                                 {
+                                     ERI3SOCKernel k(A, C);
+                                     for (size_t p = 0; p < A.n_pairs; p++)
+                                         for (size_t q = 0; q < C.n_shells; q++)
+                                             tot += absSum(k(p, q, A, C));
+                                 }
+                         }
+                         else
+                             throw std::runtime_error("orc_deploy_sum2: unknown kernel");
+                         *sum = tot;
+                     });
+    }
+
+    // Blocks of listed shell quartets / triplets / doublets in the caller's shell order, as the
+    // per-block kernels return them for the ordered pairs of shellPairData(false):
+    // which 0 eri4, 1 eri3, 2 eri2, 3 eri4d1, 4 eri3d1, 5 eri2d1, 6 eri2d2, 7 eri4soc,
+    // 8 eri3soc; shells: n entries of 4 / 3 / 2 shell indices (main, main, main, main / main,
+    // main, aux / aux, aux); out: n consecutive kernel outputs.  Entries whose pair has no
+    // primitive pair above the screen are zero.
+    int orc_blocks(void *h, int which, long n, const int *shells, double *out)
+    {
+        return guard([&]
+                     {
+                         const Structure &s = *static_cast<Structure *>(h);
+                         const int kind = (which == 0 || which == 3 || which == 7) ? 4
+                                                                                   : (which == 1 || which == 4 || which == 8) ? 3 : 2;
+                         auto sp = shellPairData(false, s);
+                         auto sh = kind == 4 ? std::vector<ShellData>{} : shellDataAux(s);
+                         std::map<std::pair<size_t, size_t>, std::pair<int, size_t>> pairAt;
+                         for (size_t c = 0; c < sp.size(); c++)
+                             for (size_t p = 0; p < sp[c].n_pairs; p++)
+                                 pairAt[{sp[c].shell_idxs[2 * p], sp[c].shell_idxs[2 * p + 1]}] = {int(c), p};
+                         std::map<size_t, std::pair<int, size_t>> auxAt;
+                         for (size_t c = 0; c < sh.size(); c++)
+                             for (size_t p = 0; p < sh[c].n_shells; p++)
+                                 auxAt[sh[c].shell_idxs[p]] = {int(c), p};
+                         const int per = kind;
+                         auto lOf = [&](int pos, int idx) {
+                             const bool aux = kind == 2 || (kind == 3 && pos == 2);
+                             const auto &v = aux ? s.shells_aux : s.shells;
+                             if (idx < 0 || static_cast<size_t>(idx) >= v.size())
+                                 throw std::runtime_error("orc_blocks: shell index out of range");
+                             return v[idx].l;
+                         };
+                         size_t bsize = 1;
+                         for (int k = 0; k < per; k++)
+                             bsize *= static_cast<size_t>(2 * lOf(k, shells[k]) + 1);
+                         const size_t ncomp = which < 3 ? 1 : which < 6 ? (which == 3 ? 12 : which == 4 ? 9 : 6)
+                                                                        : which == 6 ? 36 : 3;
+                         const size_t stride = ncomp * bsize;
+                         std::map<std::pair<int, int>, std::shared_ptr<void>> kernels;
+                         auto kern = [&](auto tag, int ca, int cb, auto make) {
+                             using K = typename decltype(tag)::type;
+                             auto &slot = kernels[{ca, cb}];
+                             if (!slot)
+                                 slot = std::shared_ptr<void>(make(), [](void *q) { delete static_cast<K *>(q); });
+                             return static_cast<K *>(slot.get());
+                         };
+                         for (long t = 0; t < n; t++)
+                         {
+                             const int *e = shells + per * t;
+                             double *dst = out + t * stride;
+                             std::fill(dst, dst + stride, 0.0);
+                             std::vector<double> blk;
+                             if (kind == 2)
+                             {
+                                 (void)lOf(0, e[0]);
+                                 (void)lOf(1, e[1]);
+                                 auto a = auxAt.at(e[0]), b = auxAt.at(e[1]);
+                                 const ShellData &A = sh[a.first], &B = sh[b.first];
+                                 if (which == 2)
+                                     blk = (*kern(Tag<ERI2Kernel>{}, a.first, b.first,
+                                                  [&] { return new ERI2Kernel(A, B); }))(a.second, b.second, A, B);
+                                 else if (which == 5)
+                                     blk = (*kern(Tag<ERI2D1Kernel>{}, a.first, b.first,
+                                                  [&] { return new ERI2D1Kernel(A, B); }))(a.second, b.second, A, B);
+                                 else
+                                     blk = (*kern(Tag<ERI2D2Kernel>{}, a.first, b.first,
+                                                  [&] { return new ERI2D2Kernel(A, B); }))(a.second, b.second, A, B);
+                             }
+                             else
+                             {
+                                 for (int k = 0; k < per; k++)
+                                     (void)lOf(k, e[k]);
+                                 auto ia = pairAt.find({static_cast<size_t>(e[0]), static_cast<size_t>(e[1])});
+                                 if (ia == pairAt.end())
+                                     continue;
+                                 const ShellPairData &A = sp[ia->second.first];
+                                 const size_t pa = ia->second.second;
+                                 if (kind == 3)
+                                 {
+                                     auto c = auxAt.at(e[2]);
+                                     const ShellData &C = sh[c.first];
+                                     if (which == 1)
+                                         blk = (*kern(Tag<ERI3Kernel>{}, ia->second.first, c.first,
+                                                      [&] { return new ERI3Kernel(A, C); }))(pa, c.second, A, C);
+                                     else if (which == 4)
+                                         blk = (*kern(Tag<ERI3D1Kernel>{}, ia->second.first, c.first,
+                                                      [&] { return new ERI3D1Kernel(A, C); }))(pa, c.second, A, C);
+                                     else
+                                         blk = (*kern(Tag<ERI3SOCKernel>{}, ia->second.first, c.first,
+                                                      [&] { return new ERI3SOCKernel(A, C); }))(pa, c.second, A, C);
+                                 }
+                                 else
+                                 {
+                                     auto ic = pairAt.find({static_cast<size_t>(e[2]), static_cast<size_t>(e[3])});
+                                     if (ic == pairAt.end())
+                                         continue;
+                                     const ShellPairData &C = sp[ic->second.first];
+                                     const size_t pc = ic->second.second;
+                                     if (which == 0)
+                                         blk = (*kern(Tag<ERI4Kernel>{}, ia->second.first, ic->second.first,
+                                                      [&] { return new ERI4Kernel(A, C); }))(pa, pc, A, C);
+                                     else if (which == 3)
+                                         blk = (*kern(Tag<ERI4D1Kernel>{}, ia->second.first, ic->second.first,
+                                                      [&] { return new ERI4D1Kernel(A, C); }))(pa, pc, A, C);
+                                     else
+                                         blk = (*kern(Tag<ERI4SOCKernel>{}, ia->second.first, ic->second.first,
+                                                      [&] { return new ERI4SOCKernel(A, C); }))(pa, pc, A, C);
+                                 }
+                             }
+                             if (blk.size() != stride)
+                                 throw std::runtime_error("orc_blocks: entries of different angular momenta");
+                             std::copy(blk.begin(), blk.end(), dst);
+                         }
                      });
     }
 

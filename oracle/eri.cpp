@@ -2,6 +2,7 @@
 // derivative) and the dense drivers, restated from the reference generic-L code paths.
 #include "oracle.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <stdexcept>
 
@@ -397,6 +398,196 @@ namespace orc
         for (int k = 3; k < 6; k++)
             for (size_t i = 0; i < nsab; i++)
                 out[k * nsab + i] = -1 * out[(k - 3) * nsab + i];
+        return out;
+    }
+
+    // ---------------------------------------------------------------- ERI2 D2
+    // eri_kernels.cpp:1230-1249
+    ERI2D2Kernel::ERI2D2Kernel(const ShellData &a, const ShellData &b)
+        : e_bra(ecoeffsSHARK(a, false)), e_ket(ecoeffsSHARK(b, true)), grid(a.l + b.l + 2)
+    {
+    }
+
+    // eri_kernel_funs.cpp:343-433, R of rints.cpp:228-270: the six second-order shifts
+    // 200, 110, 101, 020, 011, 002 of the Hermite sum; AA from them, AB = BA = -AA, BB = AA
+    std::vector<double> ERI2D2Kernel::operator()(size_t isa, size_t isb, const ShellData &A,
+                                                 const ShellData &B) const
+    {
+        int la = A.l, lb = B.l, L = la + lb;
+        int nsa = nSph(la), nsb = nSph(lb), nha = nHerm(la), nhb = nHerm(lb);
+        int nea = nsa * nha, neb = nsb * nhb;
+        size_t nr = static_cast<size_t>(nha) * nhb, nrxe = static_cast<size_t>(nha) * nsb;
+        size_t nsab = static_cast<size_t>(nsa) * nsb;
+        size_t Ka = A.cdepths[isa], Kb = B.cdepths[isb];
+        const double *ea = &e_bra[A.offsets_ecoeffs[isa]];
+        const double *eb = &e_ket[B.offsets_ecoeffs[isb]];
+        const double *exa = &A.exps[A.coffsets[isa]], *exb = &B.exps[B.coffsets[isb]];
+        const double *xa = &A.coords[3 * isa], *xb = &B.coords[3 * isb];
+        auto hb = hermiteIdxs(la), hk = hermiteIdxs(lb);
+        double AB[3] = {xa[0] - xb[0], xa[1] - xb[1], xa[2] - xb[2]};
+        double r2 = AB[0] * AB[0] + AB[1] * AB[1] + AB[2] * AB[2];
+        static const int shifts[6][3] = {{2, 0, 0}, {1, 1, 0}, {1, 0, 1}, {0, 2, 0}, {0, 1, 1}, {0, 0, 2}};
+        static const int slot[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+        std::vector<double> aa(9 * nsab, 0.0);   // AA[i][j] for i <= j
+        std::vector<double> fnx(L + 3), R(6 * nr);
+        for (size_t ia = 0; ia < Ka; ia++)
+        {
+            std::vector<double> RxE(6 * nrxe, 0.0);
+            for (size_t ib = 0; ib < Kb; ib++)
+            {
+                double a = exa[ia], b = exb[ib], alpha = a * b / (a + b);
+                calcBoysF(L + 2, alpha * r2, grid, fnx.data());
+                double fac = k2pi52 / (a * b * std::sqrt(a + b));
+                auto r3 = calcRInts3D(L + 2, alpha, AB, fnx.data());
+                for (int k = 0; k < 6; k++)
+                {
+                    rMatrix(r3, L + 3, fac, hb, hk, shifts[k][0], shifts[k][1], shifts[k][2], &R[k * nr]);
+                    gemmAcc(nha, nsb, nhb, &R[k * nr], &eb[ib * neb], &RxE[k * nrxe]);
+                }
+            }
+            for (int k = 0; k < 6; k++)
+                gemmAcc(nsa, nsb, nha, &ea[ia * nea], &RxE[k * nrxe],
+                        &aa[(3 * slot[k][0] + slot[k][1]) * nsab]);
+        }
+        for (int i = 0; i < 3; i++)
+            for (int j = i + 1; j < 3; j++)
+                std::copy_n(&aa[(3 * i + j) * nsab], nsab, &aa[(3 * j + i) * nsab]);
+        std::vector<double> out(36 * nsab, 0.0);
+        for (int i = 0; i < 6; i++)
+            for (int j = 0; j < 6; j++)
+            {
+                const double sg = ((i < 3) == (j < 3)) ? 1.0 : -1.0;
+                const double *src = &aa[(3 * (i % 3) + (j % 3)) * nsab];
+                for (size_t e = 0; e < nsab; e++)
+                    out[(6 * i + j) * nsab + e] = sg * src[e];
+            }
+        return out;
+    }
+
+    // ---------------------------------------------------------------- ERI4 SOC
+    // eri_kernels.cpp:1251-1270
+    ERI4SOCKernel::ERI4SOCKernel(const ShellPairData &ab, const ShellPairData &cd)
+        : e1_bra(ecoeffsD1SHARK(ab, false)), e0_ket(ecoeffsSHARK(cd, true)),
+          grid(ab.la + ab.lb + cd.la + cd.lb + 1)
+    {
+    }
+
+    namespace
+    {
+        // The bra's E' (d/d(A-B), per axis) against the ket-transformed R shifted by one unit
+        // along each axis (calcRInts_ERISOC, rints.cpp:272-308): component x = E'_y R_z - E'_z R_y,
+        // y = E'_z R_x - E'_x R_z, z = E'_x R_y - E'_y R_x (eri_kernel_funs.cpp:932-971)
+        void socBra(int nsab, int nscd, int nhab, const double *e1, size_t neab, const double *RxE,
+                    size_t nrxe, size_t nsabcd, double *out)
+        {
+            static const int ax[3][2] = {{1, 2}, {2, 0}, {0, 1}};
+            for (int k = 0; k < 3; k++)
+            {
+                std::vector<double> p(nsabcd, 0.0), m(nsabcd, 0.0);
+                gemmAcc(nsab, nscd, nhab, &e1[ax[k][0] * neab], &RxE[ax[k][1] * nrxe], p.data());
+                gemmAcc(nsab, nscd, nhab, &e1[ax[k][1] * neab], &RxE[ax[k][0] * nrxe], m.data());
+                axpy(nsabcd, 1.0, p.data(), &out[k * nsabcd]);
+                axpy(nsabcd, -1.0, m.data(), &out[k * nsabcd]);
+            }
+        }
+    }
+
+    // eri_kernel_funs.cpp:846-974
+    std::vector<double> ERI4SOCKernel::operator()(size_t ipab, size_t ipcd, const ShellPairData &ab,
+                                                  const ShellPairData &cd) const
+    {
+        int lab = ab.la + ab.lb, lcd = cd.la + cd.lb, L = lab + lcd;
+        int nsab = nSph(ab.la) * nSph(ab.lb), nscd = nSph(cd.la) * nSph(cd.lb);
+        size_t nsabcd = static_cast<size_t>(nsab) * nscd;
+        int nhab = nHerm(lab), nhcd = nHerm(lcd);
+        size_t nr = static_cast<size_t>(nhab) * nhcd, nrxe = static_cast<size_t>(nhab) * nscd;
+        size_t neab = static_cast<size_t>(nsab) * nhab;
+        int necd = nscd * nhcd;
+        const double *ex_ab = &ab.exps[ab.offsets_prims[ipab]];
+        const double *ex_cd = &cd.exps[cd.offsets_prims[ipcd]];
+        const double *xa = &ab.coords[6 * ipab], *xb = &ab.coords[6 * ipab + 3];
+        const double *xc = &cd.coords[6 * ipcd], *xd = &cd.coords[6 * ipcd + 3];
+        const double *e1ab = &e1_bra[ab.offsets_ecoeffs_d1[ipab]];
+        const double *e0cd = &e0_ket[cd.offsets_ecoeffs[ipcd]];
+        auto hb = hermiteIdxs(lab), hk = hermiteIdxs(lcd);
+        std::vector<double> out(3 * nsabcd, 0.0);
+        std::vector<double> fnx(L + 2), R(3 * nr);
+        for (size_t iab = 0; iab < ab.nrs_ppairs[ipab]; iab++)
+        {
+            double a = ex_ab[2 * iab], b = ex_ab[2 * iab + 1], p = a + b;
+            std::vector<double> RxE(3 * nrxe, 0.0);
+            for (size_t icd = 0; icd < cd.nrs_ppairs[ipcd]; icd++)
+            {
+                double c = ex_cd[2 * icd], d = ex_cd[2 * icd + 1];
+                double q = c + d, alpha = p * q / (p + q);
+                double PQ[3];
+                for (int i = 0; i < 3; i++)
+                    PQ[i] = (a * xa[i] + b * xb[i]) / p - (c * xc[i] + d * xd[i]) / q;
+                double x = alpha * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
+                calcBoysF(L + 1, x, grid, fnx.data());
+                double fac = k2pi52 / (p * q * std::sqrt(p + q));
+                auto r3 = calcRInts3D(L + 1, alpha, PQ, fnx.data());
+                rMatrix(r3, L + 2, fac, hb, hk, 1, 0, 0, &R[0 * nr]);
+                rMatrix(r3, L + 2, fac, hb, hk, 0, 1, 0, &R[1 * nr]);
+                rMatrix(r3, L + 2, fac, hb, hk, 0, 0, 1, &R[2 * nr]);
+                for (int k = 0; k < 3; k++)
+                    gemmAcc(nhab, nscd, nhcd, &R[k * nr], &e0cd[icd * necd], &RxE[k * nrxe]);
+            }
+            socBra(nsab, nscd, nhab, &e1ab[3 * iab * neab], neab, RxE.data(), nrxe, nsabcd, out.data());
+        }
+        return out;
+    }
+
+    // ---------------------------------------------------------------- ERI3 SOC
+    // eri_kernels.cpp:1272-1291
+    ERI3SOCKernel::ERI3SOCKernel(const ShellPairData &ab, const ShellData &c)
+        : e1_bra(ecoeffsD1SHARK(ab, false)), e0_ket(ecoeffsSHARK(c, true)), grid(ab.la + ab.lb + c.l + 1)
+    {
+    }
+
+    // eri_kernel_funs.cpp:976-1090
+    std::vector<double> ERI3SOCKernel::operator()(size_t ipab, size_t ishc, const ShellPairData &ab,
+                                                  const ShellData &C) const
+    {
+        int lab = ab.la + ab.lb, L = lab + C.l;
+        int nsab = nSph(ab.la) * nSph(ab.lb), nsc = nSph(C.l);
+        size_t nsabc = static_cast<size_t>(nsab) * nsc;
+        int nhab = nHerm(lab), nhc = nHerm(C.l);
+        size_t nr = static_cast<size_t>(nhab) * nhc, nrxe = static_cast<size_t>(nhab) * nsc;
+        size_t neab = static_cast<size_t>(nsab) * nhab;
+        int nec = nsc * nhc;
+        size_t Kc = C.cdepths[ishc], co = C.coffsets[ishc];
+        const double *ex_ab = &ab.exps[ab.offsets_prims[ipab]];
+        const double *exc = &C.exps[co];
+        const double *xa = &ab.coords[6 * ipab], *xb = &ab.coords[6 * ipab + 3];
+        const double *xc = &C.coords[3 * ishc];
+        const double *e1ab = &e1_bra[ab.offsets_ecoeffs_d1[ipab]];
+        const double *e0c = &e0_ket[C.offsets_ecoeffs[ishc]];
+        auto hb = hermiteIdxs(lab), hk = hermiteIdxs(C.l);
+        std::vector<double> out(3 * nsabc, 0.0);
+        std::vector<double> fnx(L + 2), R(3 * nr);
+        for (size_t iab = 0; iab < ab.nrs_ppairs[ipab]; iab++)
+        {
+            double a = ex_ab[2 * iab], b = ex_ab[2 * iab + 1], p = a + b;
+            std::vector<double> RxE(3 * nrxe, 0.0);
+            for (size_t ic = 0; ic < Kc; ic++)
+            {
+                double c = exc[ic], alpha = p * c / (p + c);
+                double PC[3];
+                for (int i = 0; i < 3; i++)
+                    PC[i] = (a * xa[i] + b * xb[i]) / p - xc[i];
+                double x = alpha * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]);
+                calcBoysF(L + 1, x, grid, fnx.data());
+                double fac = k2pi52 / (p * c * std::sqrt(p + c));
+                auto r3 = calcRInts3D(L + 1, alpha, PC, fnx.data());
+                rMatrix(r3, L + 2, fac, hb, hk, 1, 0, 0, &R[0 * nr]);
+                rMatrix(r3, L + 2, fac, hb, hk, 0, 1, 0, &R[1 * nr]);
+                rMatrix(r3, L + 2, fac, hb, hk, 0, 0, 1, &R[2 * nr]);
+                for (int k = 0; k < 3; k++)
+                    gemmAcc(nhab, nsc, nhc, &R[k * nr], &e0c[ic * nec], &RxE[k * nrxe]);
+            }
+            socBra(nsab, nsc, nhab, &e1ab[3 * iab * neab], neab, RxE.data(), nrxe, nsabc, out.data());
+        }
         return out;
     }
 

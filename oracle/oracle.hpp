@@ -194,6 +194,38 @@ namespace orc
         BoysGrid grid;
     };
 
+    // Second derivatives of (a|b): 36 blocks [i][j], i, j over [Ax,Ay,Az,Bx,By,Bz], each
+    // (Sa,Sb), concatenated row-major in (i, j)  (eri_kernel_funs.cpp:343-433)
+    struct ERI2D2Kernel
+    {
+        ERI2D2Kernel(const ShellData &a, const ShellData &b);
+        std::vector<double> operator()(size_t isha, size_t ishb, const ShellData &a,
+                                       const ShellData &b) const;
+        std::vector<double> e_bra, e_ket;
+        BoysGrid grid;
+    };
+
+    // Spin-orbit-type integrals: 3 blocks (x, y, z), each (Sa,Sb,Sc,Sd)
+    // (eri_kernel_funs.cpp:846-974)
+    struct ERI4SOCKernel
+    {
+        ERI4SOCKernel(const ShellPairData &ab, const ShellPairData &cd);
+        std::vector<double> operator()(size_t ipab, size_t ipcd, const ShellPairData &ab,
+                                       const ShellPairData &cd) const;
+        std::vector<double> e1_bra, e0_ket;
+        BoysGrid grid;
+    };
+
+    // 3 blocks (x, y, z), each (Sa,Sb,Sc)  (eri_kernel_funs.cpp:976-1090)
+    struct ERI3SOCKernel
+    {
+        ERI3SOCKernel(const ShellPairData &ab, const ShellData &c);
+        std::vector<double> operator()(size_t ipab, size_t ishc, const ShellPairData &ab,
+                                       const ShellData &c) const;
+        std::vector<double> e1_bra, e0_ket;
+        BoysGrid grid;
+    };
+
     // ---- drivers (eri4.cpp, eri3.cpp, eri2.cpp) ----
     std::vector<double> eri4(const Structure &s);          // nbf^4
     std::vector<double> eri4Diagonal(const Structure &s);  // nbf^2

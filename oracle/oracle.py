@@ -43,6 +43,8 @@ def lib():
         for name in ("orc_eri4d1", "orc_eri3d1", "orc_eri2d1"):
             getattr(L, name).argtypes = [_p, _p]
         L.orc_deploy_sum.argtypes = [_p, _i, _i, _p]
+        L.orc_deploy_sum2.argtypes = [_p, _i, _p]
+        L.orc_blocks.argtypes = [_p, _i, _l, _p, _p]
         L.orc_jk.argtypes = [_p, _p, _p, _p, _d, _i, _p]
         L.orc_jk_grad.argtypes = [_p, _p, _p, _d]
         L.orc_boys.argtypes = [_i, _d, _p]
@@ -234,6 +236,27 @@ class Structure:
         s = _d()
         _check(lib().orc_deploy_sum(self.h, code, int(use_symm), ctypes.byref(s)))
         return s.value
+
+    def deploy_sum2(self, which: str) -> float:
+        """Sum of |x| over the blocks of the second-derivative / spin-orbit kernels for all
+        ordered pairs (deployERI2D2Kernel / deployERI4SOCKernel / deployERI3SOCKernel)."""
+        code = {"eri2d2": 6, "eri4soc": 7, "eri3soc": 8}[which]
+        s = _d()
+        _check(lib().orc_deploy_sum2(self.h, code, ctypes.byref(s)))
+        return s.value
+
+    BLOCK_KINDS = {"eri4": (0, 1), "eri3": (1, 1), "eri2": (2, 1), "eri4d1": (3, 12),
+                   "eri3d1": (4, 9), "eri2d1": (5, 6), "eri2d2": (6, 36), "eri4soc": (7, 3),
+                   "eri3soc": (8, 3)}
+
+    def blocks(self, which: str, shells: np.ndarray, dims) -> np.ndarray:
+        """Blocks of the listed entries (n, 4 / 3 / 2 shell indices) in the caller's shell order
+        -> (n, ncomp, *dims) with dims the entries' spherical sizes."""
+        code, ncomp = self.BLOCK_KINDS[which]
+        sh = np.ascontiguousarray(shells, dtype=np.int32)
+        out = np.zeros((sh.shape[0], ncomp, *dims))
+        _check(lib().orc_blocks(self.h, code, sh.shape[0], _ptr(sh), _ptr(out)))
+        return out
 
     def jk(self, D: np.ndarray, thresh: float = 1e-12, stride: int = 1):
         D = np.ascontiguousarray(D, np.float64)

@@ -8,11 +8,14 @@
 // Prints one "PASS <name>" / "FAIL <name>: ..." line per check; exit status = number of failures.
 #include <lible/ints/ints.hpp>
 
+#include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <functional>
 #include <string>
+#include <vector>
 
 namespace lints = lible::ints;
 using lible::vec2d;
@@ -192,6 +195,42 @@ namespace
                 }
             report("coulomb_exchange", err <= 1e-12 * scale,
                    "max err " + std::to_string(err / scale) + " of scale");
+        });
+        guarded("coulomb_exchange_gradient", [] {
+            // the backward entry against central differences (h = 1e-4 bohr) of the energy of
+            // coulombExchange, each difference formed as 1/2 tr(D dJ) - 1/4 tr(D dK), and the
+            // translational invariance sum_A dE/dR_A = 0
+            std::vector<std::array<double, 3>> xyz = xyz_h2o;
+            lints::Structure s("def2-svp", nrs_h2o, xyz);
+            const size_t n = s.getDimAO();
+            vec2d D(n, n);
+            for (size_t i = 0; i < n; i++)
+                for (size_t j = 0; j <= i; j++)
+                    D(i, j) = D(j, i) = std::cos(0.29 * i - 0.17 * j) / (1.0 + 0.2 * (i + j));
+            vec2d g = lints::coulombExchangeGradient(s, D, 0.0);
+            const double h = 1e-4;
+            double err = 0, gmax = 0, tsum[3] = {0, 0, 0};
+            for (size_t a = 0; a < g.dim<0>(); a++)
+                for (int x = 0; x < 3; x++)
+                {
+                    tsum[x] += g(a, x);
+                    gmax = std::max(gmax, std::fabs(g(a, x)));
+                    auto shifted = [&](double sgn) {
+                        auto c = xyz;
+                        c[a][x] += sgn * h * lints::_bohr_to_ang_;
+                        return lints::coulombExchange(lints::Structure("def2-svp", nrs_h2o, c), D, 0.0);
+                    };
+                    auto [Jp, Kp] = shifted(1.0);
+                    auto [Jm, Km] = shifted(-1.0);
+                    double dE = 0;
+                    for (size_t i = 0; i < n * n; i++)
+                        dE += D.memptr()[i] * (0.5 * (Jp.memptr()[i] - Jm.memptr()[i]) -
+                                               0.25 * (Kp.memptr()[i] - Km.memptr()[i]));
+                    err = std::max(err, std::fabs(g(a, x) - dE / (2 * h)));
+                }
+            const double tmax = std::max({std::fabs(tsum[0]), std::fabs(tsum[1]), std::fabs(tsum[2])});
+            report("coulomb_exchange_gradient", g.dim<0>() == 3 && err <= 1e-7 && tmax <= 1e-10 * gmax,
+                   "max |g - fd| " + std::to_string(err) + ", |sum| " + std::to_string(tmax));
         });
         guarded("eri4_diagonal", [] {
             lints::Structure s("def2-tzvp", nrs_o3, xyz_o3);

@@ -44,3 +44,7 @@ def test_null_arguments_rejected():
     assert L.lbx_ri_sums(None, None, 0, 0.0, None) == -1
     for name in ("lbx_eri4d1", "lbx_eri3d1", "lbx_eri2d1"):
         assert getattr(L, name)(None, None) == -1
+    for name in ("lbx_eri4_batch", "lbx_eri3_batch", "lbx_eri2_batch", "lbx_eri4d1_batch",
+                 "lbx_eri3d1_batch", "lbx_eri2d1_batch"):
+        assert getattr(L, name)(None, 1, None, None) == -1
+        assert getattr(L, name)(None, -1, None, None) == -1

@@ -164,6 +164,79 @@ def test_d1_f_and_g_shells(lbx, oracle):
     close(ctx.eri2d1().cpu().numpy(), s.eri2d1(), floor=1e-13)
 
 
+def close_block(got, ref, scale, rel=REL):
+    """Block comparison: within rel of each element, plus 1e-14 of the scale of the whole tensor
+    the block comes from (a block may be all zero, or nearly so, by symmetry)."""
+    err = np.abs(np.asarray(got) - ref)
+    tol = rel * np.abs(ref) + FLOOR * scale
+    assert (err <= tol).all(), f"max err {err.max():.3e}, scale {scale:.3e}"
+
+
+def shell_offsets(sh):
+    l = np.asarray(sh.l)
+    return l, np.concatenate([[0], np.cumsum(2 * l + 1)])[:-1]
+
+
+@pytest.mark.parametrize("cls", [(1, 0, 2, 1), (0, 1, 1, 2), (2, 2, 0, 0), (1, 1, 1, 1), (2, 1, 2, 2)])
+def test_eri4_blocks(lbx, oracle, cls):
+    """lbx_eri4_batch / lbx_eri4d1_batch: blocks of listed quartets in the caller's shell order
+    (pairs in both orders, same-l pairs whichever way the library stores them) equal the
+    oracle's dense tensors at those positions."""
+    m = mol("def2-svp", O3)
+    ctx = lbx.Context(m)
+    s = oracle.Structure(m)
+    e4, d4 = s.eri4(), s.eri4d1()
+    l, off = shell_offsets(m.shells)
+    rng = np.random.default_rng(sum(cls))
+    pick = [rng.choice(np.flatnonzero(l == x), 40) for x in cls]
+    q = np.stack(pick, axis=1)
+    got = ctx.blocks(q).cpu().numpy()
+    gd = ctx.blocks(q, deriv=True).cpu().numpy()
+    S = [2 * x + 1 for x in cls]
+    assert got.shape == (len(q), 1, *S) and gd.shape == (len(q), 12, *S)
+    for t, (a, b, c, d) in enumerate(q):
+        sl = (slice(off[a], off[a] + S[0]), slice(off[b], off[b] + S[1]),
+              slice(off[c], off[c] + S[2]), slice(off[d], off[d] + S[3]))
+        close_block(got[t, 0], e4[sl], np.abs(e4).max())
+        close_block(gd[t], d4[(slice(None),) + sl], np.abs(d4).max())
+
+
+def test_eri3_eri2_blocks(lbx, oracle):
+    """lbx_eri3_batch / lbx_eri2_batch and their derivatives against the oracle's dense
+    tensors (both orders of the main pair)."""
+    m = mol("def2-svp", O3, "def2-universal-jkfit")
+    ctx = lbx.Context(m)
+    s = oracle.Structure(m)
+    e3, d3, e2, d2 = s.eri3(), s.eri3d1(), s.eri2(), s.eri2d1()
+    l, off = shell_offsets(m.shells)
+    la, offa = shell_offsets(m.aux)
+    rng = np.random.default_rng(7)
+    for cls in [(2, 1, 3), (1, 2, 0), (1, 1, 2)]:
+        q = np.stack([rng.choice(np.flatnonzero(l == cls[0]), 30),
+                      rng.choice(np.flatnonzero(l == cls[1]), 30),
+                      rng.choice(np.flatnonzero(la == cls[2]), 30)], axis=1)
+        got = ctx.blocks(q).cpu().numpy()
+        gd = ctx.blocks(q, deriv=True).cpu().numpy()
+        S = [2 * x + 1 for x in cls]
+        for t, (a, b, c) in enumerate(q):
+            sl = (slice(off[a], off[a] + S[0]), slice(off[b], off[b] + S[1]),
+                  slice(offa[c], offa[c] + S[2]))
+            close_block(got[t, 0], e3[sl], np.abs(e3).max())
+            close_block(gd[t], d3[(slice(None),) + sl], np.abs(d3).max())
+    for cls in [(3, 1), (0, 2), (2, 2)]:
+        q = np.stack([rng.choice(np.flatnonzero(la == cls[0]), 30),
+                      rng.choice(np.flatnonzero(la == cls[1]), 30)], axis=1)
+        got = ctx.blocks(q).cpu().numpy()
+        gd = ctx.blocks(q, deriv=True).cpu().numpy()
+        S = [2 * x + 1 for x in cls]
+        for t, (a, b) in enumerate(q):
+            sl = (slice(offa[a], offa[a] + S[0]), slice(offa[b], offa[b] + S[1]))
+            close_block(got[t, 0], e2[sl], np.abs(e2).max())
+            close_block(gd[t], d2[(slice(None),) + sl], np.abs(d2).max())
+    with pytest.raises(lbx.LbxError):
+        ctx.blocks(np.array([[0, 0, 0], [0, 0, 10 ** 6]]))
+
+
 CLASS_PAIRS = [(la, lb, lc, ld) for la in range(4) for lb in range(la + 1)
                for lc in range(4) for ld in range(lc + 1)
                if la * (la + 1) // 2 + lb >= lc * (lc + 1) // 2 + ld]
@@ -220,13 +293,14 @@ def test_jk_mixed_energy(lbx):
     Jm, Km, sm = ctx.jk(D, schwarz=1e-12, precision=lbx.LBX_MIXED, mixed_thresh=1e-5)
     assert sm["quartets"] == st["quartets"]
     assert sm["quartets_fp32"] > 0.2 * sm["quartets"]
-    E = 0.5 * float((D * J).sum()) - 0.25 * float((D * K).sum())
-    Em = 0.5 * float((D * Jm).sum()) - 0.25 * float((D * Km).sum())
+    # the energy change from the matrix differences (the two energies themselves are ~1e3 Eh
+    # here, whose rounding would hide the difference)
+    dE = 0.5 * float((D * (Jm - J)).sum()) - 0.25 * float((D * (Km - K)).sum())
     dJ = float((Jm - J).abs().max())
     dK = float((Km - K).abs().max())
-    print(f"mixed: fp32 share {sm['quartets_fp32'] / sm['quartets']:.3f} dE {Em - E:.3e} "
+    print(f"mixed: fp32 share {sm['quartets_fp32'] / sm['quartets']:.3f} dE {dE:.3e} "
           f"max dJ {dJ:.3e} max dK {dK:.3e}")
-    assert abs(Em - E) <= 1e-10
+    assert abs(dE) <= 1e-10
     assert dJ <= 1e-9 and dK <= 1e-9
 
 
@@ -353,6 +427,59 @@ def test_jk_grad_finite_difference(lbx):
             cm[ia, x] -= h
             fd[ia, x] = (energy(cp) - energy(cm)) / (2 * h)
     assert np.abs(g - fd).max() < 1e-7, np.abs(g - fd).max()
+
+
+def test_jk_graph_replays(lbx, jk_case):
+    """A second lbx_jk call with the same buffers and thresholds replays the launch sequence the
+    first call captured into a HIP graph: three replays, with eager work of another context and
+    synchronisations between them, each give the first build's J and K (atomics may reorder the
+    last bits); a changed variant choice drops the graph and the next build is still right."""
+    import torch
+    m, D, Jr, Kr, sr = jk_case
+    ctx = lbx.Context(m)
+    Dt = torch.tensor(D, device="cuda")
+    J, K = torch.empty_like(Dt), torch.empty_like(Dt)
+    ctx.jk(Dt, schwarz=1e-12, J=J, K=K)
+    J0, K0 = J.cpu().numpy().copy(), K.cpu().numpy().copy()
+    close(J0, Jr, floor=1e-13)
+    close(K0, Kr, floor=1e-13)
+    other = lbx.Context(B.synthetic_molecule(5, 2, 3.0, B.RECIPE_SMALL))
+    for _ in range(3):
+        J.fill_(7.0)
+        K.fill_(-3.0)
+        _, _, st = ctx.jk(Dt, schwarz=1e-12, J=J, K=K)
+        assert st["quartets"] == sr["quartets"]
+        close(J.cpu().numpy(), J0, rel=1e-13, floor=1e-15)
+        close(K.cpu().numpy(), K0, rel=1e-13, floor=1e-15)
+        other.eri4()
+        torch.cuda.synchronize()
+    assert ctx.set_variant((1, 0, 1, 0), 2)
+    ctx.jk(Dt, schwarz=1e-12, J=J, K=K)
+    ctx.jk(Dt, schwarz=1e-12, J=J, K=K)
+    close(J.cpu().numpy(), Jr, floor=1e-13)
+    close(K.cpu().numpy(), Kr, floor=1e-13)
+
+
+def test_jk_energy_autograd(lbx, oracle):
+    """The harness's torch.autograd.Function over lbx_jk / lbx_jk_grad: torch's backward gives
+    the oracle's gradient (its contraction of the derivative integrals), scaled through the chain
+    rule, and passes torch.autograd.gradcheck against central differences of the forward."""
+    import torch
+    from tests.jk_autograd import jk_energy
+    m = B.synthetic_molecule(2025, 3, 4.5, B.RECIPE_MAIN)
+    D = B.density_from_orbitals(6, m.shells.nbf, 15, 0.1)
+    Dt = torch.tensor(D, device="cuda")
+    x = torch.tensor(m.coords, dtype=torch.float64, requires_grad=True)
+    E = jk_energy(x, Dt, m.shells)
+    (2.5 * E + (x ** 2).sum()).backward()
+    gr = oracle.Structure(m).jk_grad(D, thresh=1e-12)
+    close(x.grad.numpy() - 2 * m.coords, 2.5 * gr, rel=1e-10, floor=1e-12)
+    small = B.synthetic_molecule(11, 3, 3.0, B.RECIPE_SMALL)
+    Ds = torch.tensor(B.density_from_orbitals(5, small.shells.nbf, 4, 0.3), device="cuda")
+    xs = torch.tensor(small.coords, dtype=torch.float64, requires_grad=True)
+    # the backward accumulates with atomics, so two backward calls agree to rounding only
+    assert torch.autograd.gradcheck(lambda c: jk_energy(c, Ds, small.shells, 0.0), (xs,),
+                                    eps=1e-4, atol=1e-7, rtol=1e-6, nondet_tol=1e-11)
 
 
 def test_jk_tuned(lbx, jk_case):

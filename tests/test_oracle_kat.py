@@ -130,3 +130,14 @@ def test_ri_sums_match_dense(oracle):
     ref = np.array([np.abs(e3).sum(), e3.sum(), np.abs(e2).sum(), e2.sum()])
     got = s.ri_sums()
     assert np.allclose(got, ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max()), (got, ref)
+
+
+@pytest.mark.parametrize("which,aux,ref", [
+    ("eri2d2", "def2-universal-jkfit", 175567.635699242965),   # deployERI2D2Kernel :1055-1085
+    ("eri4soc", None, 12940.953099658431),                     # deployERI4SOCKernel :1087-1116
+    ("eri3soc", "def2-universal-jkfit", 18124.713152990007),   # deployERI3SOCKernel :1118-1148
+])
+def test_deploy_d2_soc(oracle, which, aux, ref):
+    s = oracle.Structure(mol("def2-svp", O3, aux))
+    val = s.deploy_sum2(which)
+    assert abs(val - ref) <= REL * ref, (val, ref)

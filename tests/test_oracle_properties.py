@@ -2,6 +2,7 @@
 differences, chosen so that a dropped term, a wrong sign or a transposed index fails one of them.
 J, K, E and the gradient have no reference counterpart; they are pinned here against the dense
 eri4 tensor (itself pinned by the reference's eri4 checksum) and by central differences."""
+import itertools
 import math
 
 import numpy as np
@@ -147,3 +148,103 @@ def test_gradient_finite_difference(oracle):
     assert np.abs(g - fd).max() < 1e-7, np.abs(g - fd).max()
     # translational invariance: forces sum to zero
     assert np.abs(g.sum(axis=0)).max() < 1e-10
+
+
+# One primitive per shell (s, p, d on every atom): the spin-orbit-type kernels then reduce to
+# second derivatives of the plain integrals with respect to the centres, which central
+# differences of the first-derivative kernels give independently of the SOC code path.
+K1_MAIN = [(0, 1, 1), (1, 1, 1), (2, 1, 1)]
+K1_AUX = [(0, 1, 1), (1, 1, 1), (2, 1, 1)]
+
+
+def _shifted(m, atom, axis, h):
+    c = m.coords.copy()
+    c[atom, axis] += h
+    return B.Molecule(c, m.shells, m.aux)
+
+
+def _dims(m, e, aux_pos):
+    ls = [m.aux.l[i] if k in aux_pos else m.shells.l[i] for k, i in enumerate(e)]
+    return [2 * l + 1 for l in ls]
+
+
+def test_eri4soc_is_antisymmetrised_second_derivative(oracle):
+    """For uncontracted shells on four different atoms, ERI4SOC_x = d2/dA_y dB_z - d2/dA_z dB_y
+    of (ab|cd) (per primitive pair d/d(A-B) = (b d/dA - a d/dB)/p on E and d/dP = d/dA + d/dB on
+    the Hermite Gaussian; the symmetric second derivatives cancel), y and z likewise, with the
+    mixed derivatives from central differences of ERI4D1 (its A components, moving B's atom)."""
+    m = B.synthetic_molecule(91, 4, 4.0, K1_MAIN)
+    s = oracle.Structure(m)
+    h = 1e-4
+    # the quartets (one shell per atom, atoms in each order) whose SOC blocks are largest
+    cand = []
+    for perm in itertools.permutations(range(4)):
+        for ks in itertools.product(range(3), repeat=4):
+            e = [3 * a + k for a, k in zip(perm, ks)]
+            v = np.abs(s.blocks("eri4soc", np.array([e]), _dims(m, e, ()))).max()
+            cand.append((v, e))
+    cand.sort(key=lambda t: -t[0])
+    picks, seen = [], set()
+    for _, e in cand:   # the three largest distinct shell sets, each in its leading order
+        if frozenset(e) not in seen:
+            seen.add(frozenset(e))
+            picks.append(e)
+        if len(picks) == 3:
+            break
+    for e in picks:
+        atoms = [i // 3 for i in e]
+        assert len(set(atoms)) == 4
+        dims = _dims(m, e, ())
+        soc = s.blocks("eri4soc", np.array([e]), dims)[0]
+        d2 = np.zeros((3, 3) + tuple(dims))   # d2[i][j] = d/dA_i d/dB_j
+        for j in range(3):
+            p = oracle.Structure(_shifted(m, atoms[1], j, h)).blocks("eri4d1", np.array([e]), dims)[0]
+            q = oracle.Structure(_shifted(m, atoms[1], j, -h)).blocks("eri4d1", np.array([e]), dims)[0]
+            d2[:, j] = (p[0:3] - q[0:3]) / (2 * h)
+        ref = np.stack([d2[1, 2] - d2[2, 1], d2[2, 0] - d2[0, 2], d2[0, 1] - d2[1, 0]])
+        scale = np.abs(ref).max()
+        assert scale > 1e-10   # a vanishing block would pin nothing
+        assert np.abs(soc - ref).max() <= 1e-6 * scale, np.abs(soc - ref).max() / scale
+
+
+def test_eri3soc_is_antisymmetrised_second_derivative(oracle):
+    """The same identity for (ab|P): ERI3SOC against differences of ERI3D1."""
+    m = B.synthetic_molecule(92, 3, 4.0, K1_MAIN, K1_AUX)
+    s = oracle.Structure(m)
+    h = 1e-4
+    for e in ([4, 2, 7], [1, 5, 6]):
+        atoms = [e[0] // 3, e[1] // 3, e[2] // 3]
+        assert len(set(atoms)) == 3
+        dims = _dims(m, e, (2,))
+        soc = s.blocks("eri3soc", np.array([e]), dims)[0]
+        d2 = np.zeros((3, 3) + tuple(dims))
+        for j in range(3):
+            p = oracle.Structure(_shifted(m, atoms[1], j, h)).blocks("eri3d1", np.array([e]), dims)[0]
+            q = oracle.Structure(_shifted(m, atoms[1], j, -h)).blocks("eri3d1", np.array([e]), dims)[0]
+            d2[:, j] = (p[0:3] - q[0:3]) / (2 * h)
+        ref = np.stack([d2[1, 2] - d2[2, 1], d2[2, 0] - d2[0, 2], d2[0, 1] - d2[1, 0]])
+        scale = np.abs(ref).max()
+        assert scale > 1e-10   # a vanishing block would pin nothing
+        assert np.abs(soc - ref).max() <= 1e-6 * scale, np.abs(soc - ref).max() / scale
+
+
+def test_eri2d2_is_derivative_of_eri2d1(oracle):
+    """ERI2D2 blocks [i][j] against central differences of ERI2D1 component i with respect to
+    coordinate j of the first (j < 3) or second (j >= 3) shell's atom; contracted shells."""
+    m = B.synthetic_molecule(93, 2, 3.0, B.RECIPE_SMALL, B.RECIPE_AUX)
+    s = oracle.Structure(m)
+    h = 1e-4
+    la = np.asarray(m.aux.l)
+    per_atom = len(B.RECIPE_AUX)
+    for e in ([2, per_atom + 3], [per_atom + 4, 1]):
+        atoms = [e[0] // per_atom, e[1] // per_atom]
+        assert atoms[0] != atoms[1]
+        dims = [2 * la[e[0]] + 1, 2 * la[e[1]] + 1]
+        d2 = s.blocks("eri2d2", np.array([e]), dims)[0].reshape(6, 6, *dims)
+        for j in range(6):
+            at, ax = atoms[j // 3], j % 3
+            p = oracle.Structure(_shifted(m, at, ax, h)).blocks("eri2d1", np.array([e]), dims)[0]
+            q = oracle.Structure(_shifted(m, at, ax, -h)).blocks("eri2d1", np.array([e]), dims)[0]
+            fd = (p - q) / (2 * h)
+            scale = np.abs(fd).max()
+            assert np.abs(d2[:, j] - fd).max() <= 1e-6 * max(scale, 1e-12), (j, np.abs(d2[:, j] - fd).max())
