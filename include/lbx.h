@@ -102,6 +102,20 @@ int lbx_eri4d1_batch(lbx_context *ctx, long n, const int *shells, double *out);
 int lbx_eri3d1_batch(lbx_context *ctx, long n, const int *shells, double *out);
 int lbx_eri2d1_batch(lbx_context *ctx, long n, const int *shells, double *out);
 
+/* The same block layout for the reference's remaining per-block kernels:
+ *   eri4soc (ncomp 3)  replaces ERI4SOCKernel::operator() (eri_kernels.hpp:188-204,
+ *                      eri_kernel_funs.cpp:846-974): components x, y, z of
+ *                      sum E'_{d1} R_{+d2} E - E'_{d2} R_{+d1} E, (d1, d2) = (y,z), (z,x), (x,y),
+ *                      E' = dE/d(A - B) of the bra pair; odd under the exchange of a and b
+ *   eri3soc (ncomp 3)  replaces ERI3SOCKernel::operator() (eri_kernels.hpp:206-222,
+ *                      eri_kernel_funs.cpp:976-1090)
+ *   eri2d2  (ncomp 36) replaces ERI2D2Kernel::operator() (eri_kernels.hpp:171-186,
+ *                      eri_kernel_funs.cpp:343-433): d2(a|b)/dX_i dX_j, i and j over
+ *                      [Ax, Ay, Az, Bx, By, Bz], component 6 i + j */
+int lbx_eri4soc_batch(lbx_context *ctx, long n, const int *shells, double *out);
+int lbx_eri3soc_batch(lbx_context *ctx, long n, const int *shells, double *out);
+int lbx_eri2d2_batch(lbx_context *ctx, long n, const int *shells, double *out);
+
 /* Coulomb and exchange matrices, no reference counterpart (the reference stops at the dense
  * tensor).  D, J, K: device (nbf,nbf) row-major, D symmetric.
  *   J_mn = sum_kl (mn|kl) D_kl,   K_mk = sum_nl (mn|kl) D_nl

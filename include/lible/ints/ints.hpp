@@ -8,11 +8,13 @@
 
 #include <array>
 #include <memory>
+#include <set>
 #include <stdexcept>
 #include <string>
 #include <utility>
 #include <vector>
 
+#include "../types.hpp"
 #include "../vectormd.hpp"
 
 struct lbx_context;
@@ -276,6 +278,75 @@ namespace lible::ints
     private:
         std::shared_ptr<detail::BlockCache> c_;
     };
+
+    /// 6 x 6 blocks: second derivatives with respect to [Ax, Ay, Az, Bx, By, Bz] twice
+    /// (eri_kernels.hpp:171-186).
+    class ERI2D2Kernel
+    {
+    public:
+        ERI2D2Kernel(const ShellData &sh_data_a, const ShellData &sh_data_b);
+        arr2d<vec2d, 6, 6> operator()(size_t ishell_a, size_t ishell_b, const ShellData &sh_data_a,
+                                      const ShellData &sh_data_b) const;
+    private:
+        std::shared_ptr<detail::BlockCache> c_;
+    };
+
+    /// 3 blocks x, y, z of the spin-orbit-type integrals (eri_kernels.hpp:188-204).
+    class ERI4SOCKernel
+    {
+    public:
+        ERI4SOCKernel(const ShellPairData &sp_data_ab, const ShellPairData &sp_data_cd);
+        std::array<vec4d, 3> operator()(size_t ipair_ab, size_t ipair_cd,
+                                        const ShellPairData &sp_data_ab,
+                                        const ShellPairData &sp_data_cd) const;
+    private:
+        std::shared_ptr<detail::BlockCache> c_;
+    };
+
+    /// 3 blocks x, y, z (eri_kernels.hpp:206-222).
+    class ERI3SOCKernel
+    {
+    public:
+        ERI3SOCKernel(const ShellPairData &sp_data_ab, const ShellData &sh_data_c);
+        std::array<vec3d, 3> operator()(size_t ipair_ab, size_t ishell_c,
+                                        const ShellPairData &sp_data_ab,
+                                        const ShellData &sh_data_c) const;
+    private:
+        std::shared_ptr<detail::BlockCache> c_;
+    };
+
+    /// Tabulated Boys function (boys_function.hpp:10-59): F_0 .. F_{n+6} on x = 0, 0.01, .., 30,
+    /// the table behind the GPU kernels' Taylor expansions; n <= 16.
+    class BoysGrid
+    {
+    public:
+        BoysGrid() = default;
+        explicit BoysGrid(int n);
+        double getLargeX() const { return 30.0; }
+        double getIntervalSize() const { return 0.01; }
+        int getN() const { return n_; }
+        int getMaxN() const { return max_n_; }
+        /// rows x = 0, 0.01, .. (getMaxN() + 1 values each)
+        const std::vector<double> &getFnxGrid() const { return fnx_grid_; }
+
+    private:
+        int n_ = 0, max_n_ = 0;
+        std::vector<double> fnx_grid_;
+    };
+
+    /// F_0(x) .. F_n(x) (boys_function.cpp:71-117): 6-term Taylor expansion around the grid
+    /// point below x and downward recursion; closed forms at x = 0 and beyond x = 30.  A host
+    /// function for scalar callers; lbx_boys evaluates arrays of points on the GPU.
+    std::vector<double> calcBoysF(int n, double x, const BoysGrid &boys_grid);
+
+    /// Names of the basis sets found in the main / auxiliary basis directories
+    /// (basis_sets.cpp:367-382: the reference's catalogue of the files it ships).
+    std::set<std::string> availableBasisSets();
+    std::set<std::string> availableBasisSetsAux();
+
+    /// Per class pair (la lb | lc ld) of the unique quartets: count and GPU time of their
+    /// integrals, then the sum of |(mn|kl)| over them, printed to stdout (eri4.cpp:97-150).
+    void eri4Benchmark(const Structure &structure);
 
     /// (mn|kl), dense nbf^4 (eri4.cpp:38-95).
     vec4d eri4(const Structure &structure);

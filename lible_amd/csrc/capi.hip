@@ -1443,7 +1443,8 @@ extern "C"
     }
 
     // Blocks of listed shell quartets / triplets / doublets (kind 4, 3, 2) in the caller's
-    // shell order, the integrals themselves (plain) or their first derivatives; every entry of
+    // shell order: the integrals themselves, their first derivatives, the spin-orbit-type
+    // integrals (kinds 4, 3) or the second derivatives (kind 2) (op: D1_OP_*); every entry of
     // one call has the same angular momenta.  Entries whose shell pair has no primitive pair
     // above the 1e-15 screen (the reference's ShellPairData drops such pairs) are zero.
     namespace
@@ -1466,7 +1467,7 @@ extern "C"
         }
 
         void runBlocks(lbx_context &c, std::vector<std::unordered_map<long long, int>> &maps, int kind,
-                       bool plain, long long nq, const int *sh, double *out, const char *who)
+                       int op, long long nq, const int *sh, double *out, const char *who)
         {
             const int per = kind == 4 ? 4 : (kind == 3 ? 3 : 2);
             const int nsh = static_cast<int>(c.shells.size()), nax = static_cast<int>(c.shells_aux.size());
@@ -1488,7 +1489,8 @@ extern "C"
                     if (lOf(k, sh[per * t + k]) != l[k])
                         throw Error(LBX_ERR_INVALID, std::string(who) + ": every entry must have the "
                                                                         "angular momenta of the first");
-            const int ncomp = plain ? 1 : 3 * (kind == 4 ? 4 : (kind == 3 ? 3 : 2));
+            const int ncomp = op == D1_OP_PLAIN ? 1 : op == D1_OP_SOC ? 3 : op == D1_OP_D2 ? 36
+                                                                                       : 3 * (kind == 4 ? 4 : (kind == 3 ? 3 : 2));
             long long bsize = 1;
             for (int k = 0; k < per; k++)
                 bsize *= nSph(l[k]);
@@ -1549,11 +1551,11 @@ extern "C"
             }
             if (tasks.empty())
                 return;
-            if (!plain)
+            if (op == D1_OP_DERIV || op == D1_OP_SOC)
             {
                 if (kind != 2)
                     computeEcoeffsD1(c, *bra);
-                if (kind == 4)
+                if (kind == 4 && op == D1_OP_DERIV)
                     computeEcoeffsD1(c, *ket);
             }
             void *d = nullptr;
@@ -1579,7 +1581,7 @@ extern "C"
                 a.nbf = static_cast<long long>(c.nbf);
                 a.naux = static_cast<long long>(c.nbf_aux);
                 a.blocks = 1;
-                a.plain = plain ? 1 : 0;
+                a.op = op;
                 launchD1(a, c.stream);
                 LBX_HIP(hipStreamSynchronize(c.stream));
             }
@@ -1592,7 +1594,7 @@ extern "C"
             (void)hipFree(d);
         }
 
-        int blocksEntry(lbx_context *ctx, int kind, bool plain, long nq, const int *shells, double *out,
+        int blocksEntry(lbx_context *ctx, int kind, int op, long nq, const int *shells, double *out,
                         const char *who)
         {
             return guarded([&]
@@ -1603,39 +1605,54 @@ extern "C"
                                requireSingle(*ctx, who);
                                if (kind != 4 && ctx->shells_aux.empty())
                                    throw Error(LBX_ERR_STATE, std::string(who) + ": RI approximation is not enabled");
-                               runBlocks(*ctx, ctx->pair_maps, kind, plain, nq, shells, out, who);
+                               runBlocks(*ctx, ctx->pair_maps, kind, op, nq, shells, out, who);
                            });
         }
     }
 
     int lbx_eri4_batch(lbx_context *ctx, long nq, const int *shells, double *out)
     {
-        return blocksEntry(ctx, 4, true, nq, shells, out, "lbx_eri4_batch");
+        return blocksEntry(ctx, 4, D1_OP_PLAIN, nq, shells, out, "lbx_eri4_batch");
     }
 
     int lbx_eri3_batch(lbx_context *ctx, long nq, const int *shells, double *out)
     {
-        return blocksEntry(ctx, 3, true, nq, shells, out, "lbx_eri3_batch");
+        return blocksEntry(ctx, 3, D1_OP_PLAIN, nq, shells, out, "lbx_eri3_batch");
     }
 
     int lbx_eri2_batch(lbx_context *ctx, long nq, const int *shells, double *out)
     {
-        return blocksEntry(ctx, 2, true, nq, shells, out, "lbx_eri2_batch");
+        return blocksEntry(ctx, 2, D1_OP_PLAIN, nq, shells, out, "lbx_eri2_batch");
     }
 
     int lbx_eri4d1_batch(lbx_context *ctx, long nq, const int *shells, double *out)
     {
-        return blocksEntry(ctx, 4, false, nq, shells, out, "lbx_eri4d1_batch");
+        return blocksEntry(ctx, 4, D1_OP_DERIV, nq, shells, out, "lbx_eri4d1_batch");
     }
 
     int lbx_eri3d1_batch(lbx_context *ctx, long nq, const int *shells, double *out)
     {
-        return blocksEntry(ctx, 3, false, nq, shells, out, "lbx_eri3d1_batch");
+        return blocksEntry(ctx, 3, D1_OP_DERIV, nq, shells, out, "lbx_eri3d1_batch");
     }
 
     int lbx_eri2d1_batch(lbx_context *ctx, long nq, const int *shells, double *out)
     {
-        return blocksEntry(ctx, 2, false, nq, shells, out, "lbx_eri2d1_batch");
+        return blocksEntry(ctx, 2, D1_OP_DERIV, nq, shells, out, "lbx_eri2d1_batch");
+    }
+
+    int lbx_eri4soc_batch(lbx_context *ctx, long nq, const int *shells, double *out)
+    {
+        return blocksEntry(ctx, 4, D1_OP_SOC, nq, shells, out, "lbx_eri4soc_batch");
+    }
+
+    int lbx_eri3soc_batch(lbx_context *ctx, long nq, const int *shells, double *out)
+    {
+        return blocksEntry(ctx, 3, D1_OP_SOC, nq, shells, out, "lbx_eri3soc_batch");
+    }
+
+    int lbx_eri2d2_batch(lbx_context *ctx, long nq, const int *shells, double *out)
+    {
+        return blocksEntry(ctx, 2, D1_OP_D2, nq, shells, out, "lbx_eri2d2_batch");
     }
 
     int lbx_eri4d1(lbx_context *ctx, double *out)

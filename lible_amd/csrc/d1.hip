@@ -71,6 +71,9 @@ namespace lbx
         }
 
         // image permutations: image position j holds original index perm[j]
+        // second derivatives of (a|b): the six distinct (i, j), i <= j, of the A block
+        __constant__ int kD2Pair[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+
         __constant__ int kImg4[8][4] = {{0, 1, 2, 3}, {1, 0, 2, 3}, {0, 1, 3, 2}, {1, 0, 3, 2},
                                         {2, 3, 0, 1}, {3, 2, 0, 1}, {2, 3, 1, 0}, {3, 2, 1, 0}};
 
@@ -82,7 +85,8 @@ namespace lbx
             __shared__ int sTuv[kD1MaxHerm];
             const int lane = threadIdx.x;
             const int la = A.la, lb = A.lb, lc = A.lc, ld = A.ld;
-            const int L1 = la + lb + lc + ld + 1;
+            // R to order L + 1 (first derivatives, spin-orbit) or L + 2 (second derivatives)
+            const int L1 = la + lb + lc + ld + (A.op == D1_OP_D2 ? 2 : 1);
             const int Hab = nHerm(la + lb), Hcd = nHerm(lc + ld), H1 = nHerm(L1);
             const int Sa = nSph(la), Sb = nSph(lb), Sc = nSph(lc), Sd = nSph(ld);
             const int Sab = Sa * Sb, Scd = Sc * Sd, nOut = Sab * Scd;
@@ -92,7 +96,10 @@ namespace lbx
 
             // components: kind 4 centres A B C D, kind 3 A B C, kind 2 A C
             const int ncent = A.kind == 4 ? 4 : (A.kind == 3 ? 3 : 2);
-            const int ncomp = A.plain ? 1 : 3 * ncent;
+            // components computed per entry, and components per entry in the output
+            const int ncomp = A.op == D1_OP_PLAIN ? 1 : A.op == D1_OP_SOC ? 3 : A.op == D1_OP_D2 ? 6 : 3 * ncent;
+            const int nstore = A.op == D1_OP_D2 ? 36 : ncomp;
+            const bool plain = A.op == D1_OP_PLAIN, extra = A.op == D1_OP_SOC || A.op == D1_OP_D2;
             const long long n = A.nbf;
             const int ts = A.blocks ? 4 : 2;
             for (long long task = blockIdx.x; task < A.ntasks; task += gridDim.x)
@@ -105,7 +112,7 @@ namespace lbx
                 {
                     const int ci = comp / 3, x = comp % 3;
                     // internal centre: 0 A, 1 B, 2 C, 3 D; -1 for the integrals themselves
-                    const int cent = A.plain ? -1 : (A.kind == 2 ? (ci == 0 ? 0 : 2) : ci);
+                    const int cent = (plain || extra) ? -1 : (A.kind == 2 ? (ci == 0 ? 0 : 2) : ci);
                     const int ex = cent >= 0 && x == 0, ey = cent >= 0 && x == 1, ez = cent >= 0 && x == 2;
                     double acc[kD1Acc];
 #pragma unroll
@@ -115,11 +122,11 @@ namespace lbx
                     for (int i = poff; i < poff + pcnt; i++)
                     {
                         const double4 P = A.bra.pp[i];
-                        const double rai = (A.bra_single || A.plain) ? 1.0 : A.bra.ra[i];
+                        const double rai = (A.bra_single || plain || extra) ? 1.0 : A.bra.ra[i];
                         for (int k = qoff; k < qoff + qcnt; k++)
                         {
                             const double4 Q = A.ket.pp[k];
-                            const double rck = (A.ket_single || A.plain) ? 1.0 : A.ket.ra[k];
+                            const double rck = (A.ket_single || plain || extra) ? 1.0 : A.ket.ra[k];
                             const double pq = P.x * Q.x, ps = P.x + Q.x, alpha = pq / ps;
                             const double X = P.y - Q.y, Y = P.z - Q.z, Z = P.w - Q.w;
                             if (lane == 0)
@@ -156,6 +163,70 @@ namespace lbx
                                 cur = nxt;
                             }
                             const double *R = sR[cur];
+
+                            if (extra)
+                            {
+                                // spin-orbit: component x = sum Eb'_{d1} R_{h+g+e_d2} Ek - Eb'_{d2}
+                                // R_{h+g+e_d1} Ek with (d1, d2) = (y, z), (z, x), (x, y) for x, y, z;
+                                // second derivatives of (a|b): component (i, j) = Eb R_{h+g+e_i+e_j} Ek
+                                const double *ek = A.ket.ek + static_cast<size_t>(k) * Hcd * Scd;
+                                const int nterm = A.op == D1_OP_SOC ? 2 : 1;
+                                for (int term = 0; term < nterm; term++)
+                                {
+                                    int sh0 = 0, sh1 = 0, sh2 = 0;
+                                    const double *bm;
+                                    double sg = 1.0;
+                                    if (A.op == D1_OP_SOC)
+                                    {
+                                        const int d1 = (x + 1) % 3, d2 = (x + 2) % 3;
+                                        const int dsh = term == 0 ? d2 : d1, dbra = term == 0 ? d1 : d2;
+                                        sh0 = dsh == 0;
+                                        sh1 = dsh == 1;
+                                        sh2 = dsh == 2;
+                                        bm = A.bra.e1b + (static_cast<size_t>(i) * 3 + dbra) * Hab * Sab;
+                                        sg = term == 0 ? 1.0 : -1.0;
+                                    }
+                                    else
+                                    {
+                                        const int ii = kD2Pair[comp][0], jj = kD2Pair[comp][1];
+                                        sh0 = (ii == 0) + (jj == 0);
+                                        sh1 = (ii == 1) + (jj == 1);
+                                        sh2 = (ii == 2) + (jj == 2);
+                                        bm = A.bra.eb + static_cast<size_t>(i) * Hab * Sab;
+                                    }
+                                    for (int e = lane; e < Hab * Scd; e += 64)
+                                    {
+                                        const int h = e / Scd, s = e - h * Scd;
+                                        const int th = sTuv[h];
+                                        const int t = th & 255, u = (th >> 8) & 255, v = th >> 16;
+                                        double y = 0.0;
+                                        for (int g = 0; g < Hcd; g++)
+                                        {
+                                            const int tg = sTuv[g];
+                                            y = fma(R[hermIdx(t + (tg & 255) + sh0, u + ((tg >> 8) & 255) + sh1,
+                                                              v + (tg >> 16) + sh2)],
+                                                    ek[g * Scd + s], y);
+                                        }
+                                        sY[e] = y;
+                                    }
+                                    __syncthreads();
+#pragma unroll
+                                    for (int r = 0; r < kD1Acc; r++)
+                                    {
+                                        const int o = lane + 64 * r;
+                                        if (o < nOut)
+                                        {
+                                            const int sab = o / Scd, s = o - sab * Scd;
+                                            double a = 0.0;
+                                            for (int h = 0; h < Hab; h++)
+                                                a = fma(bm[h * Sab + sab], sY[h * Scd + s], a);
+                                            acc[r] = fma(sg, a, acc[r]);
+                                        }
+                                    }
+                                    __syncthreads();
+                                }
+                                continue;
+                            }
 
                             // ket transform: Y = aT sum_g R_{h+g+e_x} Ek_g + a0 sum_g R_{h+g} Ek'_g
                             double aT, a0 = 0.0;
@@ -258,7 +329,8 @@ namespace lbx
                         const bool swab = flags & 1, swcd = flags & 2;
                         const int rb = swab ? Sa : Sb, rc = swcd ? Sd : Sc, rd = swcd ? Sc : Sd;
                         // component of the caller's centre: its shell position k, 3 k + x
-                        int rcomp = 0;
+                        int rcomp = comp;
+                        double sg = 1.0;
                         if (cent >= 0)
                         {
                             int pos = cent;
@@ -270,18 +342,35 @@ namespace lbx
                                 pos = swcd ? 5 - cent : cent;
                             rcomp = 3 * pos + x;
                         }
-                        double *dst = A.out + (slot * ncomp + rcomp) * nOut;
-#pragma unroll
-                        for (int r = 0; r < kD1Acc; r++)
+                        else if (A.op == D1_OP_SOC && swab)
+                            sg = -1.0;   // E' is odd in A - B: the (b, a) pair's is minus the (a, b) one's
+                        // second derivatives: AA (i, j) -> blocks [i][j], [j][i], [3+i][3+j], [3+j][3+i]
+                        // and minus it at the AB / BA positions [i][3+j], [j][3+i], [3+i][j], [3+j][i]
+                        const int nwrite = A.op == D1_OP_D2 ? 8 : 1;
+                        for (int wi = 0; wi < nwrite; wi++)
                         {
-                            const int o = lane + 64 * r;
-                            if (o >= nOut)
-                                continue;
-                            const int sab = o / Scd, scd = o - sab * Scd;
-                            const int i0 = sab / Sb, i1 = sab % Sb, i2 = scd / Sd, i3 = scd % Sd;
-                            const int m0 = swab ? i1 : i0, m1 = swab ? i0 : i1;
-                            const int m2 = swcd ? i3 : i2, m3 = swcd ? i2 : i3;
-                            dst[((m0 * rb + m1) * rc + m2) * rd + m3] = acc[r];
+                            double wsg = sg;
+                            if (A.op == D1_OP_D2)
+                            {
+                                const int ii = kD2Pair[comp][0], jj = kD2Pair[comp][1];
+                                const int a = (wi & 1) ? jj : ii, b = (wi & 1) ? ii : jj;
+                                const int sa = (wi >> 1) & 1, sb = ((wi >> 1) ^ (wi >> 2)) & 1;
+                                rcomp = 6 * (a + 3 * sa) + b + 3 * sb;
+                                wsg = sa == sb ? 1.0 : -1.0;
+                            }
+                            double *dst = A.out + (slot * nstore + rcomp) * nOut;
+#pragma unroll
+                            for (int r = 0; r < kD1Acc; r++)
+                            {
+                                const int o = lane + 64 * r;
+                                if (o >= nOut)
+                                    continue;
+                                const int sab = o / Scd, scd = o - sab * Scd;
+                                const int i0 = sab / Sb, i1 = sab % Sb, i2 = scd / Sd, i3 = scd % Sd;
+                                const int m0 = swab ? i1 : i0, m1 = swab ? i0 : i1;
+                                const int m2 = swcd ? i3 : i2, m3 = swcd ? i2 : i3;
+                                dst[((m0 * rb + m1) * rc + m2) * rd + m3] = wsg * acc[r];
+                            }
                         }
                         continue;
                     }
@@ -335,7 +424,7 @@ namespace lbx
     {
         if (a.ntasks <= 0)
             return;
-        if (a.la + a.lb + a.lc + a.ld + 1 > kD1MaxL1 || nHerm(a.la + a.lb) * nSph(a.lc) * nSph(a.ld) > kD1MaxY ||
+        if (a.la + a.lb + a.lc + a.ld + (a.op == D1_OP_D2 ? 2 : 1) > kD1MaxL1 || nHerm(a.la + a.lb) * nSph(a.lc) * nSph(a.ld) > kD1MaxY ||
             nSph(a.la) * nSph(a.lb) * nSph(a.lc) * nSph(a.ld) > kD1MaxOut)
             throw Error(LBX_ERR_INVALID, "derivative tensors: class beyond the kernel's limits");
         const int grid = static_cast<int>(a.ntasks < 16384 ? a.ntasks : 16384);

@@ -6,6 +6,10 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <cmath>
+#include <set>
+#include <cstdio>
+#include <chrono>
 #include <cctype>
 #include <cstdlib>
 #include <filesystem>
@@ -498,6 +502,125 @@ namespace lible::ints
         return g;
     }
 
+    // ------------------------------------------------------------------ Boys, catalogue, benchmark
+    BoysGrid::BoysGrid(int n) : n_(n), max_n_(n + 6)
+    {
+        if (n < 0 || n > lbx::kMaxLBoys)
+            throw std::runtime_error("BoysGrid: n must be in 0.." + std::to_string(lbx::kMaxLBoys));
+        // the library's table (rows x = 0, 0.01, .., 30; F_0 .. F_{kBoysCols-1}), cut to max_n
+        const std::vector<double> tab = lbx::boysTable();
+        fnx_grid_.resize(static_cast<size_t>(lbx::kBoysRows) * (max_n_ + 1));
+        for (int r = 0; r < lbx::kBoysRows; r++)
+            for (int m = 0; m <= max_n_; m++)
+                fnx_grid_[static_cast<size_t>(r) * (max_n_ + 1) + m] =
+                        tab[static_cast<size_t>(r) * lbx::kBoysCols + m];
+    }
+
+    std::vector<double> calcBoysF(int n, double x, const BoysGrid &g)
+    {
+        if (n < 0 || n > g.getN())
+            throw std::runtime_error("calcBoysF: n exceeds the grid's order");
+        std::vector<double> f(n + 1);
+        if (x == 0.0)
+        {
+            for (int k = 0; k <= n; k++)
+                f[k] = 1.0 / (2 * k + 1);
+            return f;
+        }
+        if (x > g.getLargeX())
+        {
+            f[0] = 0.5 * std::sqrt(M_PI / x);
+            for (int k = 1; k <= n; k++)
+                f[k] = f[k - 1] * (k - 0.5) / x;
+            return f;
+        }
+        const int w = g.getMaxN() + 1;
+        const int iv = static_cast<int>(x / g.getIntervalSize());
+        const double dx = x - iv * g.getIntervalSize();
+        const double *row = g.getFnxGrid().data() + static_cast<size_t>(iv) * w + n;
+        // F_n(x) = sum_k F_{n+k}(x_i) (-dx)^k / k!, k = 0..6 (d/dx F_m = -F_{m+1})
+        double s = row[6] / 720.0;
+        s = s * -dx + row[5] / 120.0;
+        s = s * -dx + row[4] / 24.0;
+        s = s * -dx + row[3] / 6.0;
+        s = s * -dx + row[2] / 2.0;
+        s = s * -dx + row[1];
+        s = s * -dx + row[0];
+        f[n] = s;
+        const double ex = std::exp(-x);
+        for (int k = n - 1; k >= 0; k--)
+            f[k] = (2.0 * x * f[k + 1] + ex) / (2 * k + 1);
+        return f;
+    }
+
+    namespace
+    {
+        std::set<std::string> basisNames(const std::string &root)
+        {
+            std::set<std::string> names;
+            if (root.empty() || !fs::is_directory(root))
+                return names;
+            for (const auto &e : fs::recursive_directory_iterator(root))
+                if (e.is_regular_file() && e.path().extension() == ".txt")
+                    names.insert(lower(e.path().stem().string()));
+            return names;
+        }
+    }
+
+    std::set<std::string> availableBasisSets() { return basisNames(BasisPaths::getMainBasisSetsPath()); }
+
+    std::set<std::string> availableBasisSetsAux() { return basisNames(BasisPaths::getAuxBasisSetsPath()); }
+
+    void eri4Benchmark(const Structure &structure)
+    {
+        std::printf("Lible::%-40s\n", "ERI4 benchmark...");
+        std::fflush(stdout);
+        const auto t0 = std::chrono::steady_clock::now();
+        lbx_context *ctx = structure.context();
+        const auto sp = shellPairData(true, structure);
+        double total = 0;
+        for (size_t i = 0; i < sp.size(); i++)
+            for (size_t j = 0; j <= i; j++)
+            {
+                const auto &ab = sp[i], &cd = sp[j];
+                const auto t1 = std::chrono::steady_clock::now();
+                const size_t blk = (2 * ab.la_ + 1) * (2 * ab.lb_ + 1) * (2 * cd.la_ + 1) * (2 * cd.lb_ + 1);
+                // unique quartets of the class pair, in chunks of ~32 MiB of blocks
+                const size_t chunk = std::max<size_t>(1, (size_t(4) << 20) / blk);
+                std::vector<int> q;
+                size_t count = 0;
+                auto flush = [&]() {
+                    const size_t n = q.size() / 4;
+                    if (n == 0)
+                        return;
+                    DeviceArray d(n * blk);
+                    check(lbx_eri4_batch(ctx, static_cast<long>(n), q.data(), d.get()));
+                    check(lbx_synchronize(ctx));
+                    std::vector<double> h(n * blk);
+                    d.toHost(h.data());
+                    for (double x : h)
+                        total += std::fabs(x);
+                    q.clear();
+                };
+                for (size_t p = 0; p < ab.n_pairs_; p++)
+                    for (size_t r = 0; r < (i == j ? p + 1 : cd.n_pairs_); r++)
+                    {
+                        q.insert(q.end(), {int(ab.shell_idxs_[2 * p]), int(ab.shell_idxs_[2 * p + 1]),
+                                           int(cd.shell_idxs_[2 * r]), int(cd.shell_idxs_[2 * r + 1])});
+                        count++;
+                        if (q.size() / 4 >= chunk)
+                            flush();
+                    }
+                flush();
+                const std::chrono::duration<double> dt = std::chrono::steady_clock::now() - t1;
+                std::printf("   %d %d %d %d ; %10zu ; %.2e s\n", ab.la_, ab.lb_, cd.la_, cd.lb_, count, dt.count());
+            }
+        std::printf("   sum_eri4 = %16.12f\n", total);
+        const std::chrono::duration<double> dt = std::chrono::steady_clock::now() - t0;
+        std::printf("done %.2e s\n", dt.count());
+        std::fflush(stdout);
+    }
+
     // ------------------------------------------------------------------ shell data, kernels
     namespace detail
     {
@@ -868,6 +991,63 @@ namespace lible::ints
         const auto b = c_->at(p, k);
         std::array<vec3d, 9> r;
         for (size_t x = 0; x < 9; x++)
+            r[x] = copyBlock(vec3d(nsph(ab.la_), nsph(ab.lb_), nsph(c.l_)), b, x);
+        return r;
+    }
+
+    ERI2D2Kernel::ERI2D2Kernel(const ShellData &a, const ShellData &b)
+        : c_(makeCache(sameTensors(a, b, "ERI2D2Kernel"), lbx_eri2d2_batch, shellList(a.shell_idxs_), 1,
+                       a.n_shells_, shellList(b.shell_idxs_), 1, b.n_shells_, 36 * nsph(a.l_) * nsph(b.l_)))
+    {
+    }
+
+    arr2d<vec2d, 6, 6> ERI2D2Kernel::operator()(size_t i, size_t j, const ShellData &a,
+                                                const ShellData &b) const
+    {
+        checkIndex(i, a.n_shells_, "ERI2D2Kernel");
+        checkIndex(j, b.n_shells_, "ERI2D2Kernel");
+        const auto blk = c_->at(i, j);
+        arr2d<vec2d, 6, 6> r;
+        for (size_t x = 0; x < 6; x++)
+            for (size_t y = 0; y < 6; y++)
+                r[x][y] = copyBlock(vec2d(nsph(a.l_), nsph(b.l_)), blk, 6 * x + y);
+        return r;
+    }
+
+    ERI4SOCKernel::ERI4SOCKernel(const ShellPairData &ab, const ShellPairData &cd)
+        : c_(makeCache(sameTensors(ab, cd, "ERI4SOCKernel"), lbx_eri4soc_batch, shellList(ab.shell_idxs_), 2,
+                       ab.n_pairs_, shellList(cd.shell_idxs_), 2, cd.n_pairs_,
+                       3 * nsph(ab.la_) * nsph(ab.lb_) * nsph(cd.la_) * nsph(cd.lb_)))
+    {
+    }
+
+    std::array<vec4d, 3> ERI4SOCKernel::operator()(size_t p, size_t q, const ShellPairData &ab,
+                                                   const ShellPairData &cd) const
+    {
+        checkIndex(p, ab.n_pairs_, "ERI4SOCKernel");
+        checkIndex(q, cd.n_pairs_, "ERI4SOCKernel");
+        const auto b = c_->at(p, q);
+        std::array<vec4d, 3> r;
+        for (size_t x = 0; x < 3; x++)
+            r[x] = copyBlock(vec4d(nsph(ab.la_), nsph(ab.lb_), nsph(cd.la_), nsph(cd.lb_)), b, x);
+        return r;
+    }
+
+    ERI3SOCKernel::ERI3SOCKernel(const ShellPairData &ab, const ShellData &c)
+        : c_(makeCache(sameTensors(ab, c, "ERI3SOCKernel"), lbx_eri3soc_batch, shellList(ab.shell_idxs_), 2,
+                       ab.n_pairs_, shellList(c.shell_idxs_), 1, c.n_shells_,
+                       3 * nsph(ab.la_) * nsph(ab.lb_) * nsph(c.l_)))
+    {
+    }
+
+    std::array<vec3d, 3> ERI3SOCKernel::operator()(size_t p, size_t k, const ShellPairData &ab,
+                                                   const ShellData &c) const
+    {
+        checkIndex(p, ab.n_pairs_, "ERI3SOCKernel");
+        checkIndex(k, c.n_shells_, "ERI3SOCKernel");
+        const auto b = c_->at(p, k);
+        std::array<vec3d, 3> r;
+        for (size_t x = 0; x < 3; x++)
             r[x] = copyBlock(vec3d(nsph(ab.la_), nsph(ab.lb_), nsph(c.l_)), b, x);
         return r;
     }

@@ -192,7 +192,14 @@ namespace lbx
         // writes its block(s) to out + slot * ncomp * Sa Sb Sc Sd in the caller's shell order
         // (flags bit 0: the caller's bra shells are the pair's (b, a); bit 1: the ket's (d, c)).
         int blocks;
-        int plain;                   // the integrals themselves (one component), no derivative
+        int op;                      // D1_OP_*: what each entry holds
+    };
+    enum D1Op
+    {
+        D1_OP_DERIV = 0,   // first derivatives (12 / 9 / 6 components)
+        D1_OP_PLAIN = 1,   // the integrals themselves (one component)
+        D1_OP_SOC = 2,     // the reference's spin-orbit-type integrals (3 components; eri4, eri3)
+        D1_OP_D2 = 3       // second derivatives of (a|b) (36 components; eri2)
     };
     void launchD1(const D1Args &a, hipStream_t s);
 }

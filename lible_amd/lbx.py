@@ -22,7 +22,7 @@ SYMBOLS = ["lbx_create", "lbx_destroy", "lbx_last_error", "lbx_dims", "lbx_set_s
            "lbx_synchronize", "lbx_boys", "lbx_eri4", "lbx_eri4_diag", "lbx_eri3", "lbx_eri2",
            "lbx_eri2_diag", "lbx_eri4d1", "lbx_eri3d1", "lbx_eri2d1", "lbx_eri4_batch",
            "lbx_eri3_batch", "lbx_eri2_batch", "lbx_eri4d1_batch", "lbx_eri3d1_batch",
-           "lbx_eri2d1_batch", "lbx_jk", "lbx_create_batch", "lbx_ri_sums", "lbx_jk_grad",
+           "lbx_eri2d1_batch", "lbx_eri4soc_batch", "lbx_eri3soc_batch", "lbx_eri2d2_batch", "lbx_jk", "lbx_create_batch", "lbx_ri_sums", "lbx_jk_grad",
            "lbx_tune", "lbx_set_variant", "lbx_profile_enable",
            "lbx_profile_read", "lbx_set_class_filter"]
 VARIANTS = ("valu-bra", "valu-ket", "mfma", "valu-bra-f32", "valu-ket-f32", "mfma-f32", "block",
@@ -83,7 +83,8 @@ def lib():
                      "lbx_eri4d1", "lbx_eri3d1", "lbx_eri2d1"):
             getattr(L, name).argtypes = [p, p]
         for name in ("lbx_eri4_batch", "lbx_eri3_batch", "lbx_eri2_batch", "lbx_eri4d1_batch",
-                     "lbx_eri3d1_batch", "lbx_eri2d1_batch"):
+                     "lbx_eri3d1_batch", "lbx_eri2d1_batch", "lbx_eri4soc_batch",
+                     "lbx_eri3soc_batch", "lbx_eri2d2_batch"):
             getattr(L, name).argtypes = [p, l, p, p]
         L.lbx_jk.argtypes = [p, p, p, p, d, i, d, ctypes.POINTER(JKStats)]
         L.lbx_jk_grad.argtypes = [p, p, p, d, i, d, ctypes.POINTER(JKStats)]
@@ -246,10 +247,12 @@ class Context:
         _check(lib().lbx_eri2d1(self.h, out.data_ptr()))
         return out
 
-    def blocks(self, shells, deriv: bool = False):
+    def blocks(self, shells, deriv: bool = False, op: str | None = None):
         """Blocks of listed shell quartets (n, 4), triplets (n, 3: main, main, auxiliary) or
         doublets (n, 2: auxiliary) -> tensor (n, ncomp, S_a, S_b[, S_c[, S_d]]) with ncomp 1 for
-        the integrals, 12 / 9 / 6 for their derivatives (lbx_eri*_batch)."""
+        the integrals, 12 / 9 / 6 for their derivatives (lbx_eri*_batch); op "soc" (quartets,
+        triplets: 3 components) or "d2" (doublets: 36 components) for the spin-orbit-type
+        integrals and the second derivatives."""
         sh = np.ascontiguousarray(shells, dtype=np.int32)
         if sh.ndim != 2 or sh.shape[1] not in (2, 3, 4):
             raise LbxError("shells must be (n, 4), (n, 3) or (n, 2)")
@@ -267,9 +270,17 @@ class Context:
                 if not 0 <= idx < len(ls):
                     raise LbxError("shell index out of range")
                 dims.append(2 * int(ls[idx]) + 1)
-        ncomp = (12, 9, 6)[4 - per] if deriv else 1
+        base = {4: "lbx_eri4", 3: "lbx_eri3", 2: "lbx_eri2"}[per]
+        if op == "soc" and per in (3, 4):
+            ncomp, name = 3, base + "soc_batch"
+        elif op == "d2" and per == 2:
+            ncomp, name = 36, base + "d2_batch"
+        elif op is None:
+            ncomp = (12, 9, 6)[4 - per] if deriv else 1
+            name = base + ("d1_batch" if deriv else "_batch")
+        else:
+            raise LbxError(f"no {op} blocks for {per}-index entries")
         out = self._out((sh.shape[0], ncomp, *dims))
-        name = {4: "lbx_eri4", 3: "lbx_eri3", 2: "lbx_eri2"}[per] + ("d1_batch" if deriv else "_batch")
         _check(getattr(lib(), name)(self.h, sh.shape[0], sh.ctypes.data, out.data_ptr()))
         return out
 

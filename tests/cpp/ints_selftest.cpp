@@ -13,6 +13,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <cstdlib>
+#include <unistd.h>
 #include <functional>
 #include <string>
 #include <vector>
@@ -130,6 +132,22 @@ namespace
             lints::Structure s("def2-svp", "def2-svp", {8, 1}, {1}, {xyz_h2o[0], xyz_h2o[1]},
                                {xyz_h2o[2]});
             report("ghost_dims", s.getDimAO() == 24 && s.getNAtoms() == 2);
+        });
+        guarded("calc_boys_f", [] {
+            // tests_ints.cpp:1296-1313: sum of F_0..F_5 at x = 2.5, absolute 1e-12
+            lints::BoysGrid grid(5);
+            double sum = 0;
+            for (double v : lints::calcBoysF(5, 2.5, grid))
+                sum += v;
+            const auto f0 = lints::calcBoysF(0, 0.0, grid), f30 = lints::calcBoysF(3, 45.0, grid);
+            report("calc_boys_f", std::fabs(sum - 0.728561124180) < 1e-12 && f0[0] == 1.0 &&
+                                      std::fabs(f30[0] - 0.5 * std::sqrt(M_PI / 45.0)) < 1e-16,
+                   "sum " + std::to_string(sum));
+        });
+        guarded("available_basis_sets", [] {
+            const auto m = lints::availableBasisSets(), a = lints::availableBasisSetsAux();
+            report("available_basis_sets", m.count("def2-svp") && m.count("def2-tzvp") &&
+                                               a.count("def2-universal-jkfit"));
         });
         guarded("missing_element", [] {
             bool threw = false;
@@ -337,6 +355,95 @@ namespace
                                 sum += absSum(blk);
                 }
             checkSum("deploy_eri2d1_kernel", sum, 39949.218421684251);  // :1024-1053
+        });
+        guarded("deploy_eri2d2_kernel", [] {
+            lints::Structure structure("def2-svp", "def2-universal-jkfit", nrs_o3, xyz_o3);
+            auto sh = lints::shellDataAux(structure);
+            double sum = 0;
+            for (const auto &a : sh)
+                for (const auto &b : sh)
+                {
+                    lints::ERI2D2Kernel kernel(a, b);
+                    for (size_t i = 0; i < a.n_shells_; i++)
+                        for (size_t j = 0; j < b.n_shells_; j++)
+                            for (const auto &row : kernel(i, j, a, b))
+                                for (const vec2d &blk : row)
+                                    sum += absSum(blk);
+                }
+            checkSum("deploy_eri2d2_kernel", sum, 175567.635699242965);  // :1055-1085
+        });
+        guarded("deploy_eri4soc_kernel", [] {
+            lints::Structure structure("def2-svp", nrs_o3, xyz_o3);
+            auto sp = lints::shellPairData(false, structure);
+            double sum = 0;
+            for (const auto &ab : sp)
+                for (const auto &cd : sp)
+                {
+                    lints::ERI4SOCKernel kernel(ab, cd);
+                    for (size_t p = 0; p < ab.n_pairs_; p++)
+                        for (size_t q = 0; q < cd.n_pairs_; q++)
+                            for (const vec4d &blk : kernel(p, q, ab, cd))
+                                sum += absSum(blk);
+                }
+            checkSum("deploy_eri4soc_kernel", sum, 12940.953099658431);  // :1087-1116
+        });
+        guarded("deploy_eri3soc_kernel", [] {
+            lints::Structure structure("def2-svp", "def2-universal-jkfit", nrs_o3, xyz_o3);
+            auto sp = lints::shellPairData(false, structure);
+            auto sh = lints::shellDataAux(structure);
+            double sum = 0;
+            for (const auto &ab : sp)
+                for (const auto &c : sh)
+                {
+                    lints::ERI3SOCKernel kernel(ab, c);
+                    for (size_t p = 0; p < ab.n_pairs_; p++)
+                        for (size_t k = 0; k < c.n_shells_; k++)
+                            for (const vec3d &blk : kernel(p, k, ab, c))
+                                sum += absSum(blk);
+                }
+            checkSum("deploy_eri3soc_kernel", sum, 18124.713152990007);  // :1118-1148
+        });
+        guarded("eri4_benchmark", [] {
+            // eri4Benchmark prints per class pair and the sum of |(mn|kl)| over the unique
+            // quartets; its output is captured and the sum checked against eri4Diagonal's
+            // companion checksum of the same molecule (tests_ints.cpp:748 sums the dense tensor;
+            // over unique quartets only, the sum is the one printed by the reference's benchmark)
+            lints::Structure s("def2-svp", nrs_o3, xyz_o3);
+            std::fflush(stdout);
+            const int saved = dup(fileno(stdout));
+            std::FILE *tmp = std::tmpfile();
+            dup2(fileno(tmp), fileno(stdout));
+            lints::eri4Benchmark(s);
+            std::fflush(stdout);
+            dup2(saved, fileno(stdout));
+            close(saved);
+            std::rewind(tmp);
+            std::string text;
+            char buf[512];
+            while (std::fgets(buf, sizeof buf, tmp))
+                text += buf;
+            std::fclose(tmp);
+            const size_t at = text.find("sum_eri4 = ");
+            const double got = at == std::string::npos ? -1.0 : std::atof(text.c_str() + at + 11);
+            // the same sum from the dense tensor restricted to unique quartets
+            vec4d e = lints::eri4(s);
+            const auto sp = lints::shellPairData(true, s);
+            double want = 0;
+            for (size_t i = 0; i < sp.size(); i++)
+                for (size_t j = 0; j <= i; j++)
+                    for (size_t p = 0; p < sp[i].n_pairs_; p++)
+                        for (size_t q = 0; q < (i == j ? p + 1 : sp[j].n_pairs_); q++)
+                        {
+                            const auto &ab = sp[i], &cd = sp[j];
+                            for (size_t a = 0; a < 2 * size_t(ab.la_) + 1; a++)
+                                for (size_t b = 0; b < 2 * size_t(ab.lb_) + 1; b++)
+                                    for (size_t c = 0; c < 2 * size_t(cd.la_) + 1; c++)
+                                        for (size_t d = 0; d < 2 * size_t(cd.lb_) + 1; d++)
+                                            want += std::fabs(e(ab.offsets_sph_[2 * p] + a, ab.offsets_sph_[2 * p + 1] + b,
+                                                                cd.offsets_sph_[2 * q] + c, cd.offsets_sph_[2 * q + 1] + d));
+                        }
+            report("eri4_benchmark", std::fabs(got - want) <= 1e-11 * want && text.find("done") != std::string::npos,
+                   "printed " + std::to_string(got) + ", unique-quartet sum " + std::to_string(want));
         });
         guarded("ghost_atoms", [] {
             // Integrals do not see nuclear charges: with the last H a ghost atom, the AO order is

@@ -237,6 +237,40 @@ def test_eri3_eri2_blocks(lbx, oracle):
         ctx.blocks(np.array([[0, 0, 0], [0, 0, 10 ** 6]]))
 
 
+@pytest.mark.parametrize("op,which,classes", [
+    ("soc", "eri4soc", [(1, 0, 2, 1), (0, 1, 1, 2), (2, 2, 1, 1), (1, 1, 0, 0), (2, 1, 2, 2)]),
+    ("soc", "eri3soc", [(2, 1, 3), (1, 2, 0), (1, 1, 4), (0, 2, 2)]),
+    ("d2", "eri2d2", [(3, 1), (0, 2), (2, 2), (4, 3)]),
+])
+def test_soc_d2_blocks(lbx, oracle, op, which, classes):
+    """lbx_eri4soc_batch / lbx_eri3soc_batch / lbx_eri2d2_batch against the oracle's kernels
+    (the reference's ERI4SOC / ERI3SOC / ERI2D2 restated) on O3 / def2-svp + def2-universal-jkfit,
+    entries in both orders of every pair (the spin-orbit blocks change sign when a and b swap);
+    the reference's deploy checksums of the three run through the C++ interface test."""
+    m = mol("def2-svp", O3, "def2-universal-jkfit")
+    ctx = lbx.Context(m)
+    s = oracle.Structure(m)
+    l, _ = shell_offsets(m.shells)
+    la, _ = shell_offsets(m.aux)
+    rng = np.random.default_rng(len(which))
+    pairs = []
+    for cls in classes:
+        per = len(cls)
+        cols = []
+        for k, x in enumerate(cls):
+            aux = per == 2 or (per == 3 and k == 2)
+            cols.append(rng.choice(np.flatnonzero((la if aux else l) == x), 25))
+        q = np.stack(cols, axis=1)
+        got = ctx.blocks(q, op=op).cpu().numpy()
+        ref = s.blocks(which, q, [2 * x + 1 for x in cls])
+        assert got.shape == ref.shape
+        pairs.append((got, ref))
+    scale = max(np.abs(r).max() for _, r in pairs)
+    assert scale > 0
+    for got, ref in pairs:
+        close_block(got, ref, scale)
+
+
 CLASS_PAIRS = [(la, lb, lc, ld) for la in range(4) for lb in range(la + 1)
                for lc in range(4) for ld in range(lc + 1)
                if la * (la + 1) // 2 + lb >= lc * (lc + 1) // 2 + ld]
