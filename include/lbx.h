@@ -112,6 +112,14 @@ int lbx_eri2d1_batch(lbx_context *ctx, long n, const int *shells, double *out);
  *   eri2d2  (ncomp 36) replaces ERI2D2Kernel::operator() (eri_kernels.hpp:171-186,
  *                      eri_kernel_funs.cpp:343-433): d2(a|b)/dX_i dX_j, i and j over
  *                      [Ax, Ay, Az, Bx, By, Bz], component 6 i + j */
+/* lbx_eri4_batch that allocates nothing.  No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Same arguments, layout and results as lbx_eri4_batch; the per-call
+ * task list is staged in a device array of the library (one per GPU, held until the process
+ * ends) instead of a buffer of the call, so n is at most 1048576 (LBX_ERR_INVALID beyond).
+ * Calls on one GPU share that array: calls that overlap on one GPU (from two contexts on it, on
+ * different threads) race; calls on one context are serialised by its mutex. */
+int lbx_eri4_batch_static(lbx_context *ctx, long n, const int *shells, double *out);
 int lbx_eri4soc_batch(lbx_context *ctx, long n, const int *shells, double *out);
 int lbx_eri3soc_batch(lbx_context *ctx, long n, const int *shells, double *out);
 int lbx_eri2d2_batch(lbx_context *ctx, long n, const int *shells, double *out);

@@ -15,6 +15,13 @@
 #include <tuple>
 #include <unordered_map>
 
+// Staging area of the allocation-free block entry (lbx_eri4_batch_static): one call's task list,
+// four ints per entry, at most kStaticEntries entries (16 MiB).  Each device holds its own copy
+// for the life of the process.  Host code never passes this name to a kernel or a copy: it uses
+// the address hipGetSymbolAddress returned for the device (scratchAddress below).
+constexpr long long kStaticEntries = 1LL << 20;
+static __device__ __align__(256) int scratch[4 * kStaticEntries];
+
 namespace lbx
 {
     namespace
@@ -86,6 +93,24 @@ namespace lbx
             hipLaunchKernelGGL(fillKernel, dim3(static_cast<int>(std::min(b, 65536LL))), dim3(256), 0, s,
                                p, n, v);
             LBX_HIP(hipGetLastError());
+        }
+
+        // Device address of the scratch array on device dev, looked up once per device with that
+        // device current (context set-up) and kept for every later context on it
+        int *scratchAddress(int dev)
+        {
+            static std::mutex m;
+            static std::map<int, int *> addr;
+            std::lock_guard<std::mutex> lk(m);
+            auto it = addr.find(dev);
+            if (it != addr.end())
+                return it->second;
+            void *p = nullptr;
+            LBX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+            if (!p)
+                throw Error(LBX_ERR_HIP, "hipGetSymbolAddress(scratch) returned a null address");
+            addr[dev] = static_cast<int *>(p);
+            return static_cast<int *>(p);
         }
 
         // Boys table, one copy per device for the life of the process
@@ -623,6 +648,8 @@ struct lbx_context : Context
     int filter_i = -1, filter_j = -1;
     // lbx_eri*_batch: (shell a, shell b) -> pair index per main pair class
     std::vector<std::unordered_map<long long, int>> pair_maps;
+    // this device's scratch array (lbx_eri4_batch_static)
+    int *d_scratch = nullptr;
 
     int variantFor(int i, int j)
     {
@@ -1059,6 +1086,7 @@ namespace
     {
         Context &c = ctx;
         LBX_HIP(hipGetDevice(&c.device));
+        ctx.d_scratch = scratchAddress(c.device);
         c.natoms = main_basis.natoms;
         c.shells = buildShells(main_basis, c.nbf, c.max_l, kMaxLMain);
         if (c.shells.empty())
@@ -1467,7 +1495,8 @@ extern "C"
         }
 
         void runBlocks(lbx_context &c, std::vector<std::unordered_map<long long, int>> &maps, int kind,
-                       int op, long long nq, const int *sh, double *out, const char *who)
+                       int op, long long nq, const int *sh, double *out, const char *who,
+                       int *staging = nullptr)
         {
             const int per = kind == 4 ? 4 : (kind == 3 ? 3 : 2);
             const int nsh = static_cast<int>(c.shells.size()), nax = static_cast<int>(c.shells_aux.size());
@@ -1558,8 +1587,11 @@ extern "C"
                 if (kind == 4 && op == D1_OP_DERIV)
                     computeEcoeffsD1(c, *ket);
             }
-            void *d = nullptr;
-            LBX_HIP(hipMalloc(&d, tasks.size() * sizeof(int)));
+            // the task list: in a buffer of this call, or in the device's scratch array
+            // (staging != nullptr: the allocation-free entry, at most kStaticEntries entries)
+            void *d = staging;
+            if (!staging)
+                LBX_HIP(hipMalloc(&d, tasks.size() * sizeof(int)));
             try
             {
                 LBX_HIP(hipMemcpyAsync(d, tasks.data(), tasks.size() * sizeof(int), hipMemcpyHostToDevice,
@@ -1588,26 +1620,38 @@ extern "C"
             catch (...)
             {
                 (void)hipStreamSynchronize(c.stream);
-                (void)hipFree(d);
+                if (!staging)
+                    (void)hipFree(d);
                 throw;
             }
-            (void)hipFree(d);
+            if (!staging)
+                (void)hipFree(d);
         }
 
         int blocksEntry(lbx_context *ctx, int kind, int op, long nq, const int *shells, double *out,
-                        const char *who)
+                        const char *who, bool no_alloc = false)
         {
             return guarded([&]
                            {
                                if (!ctx || nq < 0 || (nq > 0 && (!shells || !out)))
                                    throw Error(LBX_ERR_INVALID, std::string(who) + ": null argument or negative count");
+                               if (no_alloc && nq > kStaticEntries)
+                                   throw Error(LBX_ERR_INVALID, std::string(who) + ": " + std::to_string(nq) +
+                                                                        " entries, at most " +
+                                                                        std::to_string(kStaticEntries) + " per call");
                                std::lock_guard<std::mutex> lk(ctx->mu);
                                requireSingle(*ctx, who);
                                if (kind != 4 && ctx->shells_aux.empty())
                                    throw Error(LBX_ERR_STATE, std::string(who) + ": RI approximation is not enabled");
-                               runBlocks(*ctx, ctx->pair_maps, kind, op, nq, shells, out, who);
+                               runBlocks(*ctx, ctx->pair_maps, kind, op, nq, shells, out, who,
+                                         no_alloc ? ctx->d_scratch : nullptr);
                            });
         }
+    }
+
+    int lbx_eri4_batch_static(lbx_context *ctx, long nq, const int *shells, double *out)
+    {
+        return blocksEntry(ctx, 4, D1_OP_PLAIN, nq, shells, out, "lbx_eri4_batch_static", true);
     }
 
     int lbx_eri4_batch(lbx_context *ctx, long nq, const int *shells, double *out)

@@ -22,7 +22,7 @@ SYMBOLS = ["lbx_create", "lbx_destroy", "lbx_last_error", "lbx_dims", "lbx_set_s
            "lbx_synchronize", "lbx_boys", "lbx_eri4", "lbx_eri4_diag", "lbx_eri3", "lbx_eri2",
            "lbx_eri2_diag", "lbx_eri4d1", "lbx_eri3d1", "lbx_eri2d1", "lbx_eri4_batch",
            "lbx_eri3_batch", "lbx_eri2_batch", "lbx_eri4d1_batch", "lbx_eri3d1_batch",
-           "lbx_eri2d1_batch", "lbx_eri4soc_batch", "lbx_eri3soc_batch", "lbx_eri2d2_batch", "lbx_jk", "lbx_create_batch", "lbx_ri_sums", "lbx_jk_grad",
+           "lbx_eri2d1_batch", "lbx_eri4_batch_static", "lbx_eri4soc_batch", "lbx_eri3soc_batch", "lbx_eri2d2_batch", "lbx_jk", "lbx_create_batch", "lbx_ri_sums", "lbx_jk_grad",
            "lbx_tune", "lbx_set_variant", "lbx_profile_enable",
            "lbx_profile_read", "lbx_set_class_filter"]
 VARIANTS = ("valu-bra", "valu-ket", "mfma", "valu-bra-f32", "valu-ket-f32", "mfma-f32", "block",
@@ -83,7 +83,7 @@ def lib():
                      "lbx_eri4d1", "lbx_eri3d1", "lbx_eri2d1"):
             getattr(L, name).argtypes = [p, p]
         for name in ("lbx_eri4_batch", "lbx_eri3_batch", "lbx_eri2_batch", "lbx_eri4d1_batch",
-                     "lbx_eri3d1_batch", "lbx_eri2d1_batch", "lbx_eri4soc_batch",
+                     "lbx_eri3d1_batch", "lbx_eri2d1_batch", "lbx_eri4_batch_static", "lbx_eri4soc_batch",
                      "lbx_eri3soc_batch", "lbx_eri2d2_batch"):
             getattr(L, name).argtypes = [p, l, p, p]
         L.lbx_jk.argtypes = [p, p, p, p, d, i, d, ctypes.POINTER(JKStats)]
