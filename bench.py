@@ -61,6 +61,8 @@ def cpu_baseline(mol, D, gpu_stats, target_s: float):
         "unit": "quartets/s",
         "cores": cores,
         "kind": "port",
+        "path": "the oracle's restatement of the reference's generic-L CPU path "
+                "(eri_kernel_funs.cpp:7-101, triple-loop GEMMs; not its rolled-out L <= 6 kernels)",
         "sample": (f"every {stride}th bra shell pair of each class pair: {st['quartets']:.0f} of "
                    f"{gpu_stats['quartets']:.0f} screened quartets, {st['seconds_jk']:.1f} s "
                    f"(+{st['seconds_setup']:.1f} s pair data and Schwarz factors, not counted)"),
@@ -105,6 +107,144 @@ def timed(fn, steps: int, label: str):
         log(f"[bench] {label} step {k} done")
     torch.cuda.synchronize()
     return (time.perf_counter() - t0) / steps, res
+
+
+CLASS_PAIRS = [(la, lb, lc, ld) for la in range(4) for lb in range(la + 1)
+               for lc in range(4) for ld in range(lc + 1)
+               if la * (la + 1) // 2 + lb >= lc * (lc + 1) // 2 + ld]
+
+
+def fp32_tier_error(ctx, D, per_class: bool) -> dict:
+    """The error of the FP32 tier alone: the quartets with Schwarz bound in [SCHWARZ, MIXED_SPLIT)
+    built in FP64 and in FP32 (lbx_set_bound_window), so the partial J and K are small and their
+    FP64 rounding does not hide the difference; per class pair with per_class (max |dJ|, max |dK|
+    and the energy change of each class pair's FP32 quartets)."""
+    from lible_amd import lbx
+
+    def one():
+        J, K, st = ctx.jk(D, schwarz=SCHWARZ)
+        Jm, Km, sm = ctx.jk(D, schwarz=SCHWARZ, precision=lbx.LBX_MIXED, mixed_thresh=MIXED_SPLIT)
+        r = {"quartets": sm["quartets_fp32"], "delta_E": energy_difference(D, J, K, Jm, Km),
+             "max_abs_dJ": float((Jm - J).abs().max()), "max_abs_dK": float((Km - K).abs().max())}
+        return r
+
+    ctx.set_bound_window(0.0, MIXED_SPLIT)
+    try:
+        res = one()
+        if per_class:
+            rows = []
+            for c in CLASS_PAIRS:
+                ctx.set_class_filter(c)
+                r = one()
+                if r["quartets"] > 0:
+                    rows.append({"class": "".join(map(str, c)), **r})
+            ctx.set_class_filter(None)
+            res["per_class"] = rows
+    finally:
+        ctx.set_class_filter(None)
+        ctx.set_bound_window()
+    return res
+
+
+def leg_c1() -> dict:
+    """Dense eri4 of the 6-atom, 84-function workload (3 s(K3), 2 p(K2), 1 d per atom)."""
+    import torch
+    from lible_amd import basis as B
+    from lible_amd import lbx
+    mol = B.config_molecule("c1_eri4_dense")
+    ctx = lbx.Context(mol)
+    n = ctx.nbf
+    out = torch.empty((n, n, n, n), dtype=torch.float64, device="cuda")
+    ctx.eri4(out)
+    torch.cuda.synchronize()
+    t, _ = timed(lambda: ctx.eri4(out), 3, "c1 eri4")
+    s = float(out.abs().sum())
+    ctx.close()
+    log("[bench] c1 done")
+    return {"workload": "c1_eri4_dense", "nbf": n, "ms": t * 1e3, "elements_per_s": n ** 4 / t,
+            "sum_abs": s}
+
+
+def leg_c3() -> dict:
+    """120 atoms, 3600 functions: one FP64 and one mixed build (after a first call of each that
+    builds the task tables), the energy change of the mixed build, and the FP32 tier's own
+    error (bound window)."""
+    import torch
+    from lible_amd import basis as B
+    from lible_amd import lbx
+    cfg = B.CONFIGS["c3_jk_mixed"]
+    mol = B.config_molecule("c3_jk_mixed")
+    ctx = lbx.Context(mol)
+    D = torch.tensor(B.density_from_orbitals(cfg["seed"] + 100, ctx.nbf, cfg["nocc"], cfg["sigma"]),
+                     device="cuda")
+    J, K = torch.empty_like(D), torch.empty_like(D)
+    t0 = time.perf_counter()
+    ctx.jk(D, schwarz=SCHWARZ, J=J, K=K)
+    torch.cuda.synchronize()
+    first = time.perf_counter() - t0
+    log(f"[bench] c3 first FP64 call {first:.1f} s")
+    t64, (_, _, st) = timed(lambda: ctx.jk(D, schwarz=SCHWARZ, J=J, K=K), 1, "c3 fp64")
+    Jm, Km = torch.empty_like(D), torch.empty_like(D)
+    ctx.jk(D, schwarz=SCHWARZ, precision=lbx.LBX_MIXED, mixed_thresh=MIXED_SPLIT, J=Jm, K=Km)
+    torch.cuda.synchronize()
+    log("[bench] c3 first mixed call done")
+    tm, (_, _, sm) = timed(lambda: ctx.jk(D, schwarz=SCHWARZ, precision=lbx.LBX_MIXED,
+                                          mixed_thresh=MIXED_SPLIT, J=Jm, K=Km), 1, "c3 mixed")
+    dE = energy_difference(D, J, K, Jm, Km)
+    E = 0.5 * float((D * J).sum()) - 0.25 * float((D * K).sum())
+    res = {"workload": "c3_jk_mixed", "nbf": ctx.nbf, "atoms": mol.natoms,
+           "quartets": st["quartets"], "fp64_ms": t64 * 1e3, "mixed_ms": tm * 1e3,
+           "speedup_vs_fp64": t64 / tm, "fp32_share": sm["quartets_fp32"] / sm["quartets"],
+           "fp64_tflops": st["flops"] / t64 / 1e12, "first_call_s": first,
+           "energy_2e": E, "delta_E": dE,
+           "max_abs_dJ": float((Jm - J).abs().max()), "max_abs_dK": float((Km - K).abs().max())}
+    del Jm, Km, J, K
+    res["fp32_tier"] = fp32_tier_error(ctx, D, per_class=False)
+    ctx.close()
+    log("[bench] c3 done")
+    return res
+
+
+def leg_c4() -> dict:
+    """The 256-molecule RI batch: set-up, first call (task tables, graph capture), two replays."""
+    import torch
+    from lible_amd import basis as B
+    from lible_amd import lbx
+    mols = B.batch_molecules()
+    ctx = lbx.Context.batch(mols)
+    out = torch.empty((len(mols), 4), dtype=torch.float64, device="cuda")
+    ctx.ri_sums(out)
+    torch.cuda.synchronize()
+    t, (_, st) = timed(lambda: ctx.ri_sums(out), 2, "c4 ri")
+    ctx.close()
+    log("[bench] c4 done")
+    return {"workload": "c4_ri_batch", "molecules": len(mols), "ms": t * 1e3,
+            "triplets_per_s": st["triplets"] / t, "doublets_per_s": st["doublets"] / t,
+            "tflops": st["flops"] / t / 1e12}
+
+
+def leg_c5() -> dict:
+    """The 60-atom nuclear gradient, one pass after the first call."""
+    import torch
+    from lible_amd import basis as B
+    from lible_amd import lbx
+    cfg = B.CONFIGS["c5_gradient"]
+    mol = B.config_molecule("c5_gradient")
+    ctx = lbx.Context(mol)
+    D = torch.tensor(B.density_from_orbitals(cfg["seed"] + 100, ctx.nbf, cfg["nocc"], cfg["sigma"]),
+                     device="cuda")
+    g, _ = ctx.jk_grad(D, schwarz=SCHWARZ)
+    torch.cuda.synchronize()
+    t, (_, st) = timed(lambda: ctx.jk_grad(D, schwarz=SCHWARZ, grad=g), 1, "c5 gradient")
+    gh = g.cpu().numpy()
+    ctx.close()
+    log("[bench] c5 done")
+    return {"workload": "c5_gradient", "ms": t * 1e3, "quartets_per_s": st["quartets"] / t,
+            "tflops": st["flops"] / t / 1e12,
+            "max_abs_translation_sum": float(abs(gh.sum(axis=0)).max())}
+
+
+LEGS = {"c1": leg_c1, "c3": leg_c3, "c4": leg_c4, "c5": leg_c5}
 
 
 def bench_ri_batch(a):
@@ -269,6 +409,10 @@ def main(argv=None):
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--mixed-steps", type=int, default=1,
                     help="timed mixed-precision builds after the FP64 ones (0: skip)")
+    ap.add_argument("--legs", default="c1,c3,c4",
+                    help="other workloads measured after the main one, one pass each: c1 (dense "
+                         "eri4, 84 functions), c3 (120-atom FP64 and mixed J/K), c4 (256-molecule "
+                         "RI batch), c5 (60-atom gradient)")
     ap.add_argument("--pmc-calibration", action="store_true",
                     help="end with a 1 GiB device-to-device copy, the reference reading for "
                          "rocprofv3's FETCH_SIZE / WRITE_SIZE counters")
@@ -355,6 +499,8 @@ def main(argv=None):
         torch.cuda.synchronize()
         m_s = (time.perf_counter() - t0) / a.mixed_steps
         dE = energy_difference(D, J, K, Jm, Km)
+        tier = fp32_tier_error(ctx, D, per_class=True)
+        log("[bench] FP32-tier error per class pair done")
         mixed = {
             "split": MIXED_SPLIT,
             "ms_per_step": m_s * 1e3,
@@ -365,7 +511,9 @@ def main(argv=None):
             "delta_E_rel": dE / abs(E),
             "max_abs_dJ": float((Jm - J).abs().max()),
             "max_abs_dK": float((Km - K).abs().max()),
+            "fp32_tier": tier,
         }
+        del Jm, Km
 
     out = {
         "metric": METRIC,
@@ -394,14 +542,19 @@ def main(argv=None):
         "total_frac_fp64_peak": tot_tflops / FP64_PEAK_TFLOPS,
         "energy_2e": E,
         "roofline": {
-            # FP64 compute bound: MI355X's 78.6 TFLOP/s FP64 peak is the same for MFMA and VALU
+            # FP64 compute bound (MI355X's 78.6 TFLOP/s FP64 peak is the same for the matrix and
+            # the vector units); compute_unit says which of them the kernel's transforms use
             "bound": "mfma",
+            "compute_unit": "valu" if dom["kernel"].startswith("valu") else "mfma",
             "kernel": kernel_name(dom),
             "achieved": dom_tflops,
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": dom_tflops / FP64_PEAK_TFLOPS,
+            # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs of
+            # this kernel (not measured in this run; null when they cover another kernel)
             "traffic": measured_traffic(kernel_name(dom)),
+            "traffic_source": "profiles/jk_c2_traffic.json (separate rocprofv3 counter runs)",
             "kernel_ms": dom_ms,
             "kernel_share": dom["ms"] / max(sum(p["ms"] for p in prof), 1e-12),
         },
@@ -419,6 +572,15 @@ def main(argv=None):
     if not a.no_cpu_baseline:
         log("[bench] cpu baseline (oracle sample)")
         out["cpu_baseline"] = cpu_baseline(mol, Dh, st, a.cpu_seconds)
+    # the other workloads, one measured pass each, so the driver's run records them too
+    ctx.close()
+    del ctx, D, J, K
+    torch.cuda.empty_cache()
+    legs = {}
+    for leg in [x for x in a.legs.split(",") if x]:
+        legs[leg] = LEGS[leg]()
+        torch.cuda.empty_cache()
+    out["legs"] = legs
     print(json.dumps(out), flush=True)
     if a.pmc_calibration:
         src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")

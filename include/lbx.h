@@ -112,6 +112,10 @@ int lbx_eri2d1_batch(lbx_context *ctx, long n, const int *shells, double *out);
  *   eri2d2  (ncomp 36) replaces ERI2D2Kernel::operator() (eri_kernels.hpp:171-186,
  *                      eri_kernel_funs.cpp:343-433): d2(a|b)/dX_i dX_j, i and j over
  *                      [Ax, Ay, Az, Bx, By, Bz], component 6 i + j */
+int lbx_eri4soc_batch(lbx_context *ctx, long n, const int *shells, double *out);
+int lbx_eri3soc_batch(lbx_context *ctx, long n, const int *shells, double *out);
+int lbx_eri2d2_batch(lbx_context *ctx, long n, const int *shells, double *out);
+
 /* lbx_eri4_batch that allocates nothing.  No counterpart in the reference: for callers that
  * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
  * that forbid hipMalloc).  Same arguments, layout and results as lbx_eri4_batch; the per-call
@@ -120,9 +124,6 @@ int lbx_eri2d1_batch(lbx_context *ctx, long n, const int *shells, double *out);
  * Calls on one GPU share that array: calls that overlap on one GPU (from two contexts on it, on
  * different threads) race; calls on one context are serialised by its mutex. */
 int lbx_eri4_batch_static(lbx_context *ctx, long n, const int *shells, double *out);
-int lbx_eri4soc_batch(lbx_context *ctx, long n, const int *shells, double *out);
-int lbx_eri3soc_batch(lbx_context *ctx, long n, const int *shells, double *out);
-int lbx_eri2d2_batch(lbx_context *ctx, long n, const int *shells, double *out);
 
 /* Coulomb and exchange matrices, no reference counterpart (the reference stops at the dense
  * tensor).  D, J, K: device (nbf,nbf) row-major, D symmetric.
@@ -139,6 +140,25 @@ typedef struct lbx_jk_stats {
 } lbx_jk_stats;
 int lbx_jk(lbx_context *ctx, const double *D, double *J, double *K, double schwarz_thresh,
            int precision, double mixed_thresh, lbx_jk_stats *stats);
+
+/* Density-fitted (RI) Coulomb and exchange, no reference counterpart (the consumer of the
+ * reference's eri3() / eri2(), eri3.cpp:7-46, eri2.cpp:27-72): with V = (P|Q) = L L^T,
+ *   J_mn = sum_PQ (mn|P) (V^-1)_PQ sum_kl (Q|kl) D_kl
+ *   K_mk = sum_P sum_nl W_P(mn) D_nl W_P(kl),   W_P = sum_Q (L^-1)_PQ (Q|..)
+ * D, J, K: device (nbf, nbf) row-major, D symmetric; K may be NULL (J only).  Needs an
+ * auxiliary basis.  Works on the dense 3-index tensor: about 8 nbf^2 naux bytes of device memory
+ * for J, twice that with K, allocated for the call (LBX_ERR_NOMEM when it does not fit);
+ * LBX_ERR_STATE when (P|Q) is not numerically positive definite. */
+int lbx_ri_jk(lbx_context *ctx, const double *D, double *J, double *K);
+
+/* One-electron integrals over the main basis, dense (nbf, nbf) row-major device matrices, each
+ * optional (NULL: not computed): the overlap S (replaces overlap(), oneelints.cpp:169-197), the
+ * kinetic energy T (kineticEnergy(), oneelints.cpp:348-376) and the attraction of an electron to
+ * ncharges point charges V (externalCharges(), oneelints.cpp:1087-1120; nuclearAttraction(),
+ * oneelints.cpp:1703-1744, is the case of the nuclei with q = Z).  charges: host, (x, y, z, q)
+ * per charge, bohr; ncharges = 0 gives V = 0.  Outputs are overwritten. */
+int lbx_one_electron(lbx_context *ctx, double *S, double *T, int ncharges, const double *charges,
+                     double *V);
 
 /* Batched RI integrals over many independent molecules (no reference counterpart: the
  * reference runs eri3() / eri2() one Structure at a time, eri3.cpp:7-46, eri2.cpp:27-72).
@@ -217,6 +237,14 @@ int lbx_profile_read(lbx_context *ctx, lbx_class_profile *out, int capacity, int
  * the model.  J and K then hold that class pair's contribution only (symmetrised), which gives
  * per-class-pair error and timing reports. */
 int lbx_set_class_filter(lbx_context *ctx, int la, int lb, int lc, int ld);
+
+/* Restricts the following lbx_jk calls on this context to the quartets whose Schwarz bound
+ * Q_ab Q_cd lies in [lo, hi) (besides the call's own threshold; with LBX_MIXED each precision
+ * tier keeps its part of the window); lo = 0, hi = INFINITY lifts it.  No reference
+ * counterpart: it isolates the error of the FP32 tier of a mixed-precision build (window
+ * [threshold, split): the same quartets in FP64 and in FP32, with partial sums small enough
+ * that FP64 rounding does not hide the difference). */
+int lbx_set_bound_window(lbx_context *ctx, double lo, double hi);
 
 #ifdef __cplusplus
 }

@@ -56,8 +56,11 @@ def runtime_link() -> list[str]:
     except Exception:
         tlib = None
     if tlib and os.path.exists(os.path.join(tlib, "libamdhip64.so")):
-        return ["-no-hip-rt", f"-L{tlib}", "-l:libamdhip64.so", f"-Wl,-rpath,{tlib}"]
-    return []
+        # rocBLAS / rocSOLVER (density-fitted J/K) from the same directory, built against the
+        # same runtime
+        return ["-no-hip-rt", f"-L{tlib}", "-l:libamdhip64.so", "-l:librocblas.so", "-l:librocsolver.so",
+                f"-Wl,-rpath,{tlib}"]
+    return ["-L/opt/rocm/lib", "-lrocblas", "-lrocsolver"]
 
 
 def _big_stack():
@@ -103,6 +106,7 @@ def build(jobs: int | None = None, verbose: bool = True, incremental: bool = Fal
         (os.path.join(CSRC, "capi.hip"), os.path.join(BUILD, "capi.o"), []),
         (os.path.join(CSRC, "ints_api.cpp"), os.path.join(BUILD, "ints_api.o"), []),
         (os.path.join(CSRC, "d1.hip"), os.path.join(BUILD, "d1.o"), []),
+        (os.path.join(CSRC, "ri.hip"), os.path.join(BUILD, "ri.o"), []),
     ]
     for u in range(N_UNITS):
         work.append((os.path.join(CSRC, "instances.hip"), os.path.join(BUILD, f"inst_{u}.o"),

@@ -646,6 +646,8 @@ struct lbx_context : Context
     std::map<std::pair<int, int>, int> tuned, tuned32;
     // lbx_set_class_filter: the one class pair (bra class, ket class) lbx_jk evaluates; -1: all
     int filter_i = -1, filter_j = -1;
+    // lbx_set_bound_window: lbx_jk evaluates only quartets with window_lo <= Q_ab Q_cd < window_hi
+    double window_lo = 0.0, window_hi = kNoUpper;
     // lbx_eri*_batch: (shell a, shell b) -> pair index per main pair class
     std::vector<std::unordered_map<long long, int>> pair_maps;
     // this device's scratch array (lbx_eri4_batch_static)
@@ -1787,9 +1789,13 @@ extern "C"
                     continue;
                 if (filter_i >= 0 && (i != filter_i || j != filter_j))
                     continue;
-                launch(i, j, 0, variantFor(i, j), split, kNoUpper);
-                if (mixed)
-                    launch(i, j, 1, variant32For(i, j), schwarz_thresh, split);
+                // the tiers [split, inf) and [schwarz, split), cut to the bound window
+                const double lo0 = std::max(split, window_lo), hi0 = window_hi;
+                const double lo1 = std::max(schwarz_thresh, window_lo), hi1 = std::min(split, window_hi);
+                if (lo0 < hi0)
+                    launch(i, j, 0, variantFor(i, j), lo0, hi0);
+                if (mixed && lo1 < hi1)
+                    launch(i, j, 1, variant32For(i, j), lo1, hi1);
             }
         const int gridn = static_cast<int>(std::min<long long>((n * n + 255) / 256, 65536));
         hipLaunchKernelGGL(symmetrizeKernel, dim3(gridn), dim3(256), 0, c.stream, J, static_cast<int>(n));
@@ -1823,7 +1829,8 @@ extern "C"
                            // Replays: the launch sequence of a (D, J, K, thresholds) set is
                            // captured into a HIP graph after its first (eager) build and replayed
                            // on later calls; per-class timing and the class filter run eagerly.
-                           const bool graphs = !ctx->profiling && ctx->filter_i < 0;
+                           const bool graphs = !ctx->profiling && ctx->filter_i < 0 && ctx->window_lo <= 0.0 &&
+                                               ctx->window_hi == kNoUpper;
                            const lbx_context::JKKey gkey{D, J, K, schwarz_thresh, split, mixed ? 1 : 0};
                            if (graphs)
                            {
@@ -2098,6 +2105,88 @@ extern "C"
                                throw Error(LBX_ERR_INVALID, "lbx_set_class_filter: not a canonical class pair");
                            ctx->filter_i = pairClassIdx(la, lb);
                            ctx->filter_j = pairClassIdx(lc, ld);
+                       });
+    }
+
+    int lbx_ri_jk(lbx_context *ctx, const double *D, double *J, double *K)
+    {
+        return guarded([&]
+                       {
+                           if (!ctx || !D || !J)
+                               throw Error(LBX_ERR_INVALID, "lbx_ri_jk: null argument");
+                           long long n = 0, naux = 0;
+                           hipStream_t s = nullptr;
+                           {
+                               std::lock_guard<std::mutex> lk(ctx->mu);
+                               requireSingle(*ctx, "lbx_ri_jk");
+                               if (ctx->shells_aux.empty())
+                                   throw Error(LBX_ERR_STATE, "lbx_ri_jk: the context has no auxiliary basis");
+                               n = static_cast<long long>(ctx->nbf);
+                               naux = static_cast<long long>(ctx->nbf_aux);
+                               s = ctx->stream;
+                           }
+                           riJK(ctx, s, n, naux, D, J, K, nullptr);
+                       });
+    }
+
+    int lbx_one_electron(lbx_context *ctx, double *S, double *T, int ncharges, const double *charges, double *V)
+    {
+        return guarded([&]
+                       {
+                           if (!ctx || (!S && !T && !V))
+                               throw Error(LBX_ERR_INVALID, "lbx_one_electron: null context or no output");
+                           if (V && (ncharges < 0 || (ncharges > 0 && !charges)))
+                               throw Error(LBX_ERR_INVALID, "lbx_one_electron: bad point charges");
+                           std::lock_guard<std::mutex> lk(ctx->mu);
+                           Context &c = *ctx;
+                           requireSingle(c, "lbx_one_electron");
+                           const size_t bytes = c.nbf * c.nbf * sizeof(double);
+                           for (double *m : {S, T, V})
+                               if (m)
+                                   LBX_HIP(hipMemsetAsync(m, 0, bytes, c.stream));
+                           double4 *d_q = nullptr;
+                           if (V && ncharges > 0)
+                           {
+                               std::vector<double4> q(ncharges);
+                               for (int k = 0; k < ncharges; k++)
+                                   q[k] = double4{charges[4 * k], charges[4 * k + 1], charges[4 * k + 2],
+                                                  charges[4 * k + 3]};
+                               LBX_HIP(hipMalloc(&d_q, q.size() * sizeof(double4)));
+                               const hipError_t e = hipMemcpyAsync(d_q, q.data(), q.size() * sizeof(double4),
+                                                                   hipMemcpyHostToDevice, c.stream);
+                               if (e != hipSuccess)
+                               {
+                                   (void)hipFree(d_q);
+                                   LBX_HIP(e);
+                               }
+                           }
+                           try
+                           {
+                               computeOneElectron(c, S, T, d_q, d_q ? ncharges : 0, V);
+                               LBX_HIP(hipStreamSynchronize(c.stream));
+                           }
+                           catch (...)
+                           {
+                               (void)hipStreamSynchronize(c.stream);
+                               (void)hipFree(d_q);
+                               throw;
+                           }
+                           if (d_q)
+                               LBX_HIP(hipFree(d_q));
+                       });
+    }
+
+    int lbx_set_bound_window(lbx_context *ctx, double lo, double hi)
+    {
+        return guarded([&]
+                       {
+                           if (!ctx)
+                               throw Error(LBX_ERR_INVALID, "lbx_set_bound_window: null context");
+                           if (!(lo >= 0.0) || !(hi > lo))
+                               throw Error(LBX_ERR_INVALID, "lbx_set_bound_window: need 0 <= lo < hi");
+                           std::lock_guard<std::mutex> lk(ctx->mu);
+                           ctx->window_lo = lo;
+                           ctx->window_hi = hi;
                        });
     }
 

@@ -294,6 +294,198 @@ namespace lbx
             throw Error(LBX_ERR_INVALID, "no E-matrix kernel for this angular momentum pair");
         }
 
+        // ---- one-electron integrals (overlap, kinetic energy, attraction to point charges) ----
+        // E^{ij}_0 alone: with E^{ij}_1 = (i E^{i-1,j}_0 + j E^{i,j-1}_0) / (2p) the t = 0 row of
+        // the Hermite recurrences closes on itself (the Obara-Saika overlap recurrence, HJO
+        // eq. 9.3.8), so S holds (LA+1)(LB+1) values instead of the full E^{ij}_t table.
+        template <int LA, int LB>
+        __device__ __forceinline__ void overlap1d(double (&S)[LA + 1][LB + 1], double K, double PA, double PB,
+                                                  double o2p)
+        {
+            S[0][0] = K;
+            for (int i = 0; i < LA; i++)
+                S[i + 1][0] = PA * S[i][0] + (i > 0 ? i * o2p * S[i - 1][0] : 0.0);
+            for (int j = 0; j < LB; j++)
+                for (int i = 0; i <= LA; i++)
+                {
+                    double v = PB * S[i][j];
+                    if (i > 0)
+                        v += i * o2p * S[i - 1][j];
+                    if (j > 0)
+                        v += j * o2p * S[i][j - 1];
+                    S[i][j + 1] = v;
+                }
+        }
+
+        // Overlap and kinetic energy of one (primitive pair, mu nu) from the Cartesian 1-D
+        // Hermite coefficients E^{ij}_0 (oneelints.cpp:57-100, 199-276): per axis
+        //   S_ij = E^{ij}_0,   T_ij = b (2j + 1) E^{ij}_0 - 2 b^2 E^{i,j+2}_0 - j (j - 1)/2 E^{i,j-2}_0
+        // (-1/2 d^2/dx^2 acting on b's factor), times (pi/p)^{3/2}, the contraction coefficients
+        // and the spherical norms; added into the dense S and T at (a, b) and (b, a).
+        template <int LA, int LB>
+        __global__ __launch_bounds__(256) void stKernel(int nppairs, const PrimIn *__restrict__ prim,
+                                                        const double *__restrict__ centers,
+                                                        const double *__restrict__ norms, const int *__restrict__ ao_a,
+                                                        const int *__restrict__ ao_b, const int *__restrict__ sh_a,
+                                                        const int *__restrict__ sh_b, long long n, double *S,
+                                                        double *T)
+        {
+            constexpr int SA = nSph(LA), SB = nSph(LB), SP = SA * SB, LB2 = LB + 2;
+            const long long gid = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+            if (gid >= static_cast<long long>(nppairs) * SP)
+                return;
+            const int pp = static_cast<int>(gid / SP), mn = static_cast<int>(gid % SP);
+            const int mu = mn / SB, nu = mn % SB;
+            const PrimIn pr = prim[pp];
+            const double a = pr.a, b = pr.b, p = a + b, o2p = 1.0 / (2.0 * p), mur = a * b / p;
+            const double *C = centers + 6 * pr.pair;
+            double E[3][LA + 1][LB2 + 1];
+            for (int x = 0; x < 3; x++)
+            {
+                const double P = (a * C[x] + b * C[3 + x]) / p, d = C[x] - C[3 + x];
+                overlap1d<LA, LB2>(E[x], exp(-mur * d * d), P - C[x], P - C[3 + x], o2p);
+            }
+            auto t1 = [&](int x, int i, int j) {
+                double t = b * (2 * j + 1) * E[x][i][j] - 2.0 * b * b * E[x][i][j + 2];
+                if (j >= 2)
+                    t -= 0.5 * j * (j - 1) * E[x][i][j - 2];
+                return t;
+            };
+            double s = 0.0, t = 0.0;
+            for (int ea = cSph.start[LA][mu]; ea < cSph.start[LA][mu + 1]; ea++)
+            {
+                int ix, iy, iz;
+                cartOf(LA, cSph.cart[ea], ix, iy, iz);
+                for (int eb_ = cSph.start[LB][nu]; eb_ < cSph.start[LB][nu + 1]; eb_++)
+                {
+                    int jx, jy, jz;
+                    cartOf(LB, cSph.cart[eb_], jx, jy, jz);
+                    const double v = cSph.val[ea] * cSph.val[eb_];
+                    const double sx = E[0][ix][jx], sy = E[1][iy][jy], sz = E[2][iz][jz];
+                    s += v * sx * sy * sz;
+                    t += v * (t1(0, ix, jx) * sy * sz + sx * t1(1, iy, jy) * sz + sx * sy * t1(2, iz, jz));
+                }
+            }
+            const double *nrm = norms + static_cast<size_t>(pr.pair) * (SA + SB);
+            const double sc = nrm[mu] * nrm[SA + nu] * pr.dadb * pow(M_PI / p, 1.5);
+            const long long r = ao_a[pr.pair] + mu, c = ao_b[pr.pair] + nu;
+            const bool both = sh_a[pr.pair] != sh_b[pr.pair];
+            if (S)
+            {
+                unsafeAtomicAdd(&S[r * n + c], sc * s);
+                if (both)
+                    unsafeAtomicAdd(&S[c * n + r], sc * s);
+            }
+            if (T)
+            {
+                unsafeAtomicAdd(&T[r * n + c], sc * t);
+                if (both)
+                    unsafeAtomicAdd(&T[c * n + r], sc * t);
+            }
+        }
+
+        // Attraction of an electron to point charges q_C at C (oneelints.cpp:508-585):
+        //   V(mu nu) = -(2 pi / p) sum_tuv E[tuv][mu nu] sum_C q_C R_tuv(p, P - C)
+        // with the pair's spherical E matrix (norms and coefficients included) and R from
+        // R^(n)_000 = (-2p)^n F_n(p |P - C|^2).
+        template <int LA, int LB>
+        __global__ __launch_bounds__(256) void vKernel(int nppairs, const PrimIn *__restrict__ prim,
+                                                       const double4 *__restrict__ ppd,
+                                                       const double *__restrict__ eb, const double4 *__restrict__ charges,
+                                                       int ncharges, const double *__restrict__ boys,
+                                                       const int *__restrict__ ao_a, const int *__restrict__ ao_b,
+                                                       const int *__restrict__ sh_a, const int *__restrict__ sh_b,
+                                                       long long n, double *V)
+        {
+            constexpr int SA = nSph(LA), SB = nSph(LB), SP = SA * SB, L = LA + LB, HP = nHerm(L);
+            const long long gid = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+            if (gid >= static_cast<long long>(nppairs) * SP)
+                return;
+            const int pp = static_cast<int>(gid / SP), mn = static_cast<int>(gid % SP);
+            const int mu = mn / SB, nu = mn % SB;
+            const PrimIn pr = prim[pp];
+            const double4 P = ppd[pp];
+            // this thread's column of E; the SP threads of one primitive pair read consecutive
+            // addresses, and re-reads per charge hit the cache
+            const double *e = eb + static_cast<size_t>(pp) * HP * SP + mn;
+            double v = 0.0;
+            for (int k = 0; k < ncharges; k++)
+            {
+                const double4 q = charges[k];
+                const double X = P.y - q.x, Y = P.z - q.y, Z = P.w - q.z;
+                double R[HP + L];
+                boysF<L>(P.x * (X * X + Y * Y + Z * Z), boys, R);
+                double sc = -2.0 * P.x;
+#pragma unroll
+                for (int m = 1; m <= L; m++)
+                {
+                    R[m] *= sc;
+                    sc *= -2.0 * P.x;
+                }
+                hermiteR<L, double>(R, X, Y, Z);
+                // R_tuv sits at slot 0 for tuv = 000 and at L + hermIdx(t, u, v) otherwise
+                double acc = e[0] * R[0];
+#pragma unroll
+                for (int h = 1; h < HP; h++)
+                    acc = fma(e[static_cast<size_t>(h) * SP], R[L + h], acc);
+                v = fma(q.w, acc, v);
+            }
+            v *= -2.0 * M_PI / P.x;
+            const long long r = ao_a[pr.pair] + mu, c = ao_b[pr.pair] + nu;
+            unsafeAtomicAdd(&V[r * n + c], v);
+            if (sh_a[pr.pair] != sh_b[pr.pair])
+                unsafeAtomicAdd(&V[c * n + r], v);
+        }
+
+        using StFn = void (*)(const PairClass &, long long, double *, double *, hipStream_t);
+        using VFn = void (*)(const PairClass &, const double4 *, int, const double *, long long, double *,
+                             hipStream_t);
+
+        template <int LA, int LB>
+        void launchST(const PairClass &pc, long long n, double *S, double *T, hipStream_t s)
+        {
+            const long long total = static_cast<long long>(pc.n_ppairs) * nSph(LA) * nSph(LB);
+            if (total == 0)
+                return;
+            hipLaunchKernelGGL((stKernel<LA, LB>), dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256), 0, s,
+                               pc.n_ppairs, pc.d_prim, pc.d_centers, pc.d_norms, pc.d_ao_a, pc.d_ao_b,
+                               pc.d_sh_a, pc.d_sh_b, n, S, T);
+        }
+
+        template <int LA, int LB>
+        void launchV(const PairClass &pc, const double4 *charges, int ncharges, const double *boys, long long n,
+                     double *V, hipStream_t s)
+        {
+            const long long total = static_cast<long long>(pc.n_ppairs) * nSph(LA) * nSph(LB);
+            if (total == 0)
+                return;
+            hipLaunchKernelGGL((vKernel<LA, LB>), dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256), 0, s,
+                               pc.n_ppairs, pc.d_prim, pc.d_pp, pc.d_eb, charges, ncharges, boys, pc.d_ao_a,
+                               pc.d_ao_b, pc.d_sh_a, pc.d_sh_b, n, V);
+        }
+
+        StFn stFn(int la, int lb)
+        {
+#define LBX_E(A, B)                                                                                \
+    if (la == A && lb == B)                                                                        \
+        return &launchST<A, B>;
+            LBX_E(0, 0) LBX_E(1, 0) LBX_E(1, 1) LBX_E(2, 0) LBX_E(2, 1) LBX_E(2, 2) LBX_E(3, 0)
+            LBX_E(3, 1) LBX_E(3, 2) LBX_E(3, 3)
+#undef LBX_E
+            throw Error(LBX_ERR_INVALID, "no one-electron kernel for this angular momentum pair");
+        }
+
+        VFn vFn(int la, int lb)
+        {
+#define LBX_E(A, B)                                                                                \
+    if (la == A && lb == B)                                                                        \
+        return &launchV<A, B>;
+            LBX_E(0, 0) LBX_E(1, 0) LBX_E(1, 1) LBX_E(2, 0) LBX_E(2, 1) LBX_E(2, 2) LBX_E(3, 0)
+            LBX_E(3, 1) LBX_E(3, 2) LBX_E(3, 3)
+#undef LBX_E
+            throw Error(LBX_ERR_INVALID, "no one-electron kernel for this angular momentum pair");
+        }
+
         void uploadSphTables()
         {
             static bool done = false;   // per process; the table is identical on every device
@@ -407,6 +599,24 @@ namespace lbx
         pc.d_e1k = c.alloc<double>(ne);
         ecoefD1Fn(pc.la, pc.lb)(pc.n_ppairs, pc.d_prim, pc.d_centers, pc.d_norms, pc.d_e1b, pc.d_e1k,
                                 c.stream);
+        LBX_HIP(hipGetLastError());
+    }
+
+    // Dense overlap, kinetic-energy and point-charge attraction matrices (any may be null),
+    // accumulated into zeroed n x n device buffers on the context's stream.  charges: device,
+    // (x, y, z, q) per charge.
+    void computeOneElectron(Context &c, double *S, double *T, const double4 *charges, int ncharges, double *V)
+    {
+        const long long n = static_cast<long long>(c.nbf);
+        for (const PairClass &pc : c.cls)
+        {
+            if (pc.n_ppairs == 0)
+                continue;
+            if (S || T)
+                stFn(pc.la, pc.lb)(pc, n, S, T, c.stream);
+            if (V && ncharges > 0)
+                vFn(pc.la, pc.lb)(pc, charges, ncharges, c.d_boys, n, V, c.stream);
+        }
         LBX_HIP(hipGetLastError());
     }
 }

@@ -172,6 +172,10 @@ namespace lbx
     // d E / d (A - B)_x,y,z and the per-pair atoms and a/p ratios for the nuclear gradient
     // (once per pair class)
     void computeEcoeffsD1(Context &c, PairClass &pc);
+    // dense overlap S, kinetic energy T and attraction V to ncharges point charges (device,
+    // (x, y, z, q) each) over the main basis, added into zeroed nbf x nbf device buffers (any
+    // may be null)
+    void computeOneElectron(Context &c, double *S, double *T, const double4 *charges, int ncharges, double *V);
 
     // schwarz / task tables (capi.cpp)
     void ensureSchwarz(Context &c);
@@ -202,4 +206,10 @@ namespace lbx
         D1_OP_D2 = 3       // second derivatives of (a|b) (36 components; eri2)
     };
     void launchD1(const D1Args &a, hipStream_t s);
+
+    // ri.hip: density-fitted J (and K when K != nullptr) from the dense eri3 / eri2 of the
+    // context (called without the context's lock: it calls lbx_eri3 / lbx_eri2); gamma_out, if
+    // not null, receives g_P = sum_mn (mn|P) D_mn
+    void riJK(lbx_context *ctx, hipStream_t stream, long long nbf, long long naux, const double *D, double *J,
+              double *K, double *gamma_out);
 }

@@ -24,7 +24,8 @@ SYMBOLS = ["lbx_create", "lbx_destroy", "lbx_last_error", "lbx_dims", "lbx_set_s
            "lbx_eri3_batch", "lbx_eri2_batch", "lbx_eri4d1_batch", "lbx_eri3d1_batch",
            "lbx_eri2d1_batch", "lbx_eri4_batch_static", "lbx_eri4soc_batch", "lbx_eri3soc_batch", "lbx_eri2d2_batch", "lbx_jk", "lbx_create_batch", "lbx_ri_sums", "lbx_jk_grad",
            "lbx_tune", "lbx_set_variant", "lbx_profile_enable",
-           "lbx_profile_read", "lbx_set_class_filter"]
+           "lbx_profile_read", "lbx_set_class_filter", "lbx_set_bound_window", "lbx_ri_jk",
+           "lbx_one_electron"]
 VARIANTS = ("valu-bra", "valu-ket", "mfma", "valu-bra-f32", "valu-ket-f32", "mfma-f32", "block",
             "block-f32")
 
@@ -94,6 +95,9 @@ def lib():
         L.lbx_tune.argtypes = [p, p, d]
         L.lbx_set_variant.argtypes = [p, i, i, i, i, i]
         L.lbx_set_class_filter.argtypes = [p, i, i, i, i]
+        L.lbx_set_bound_window.argtypes = [p, d, d]
+        L.lbx_ri_jk.argtypes = [p, p, p, p]
+        L.lbx_one_electron.argtypes = [p, p, p, i, p, p]
         L.lbx_profile_enable.argtypes = [p, i]
         L.lbx_profile_read.argtypes = [p, ctypes.POINTER(ClassProfile), i, ctypes.POINTER(i)]
         _LIB = L
@@ -201,9 +205,9 @@ class Context:
         torch = _torch()
         return torch.empty(shape, dtype=torch.float64, device=self.device)
 
-    def eri4(self):
+    def eri4(self, out=None):
         n = self.nbf
-        out = self._out((n, n, n, n))
+        out = self._out((n, n, n, n)) if out is None else out
         _check(lib().lbx_eri4(self.h, out.data_ptr()))
         return out
 
@@ -312,6 +316,39 @@ class Context:
                             mixed_thresh, ctypes.byref(st)))
         return J, K, {"quartets": st.quartets, "flops": st.flops, "quartets_fp32": st.quartets_fp32}
 
+    def ri_jk(self, D, exchange: bool = True):
+        """Density-fitted J and K (K None when exchange is False) over the context's
+        auxiliary basis (lbx_ri_jk)."""
+        torch = _torch()
+        D = D.to(device=self.device, dtype=torch.float64).contiguous()
+        n = self.nbf
+        if D.shape != (n, n):
+            raise LbxError(f"density must be ({n}, {n})")
+        J = self._out((n, n))
+        K = self._out((n, n)) if exchange else None
+        _check(lib().lbx_ri_jk(self.h, D.data_ptr(), J.data_ptr(), K.data_ptr() if K is not None else None))
+        return J, K
+
+    def one_electron(self, overlap: bool = True, kinetic: bool = True, charges=None):
+        """Overlap S, kinetic energy T and the attraction V to point charges ((k, 4) rows
+        x, y, z, q in bohr; None: no V) over the main basis (lbx_one_electron) -> (S, T, V),
+        None for what was not asked."""
+        n = self.nbf
+        S = self._out((n, n)) if overlap else None
+        T = self._out((n, n)) if kinetic else None
+        V, q = None, None
+        if charges is not None:
+            q = np.ascontiguousarray(charges, np.float64).reshape(-1, 4)
+            V = self._out((n, n))
+        if S is None and T is None and V is None:
+            raise LbxError("one_electron: nothing to compute")
+        _check(lib().lbx_one_electron(self.h, S.data_ptr() if S is not None else None,
+                                      T.data_ptr() if T is not None else None,
+                                      0 if q is None else q.shape[0],
+                                      q.ctypes.data if q is not None and q.size else None,
+                                      V.data_ptr() if V is not None else None))
+        return S, T, V
+
     def tune(self, D, schwarz: float = 1e-12):
         """Times every kernel variant of every class pair on this density and keeps the
         fastest (lbx_tune)."""
@@ -322,6 +359,10 @@ class Context:
     def set_class_filter(self, cls: tuple | None):
         """Restricts the following jk() calls to one class pair (la, lb, lc, ld); None: all."""
         _check(lib().lbx_set_class_filter(self.h, *(cls if cls is not None else (-1, -1, -1, -1))))
+
+    def set_bound_window(self, lo: float = 0.0, hi: float = float("inf")):
+        """Restricts the following jk() calls to quartets with lo <= Q_ab Q_cd < hi."""
+        _check(lib().lbx_set_bound_window(self.h, float(lo), float(hi)))
 
     def set_variant(self, cls: tuple, variant: int) -> bool:
         """Fix the kernel variant of a class pair; False if it has no such variant."""

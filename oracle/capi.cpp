@@ -600,6 +600,28 @@ extern "C"
                      });
     }
 
+    // which 0 overlap, 1 kinetic energy, 2 attraction to the n point charges (x, y, z, q)
+    int orc_oneel(void *h, int which, long n, const double *charges, double *out)
+    {
+        return guard([&]
+                     {
+                         const Structure &s = *static_cast<Structure *>(h);
+                         std::vector<double> r;
+                         if (which == 0)
+                             r = overlap(s);
+                         else if (which == 1)
+                             r = kineticEnergy(s);
+                         else
+                         {
+                             std::vector<std::array<double, 4>> c(n);
+                             for (long i = 0; i < n; i++)
+                                 c[i] = {charges[4 * i], charges[4 * i + 1], charges[4 * i + 2], charges[4 * i + 3]};
+                             r = externalCharges(s, c);
+                         }
+                         std::copy(r.begin(), r.end(), out);
+                     });
+    }
+
     // stats[0] = screened unique quartets processed, stats[1] = algorithmic FLOPs
     int orc_jk(void *h, const double *D, double *J, double *K, double thresh, int stride,
                double *stats)

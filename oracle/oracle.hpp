@@ -237,6 +237,13 @@ namespace orc
     // weighted by the number of its images in the dense tensor; no tensor is formed.
     void riSums(const Structure &s, double out[4]);
 
+    // ---- one-electron integrals (oneelints.cpp), dense nbf x nbf ----
+    std::vector<double> overlap(const Structure &s);
+    std::vector<double> kineticEnergy(const Structure &s);
+    // attraction of the electrons to point charges (x, y, z in bohr, charge): with the nuclei
+    // as charges, nuclearAttraction (oneelints.cpp:1703-1744)
+    std::vector<double> externalCharges(const Structure &s, const std::vector<std::array<double, 4>> &charges);
+
     // ---- J/K, energy and gradient (new functionality; no reference counterpart) ----
     struct JKStats
     {

@@ -45,6 +45,7 @@ def lib():
         L.orc_deploy_sum.argtypes = [_p, _i, _i, _p]
         L.orc_deploy_sum2.argtypes = [_p, _i, _p]
         L.orc_blocks.argtypes = [_p, _i, _l, _p, _p]
+        L.orc_oneel.argtypes = [_p, _i, _l, _p, _p]
         L.orc_jk.argtypes = [_p, _p, _p, _p, _d, _i, _p]
         L.orc_jk_grad.argtypes = [_p, _p, _p, _d]
         L.orc_boys.argtypes = [_i, _d, _p]
@@ -256,6 +257,24 @@ class Structure:
         sh = np.ascontiguousarray(shells, dtype=np.int32)
         out = np.zeros((sh.shape[0], ncomp, *dims))
         _check(lib().orc_blocks(self.h, code, sh.shape[0], _ptr(sh), _ptr(out)))
+        return out
+
+    def overlap(self) -> np.ndarray:
+        out = np.zeros((self.nbf, self.nbf))
+        _check(lib().orc_oneel(self.h, 0, 0, None, _ptr(out)))
+        return out
+
+    def kinetic(self) -> np.ndarray:
+        out = np.zeros((self.nbf, self.nbf))
+        _check(lib().orc_oneel(self.h, 1, 0, None, _ptr(out)))
+        return out
+
+    def external_charges(self, charges) -> np.ndarray:
+        """Attraction to point charges, rows (x, y, z bohr, charge); the nuclei give the
+        nuclear attraction."""
+        c = np.ascontiguousarray(charges, np.float64).reshape(-1, 4)
+        out = np.zeros((self.nbf, self.nbf))
+        _check(lib().orc_oneel(self.h, 2, c.shape[0], _ptr(c), _ptr(out)))
         return out
 
     def jk(self, D: np.ndarray, thresh: float = 1e-12, stride: int = 1):

@@ -4,7 +4,8 @@ The reference ships its basis sets as plain text (`element=Z`, `l=<l> n=<K>`, K 
 `exp coeff`, `end=Z`; parser `src/lible/ints/basis_sets.cpp:214-278`).  The known-answer tests
 (`test/tests_ints.cpp:694-1148`) use O and H from def2-svp, def2-tzvp, def2-qzvp-rifit and
 def2-universal-jkfit.  This script copies exactly those element blocks, unchanged, into
-`tests/golden/basis/<name>.txt` so that the tests run without the reference tree.
+`tests/golden/basis/<name>.txt` so that the tests run without the reference tree (plus
+C and H of cc-pVDZ and C and O of 6-31+G for the one-electron known answers).
 
 Run once in the build container:  python tests/golden/make_basis_fixtures.py
 """
@@ -18,6 +19,10 @@ WANTED = {
     ("lible_basis/ahlrichs/def2-tzvp.txt", "def2-tzvp"): [1, 8],
     ("lible_aux_basis/ahlrichs/def2-qzvp-rifit.txt", "def2-qzvp-rifit"): [1, 8],
     ("lible_aux_basis/ahlrichs/def2-universal-jkfit.txt", "def2-universal-jkfit"): [1, 8],
+    # the one-electron known answers: overlap / kinetic on C2H4 in cc-pVDZ, nuclear attraction
+    # on CO2 in 6-31+G (tests_ints.cpp:167-318)
+    ("lible_basis/dunning/cc-pvdz.txt", "cc-pvdz"): [1, 6],
+    ("lible_basis/pople/6-31+g.txt", "6-31+g"): [6, 8],
 }
 
 

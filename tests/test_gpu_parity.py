@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from lible_amd import basis as B
-from tests.conftest import H2O, O3, basis_path
+from tests.conftest import C2H4, CO2, H2O, O3, basis_path
 
 pytestmark = pytest.mark.gpu
 
@@ -461,6 +461,73 @@ def test_jk_grad_finite_difference(lbx):
             cm[ia, x] -= h
             fd[ia, x] = (energy(cp) - energy(cm)) / (2 * h)
     assert np.abs(g - fd).max() < 1e-7, np.abs(g - fd).max()
+
+
+def test_ri_jk(lbx, oracle):
+    """Density-fitted J and K (lbx_ri_jk) against the same formulas evaluated in numpy on the
+    oracle's dense eri3 / eri2: c = V^-1 g, J = B c; W_P = (L^-1 B)_P, K = sum_P W_P D W_P
+    (V = L L^T).  The metric's condition number (random auxiliary exponents) amplifies rounding
+    differences, hence 1e-9 of the largest element; and the Coulomb-fitting energy is a lower
+    bound of the exact one (the fit minimises the Coulomb norm of the density residual)."""
+    import torch
+    from scipy.linalg import cho_solve, solve_triangular
+    m = B.synthetic_molecule(61, 3, 4.0, B.RECIPE_MAIN, B.RECIPE_AUX)
+    s = oracle.Structure(m)
+    e3, e2 = s.eri3(), s.eri2()
+    n, na = e3.shape[0], e3.shape[2]
+    D = B.density_from_orbitals(8, n, 12, 0.1)
+    L = np.linalg.cholesky(e2)
+    g = np.einsum("mnp,mn->p", e3, D)
+    Jr = e3 @ cho_solve((L, True), g)
+    W = solve_triangular(L, e3.reshape(n * n, na).T, lower=True).reshape(na, n, n)
+    Kr = np.einsum("pmn,nl,pkl->mk", W, D, W)
+    ctx = lbx.Context(m)
+    Dt = torch.tensor(D, device="cuda")
+    J, K = ctx.ri_jk(Dt)
+    close(J.cpu().numpy(), Jr, rel=0.0, floor=1e-9)
+    close(K.cpu().numpy(), Kr, rel=0.0, floor=1e-9)
+    J2, K2 = ctx.ri_jk(Dt, exchange=False)
+    assert K2 is None
+    close(J2.cpu().numpy(), J.cpu().numpy(), rel=0.0, floor=1e-13)
+    Jx, _, _ = ctx.jk(Dt, schwarz=0.0)
+    e_ri, e_exact = 0.5 * float((Dt * J).sum()), 0.5 * float((Dt * Jx).sum())
+    assert e_ri <= e_exact + 1e-10 * abs(e_exact), (e_ri, e_exact)
+
+
+@pytest.mark.parametrize("case", ["c2h4_ccpvdz", "co2_631pg", "synthetic_f"])
+def test_one_electron(lbx, oracle, case):
+    """Overlap, kinetic energy and point-charge attraction (lbx_one_electron) element-wise
+    against the oracle, on the reference's known-answer molecules (checksums tests_ints.cpp:
+    167-183, 236-252, 303-319) and on a synthetic molecule with f shells and off-atom charges."""
+    if case == "c2h4_ccpvdz":
+        m, nrs = mol("cc-pvdz", C2H4), C2H4[0]
+    elif case == "co2_631pg":
+        m, nrs = mol("6-31+g", CO2), CO2[0]
+    else:
+        m = B.synthetic_molecule(91, 4, 5.0, B.RECIPE_MAIN)
+        nrs = [6, 7, 8, 1]
+    charges = np.column_stack([m.coords, np.asarray(nrs, np.float64)])
+    if case == "synthetic_f":
+        rng = np.random.default_rng(5)
+        extra = np.column_stack([rng.uniform(-3, 8, (5, 3)), rng.uniform(-1, 1, 5)])
+        charges = np.vstack([charges, extra])
+    s = oracle.Structure(m)
+    ctx = lbx.Context(m)
+    S, T, V = (x.cpu().numpy() for x in ctx.one_electron(charges=charges))
+    close(S, s.overlap())
+    close(T, s.kinetic())
+    close(V, s.external_charges(charges))
+    # normalised functions: unit diagonal of S
+    assert np.abs(np.diag(S) - 1.0).max() < 1e-12
+    if case == "c2h4_ccpvdz":
+        assert abs(np.abs(S).sum() - 226.099184393591) <= REL * 226.1
+        assert abs(np.abs(T).sum() - 252.755644635021) <= REL * 252.8
+    if case == "co2_631pg":
+        assert abs(np.abs(V).sum() - 2141.927982294058) <= REL * 2141.9
+    # one output alone, and no charges: V = 0
+    S2, T2, V2 = ctx.one_electron(overlap=True, kinetic=False, charges=np.zeros((0, 4)))
+    assert T2 is None and float(V2.abs().max()) == 0.0
+    close(S2.cpu().numpy(), S, rel=0.0, floor=1e-15)
 
 
 def test_jk_graph_replays(lbx, jk_case):

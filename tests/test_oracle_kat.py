@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 from lible_amd import basis as B
-from tests.conftest import H2O, O3, basis_path
+from tests.conftest import C2H4, CO2, H2O, O3, basis_path
 
 ABS = 1e-12
 REL = 1e-12
@@ -140,4 +140,20 @@ def test_ri_sums_match_dense(oracle):
 def test_deploy_d2_soc(oracle, which, aux, ref):
     s = oracle.Structure(mol("def2-svp", O3, aux))
     val = s.deploy_sum2(which)
+    assert abs(val - ref) <= REL * ref, (val, ref)
+
+
+@pytest.mark.parametrize("which,basis,geom,ref", [
+    ("overlap", "cc-pvdz", C2H4, 226.099184393591),        # tests_ints.cpp:167-183
+    ("kinetic", "cc-pvdz", C2H4, 252.755644635021),        # :236-252
+    ("nuclear", "6-31+g", CO2, 2141.927982294058),         # :303-319
+])
+def test_one_electron_checksums(oracle, which, basis, geom, ref):
+    m = mol(basis, geom)
+    s = oracle.Structure(m)
+    if which == "nuclear":
+        charges = np.column_stack([m.coords, np.asarray(geom[0], np.float64)])
+        val = np.abs(s.external_charges(charges)).sum()
+    else:
+        val = np.abs(getattr(s, which)()).sum()
     assert abs(val - ref) <= REL * ref, (val, ref)
