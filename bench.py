@@ -244,7 +244,78 @@ def leg_c5() -> dict:
             "max_abs_translation_sum": float(abs(gh.sum(axis=0)).max())}
 
 
-LEGS = {"c1": leg_c1, "c3": leg_c3, "c4": leg_c4, "c5": leg_c5}
+def leg_ri() -> dict:
+    """Density-fitted J/K (lbx_ri_jk) on the 40-atom workload with a 56-function-per-atom
+    auxiliary basis, against the exact FP64 build on the same molecule and density."""
+    import torch
+    from lible_amd import basis as B
+    from lible_amd import lbx
+    cfg = B.CONFIGS["c2_jk_fp64"]
+    # the same atoms and main shells as the c2 workload (same seed), plus auxiliary shells
+    mol = B.synthetic_molecule(cfg["seed"], cfg["natoms"], cfg["box"], B.RECIPE_MAIN, B.RECIPE_AUX)
+    ctx = lbx.Context(mol)
+    D = torch.tensor(B.density_from_orbitals(cfg["seed"] + 100, ctx.nbf, cfg["nocc"], cfg["sigma"]),
+                     device="cuda")
+    ctx.ri_jk(D)
+    torch.cuda.synchronize()
+    t, (Jr, Kr) = timed(lambda: ctx.ri_jk(D), 2, "ri J/K")
+    tj, (Jr2, _) = timed(lambda: ctx.ri_jk(D, exchange=False), 2, "ri J")
+    J, K, _ = ctx.jk(D, schwarz=SCHWARZ)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    J, K, _ = ctx.jk(D, schwarz=SCHWARZ, J=J, K=K)
+    torch.cuda.synchronize()
+    t_exact = time.perf_counter() - t0
+    e_j = 0.5 * float((D * (Jr - J)).sum())
+    e_k = -0.25 * float((D * (Kr - K)).sum())
+    n, na = ctx.nbf, ctx.nbf_aux
+    # dense work of the K part: T_P = W_P D and K = sum_P T_P W_P^T, 2 n^3 naux FLOPs each
+    k_flops = 4.0 * n ** 3 * na
+    ctx.close()
+    log("[bench] ri done")
+    return {"workload": "c2 molecule + 56-function auxiliary basis", "nbf": n, "naux": na,
+            "ri_jk_ms": t * 1e3, "ri_j_ms": tj * 1e3, "exact_jk_ms": t_exact * 1e3,
+            "k_dense_tflops_lower_bound": k_flops / t / 1e12,
+            "fit_error_J_energy": e_j, "fit_error_K_energy": e_k}
+
+
+WATER = ([8, 1, 1], [[0.0, 0.0, 0.0], [0.757, 0.586, 0.0], [-0.757, 0.586, 0.0]])   # angstrom
+
+
+def leg_scf() -> dict:
+    """Restricted Hartree-Fock (lible_amd.scf) of a 2x2x2 grid of water molecules 3.2 angstrom
+    apart in def2-SVP, energy and analytic gradient."""
+    import numpy as np
+    from lible_amd import basis as B
+    from lible_amd import scf
+    nrs, xyz = [], []
+    for i in range(2):
+        for j in range(2):
+            for k in range(2):
+                for z, p in zip(*WATER):
+                    nrs.append(z)
+                    xyz.append([p[0] + 3.2 * i, p[1] + 3.2 * j, p[2] + 3.2 * k])
+    path = os.path.join(ROOT, "tests", "golden", "basis", "def2-svp.txt")
+    mol = B.molecule_from_basis_files(path, nrs, xyz)
+    t0 = time.perf_counter()
+    rhf = scf.RHF(mol, nrs)
+    t_setup = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    r = rhf.kernel()
+    t_scf = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    g = rhf.gradient()
+    t_grad = time.perf_counter() - t0
+    rhf.close()
+    log("[bench] scf done")
+    return {"workload": "(H2O)8 def2-SVP RHF", "atoms": len(nrs), "nbf": int(mol.shells.nbf),
+            "energy": r.energy, "converged": r.converged, "iterations": r.iterations,
+            "setup_s": t_setup, "scf_s": t_scf, "ms_per_iteration": t_scf / r.iterations * 1e3,
+            "gradient_s": t_grad, "max_abs_gradient": float(np.abs(g).max()),
+            "max_abs_translation_sum": float(np.abs(g.sum(axis=0)).max())}
+
+
+LEGS = {"c1": leg_c1, "c3": leg_c3, "c4": leg_c4, "c5": leg_c5, "ri": leg_ri, "scf": leg_scf}
 
 
 def bench_ri_batch(a):

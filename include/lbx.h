@@ -160,6 +160,17 @@ int lbx_ri_jk(lbx_context *ctx, const double *D, double *J, double *K);
 int lbx_one_electron(lbx_context *ctx, double *S, double *T, int ncharges, const double *charges,
                      double *V);
 
+/* Nuclear gradient of the one-electron energy E1 = sum_mn D_mn (T + V)_mn - sum_mn W_mn S_mn
+ * (D the density, W the energy-weighted density; both device (nbf, nbf); W may be NULL), the
+ * contraction of the reference's overlapD1Kernel, kineticEnergyD1Kernel and
+ * externalChargesD1Kernel (orbital part, oneelints.cpp) blocks.  grad: device (natoms, 3),
+ * overwritten: derivatives with respect to the basis-function centres.  charge_grad: device
+ * (ncharges, 3) or NULL, overwritten: dE1/d(charge position), the operator part
+ * (externalChargesOperatorD1Kernel).  For the nuclei as charges the full one-electron gradient
+ * of atom A is grad[A] + charge_grad[A]. */
+int lbx_one_electron_grad(lbx_context *ctx, const double *D, const double *W, int ncharges,
+                          const double *charges, double *grad, double *charge_grad);
+
 /* Batched RI integrals over many independent molecules (no reference counterpart: the
  * reference runs eri3() / eri2() one Structure at a time, eri3.cpp:7-46, eri2.cpp:27-72).
  * lbx_create_batch builds one context over nmol molecules (main_bases[k], aux_bases[k] for

@@ -348,6 +348,18 @@ namespace lible::ints
     /// integrals, then the sum of |(mn|kl)| over them, printed to stdout (eri4.cpp:97-150).
     void eri4Benchmark(const Structure &structure);
 
+    /// Overlap integrals, nbf x nbf (oneelints.cpp:169-197).
+    vec2d overlap(const Structure &structure);
+    /// Kinetic-energy integrals, nbf x nbf (oneelints.cpp:348-376).
+    vec2d kineticEnergy(const Structure &structure);
+    /// Attraction of an electron to the nuclei (charges Z at the real atoms), nbf x nbf
+    /// (oneelints.cpp:1703-1744).
+    vec2d nuclearAttraction(const Structure &structure);
+    /// Attraction of an electron to point charges {x, y, z, q} (bohr), nbf x nbf
+    /// (oneelints.cpp:1087-1120).
+    vec2d externalCharges(const std::vector<std::array<double, 4>> &point_charges,
+                          const Structure &structure);
+
     /// (mn|kl), dense nbf^4 (eri4.cpp:38-95).
     vec4d eri4(const Structure &structure);
     /// (mn|mn), nbf x nbf (eri4.cpp:152-174).

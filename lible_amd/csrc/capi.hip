@@ -2176,6 +2176,53 @@ extern "C"
                        });
     }
 
+    int lbx_one_electron_grad(lbx_context *ctx, const double *D, const double *W, int ncharges,
+                              const double *charges, double *grad, double *charge_grad)
+    {
+        return guarded([&]
+                       {
+                           if (!ctx || !D || !grad)
+                               throw Error(LBX_ERR_INVALID, "lbx_one_electron_grad: null argument");
+                           if (ncharges < 0 || (ncharges > 0 && !charges))
+                               throw Error(LBX_ERR_INVALID, "lbx_one_electron_grad: bad point charges");
+                           std::lock_guard<std::mutex> lk(ctx->mu);
+                           Context &c = *ctx;
+                           requireSingle(c, "lbx_one_electron_grad");
+                           LBX_HIP(hipMemsetAsync(grad, 0, static_cast<size_t>(c.natoms) * 3 * sizeof(double), c.stream));
+                           if (charge_grad && ncharges > 0)
+                               LBX_HIP(hipMemsetAsync(charge_grad, 0, static_cast<size_t>(ncharges) * 3 * sizeof(double),
+                                                      c.stream));
+                           double4 *d_q = nullptr;
+                           if (ncharges > 0)
+                           {
+                               std::vector<double4> q(ncharges);
+                               for (int k = 0; k < ncharges; k++)
+                                   q[k] = double4{charges[4 * k], charges[4 * k + 1], charges[4 * k + 2],
+                                                  charges[4 * k + 3]};
+                               LBX_HIP(hipMalloc(&d_q, q.size() * sizeof(double4)));
+                               const hipError_t e = hipMemcpyAsync(d_q, q.data(), q.size() * sizeof(double4),
+                                                                   hipMemcpyHostToDevice, c.stream);
+                               if (e != hipSuccess)
+                               {
+                                   (void)hipFree(d_q);
+                                   LBX_HIP(e);
+                               }
+                           }
+                           try
+                           {
+                               computeOneElectronGrad(c, D, W, d_q, ncharges, grad, charge_grad);
+                           }
+                           catch (...)
+                           {
+                               (void)hipStreamSynchronize(c.stream);
+                               (void)hipFree(d_q);
+                               throw;
+                           }
+                           if (d_q)
+                               LBX_HIP(hipFree(d_q));
+                       });
+    }
+
     int lbx_set_bound_window(lbx_context *ctx, double lo, double hi)
     {
         return guarded([&]

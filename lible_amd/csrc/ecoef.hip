@@ -437,6 +437,209 @@ namespace lbx
                 unsafeAtomicAdd(&V[c * n + r], v);
         }
 
+        // ---- nuclear gradient of the one-electron energy tr(D (T + V)) - tr(W S) ----
+        // Overlap and kinetic parts (oneelints.cpp overlapD1Kernel / kineticEnergyD1Kernel): the
+        // derivative of a Cartesian Gaussian with respect to its centre, d/dA_x x^i e^{-a x^2} ->
+        // 2a (i+1 term) - i (i-1 term), applied per axis to the 1-D overlap and kinetic factors;
+        // d/dB = -d/dA (the integrals depend on A - B only), so pairs within one shell give zero.
+        // One thread per primitive pair, contracted with D and W over the pair's AO block.
+        template <int LA, int LB>
+        __global__ __launch_bounds__(256) void stGradKernel(int nppairs, const PrimIn *__restrict__ prim,
+                                                            const double *__restrict__ centers,
+                                                            const double *__restrict__ norms,
+                                                            const int *__restrict__ ao_a, const int *__restrict__ ao_b,
+                                                            const int *__restrict__ sh_a, const int *__restrict__ sh_b,
+                                                            const int *__restrict__ atom_a,
+                                                            const int *__restrict__ atom_b, long long n,
+                                                            const double *__restrict__ D,
+                                                            const double *__restrict__ W, double *grad)
+        {
+            constexpr int SA = nSph(LA), SB = nSph(LB), LA1 = LA + 1, LB2 = LB + 2;
+            const int pp = static_cast<int>(blockIdx.x) * blockDim.x + threadIdx.x;
+            if (pp >= nppairs)
+                return;
+            const PrimIn pr = prim[pp];
+            if (sh_a[pr.pair] == sh_b[pr.pair])
+                return;
+            const double a = pr.a, b = pr.b, p = a + b, o2p = 1.0 / (2.0 * p), mur = a * b / p;
+            const double *C = centers + 6 * pr.pair;
+            double S[3][LA1 + 1][LB2 + 1];
+            for (int x = 0; x < 3; x++)
+            {
+                const double P = (a * C[x] + b * C[3 + x]) / p, d = C[x] - C[3 + x];
+                overlap1d<LA1, LB2>(S[x], exp(-mur * d * d), P - C[x], P - C[3 + x], o2p);
+            }
+            auto t1 = [&](int x, int i, int j) {
+                double t = b * (2 * j + 1) * S[x][i][j] - 2.0 * b * b * S[x][i][j + 2];
+                if (j >= 2)
+                    t -= 0.5 * j * (j - 1) * S[x][i][j - 2];
+                return t;
+            };
+            auto ds = [&](int x, int i, int j) { return 2.0 * a * S[x][i + 1][j] - (i > 0 ? i * S[x][i - 1][j] : 0.0); };
+            auto dt = [&](int x, int i, int j) { return 2.0 * a * t1(x, i + 1, j) - (i > 0 ? i * t1(x, i - 1, j) : 0.0); };
+            const double *nrm = norms + static_cast<size_t>(pr.pair) * (SA + SB);
+            const double pre = pr.dadb * pow(M_PI / p, 1.5);
+            const long long r0 = ao_a[pr.pair], c0 = ao_b[pr.pair];
+            double g[3] = {0.0, 0.0, 0.0};
+            for (int mu = 0; mu < SA; mu++)
+                for (int nu = 0; nu < SB; nu++)
+                {
+                    const long long r = r0 + mu, c = c0 + nu;
+                    const double wd = D[r * n + c] + D[c * n + r];
+                    const double ww = W ? W[r * n + c] + W[c * n + r] : 0.0;
+                    double dS[3] = {0.0, 0.0, 0.0}, dT[3] = {0.0, 0.0, 0.0};
+                    for (int ea = cSph.start[LA][mu]; ea < cSph.start[LA][mu + 1]; ea++)
+                    {
+                        int i[3];
+                        cartOf(LA, cSph.cart[ea], i[0], i[1], i[2]);
+                        for (int eb_ = cSph.start[LB][nu]; eb_ < cSph.start[LB][nu + 1]; eb_++)
+                        {
+                            int j[3];
+                            cartOf(LB, cSph.cart[eb_], j[0], j[1], j[2]);
+                            const double v = cSph.val[ea] * cSph.val[eb_];
+                            double s1[3], k1[3];
+                            for (int x = 0; x < 3; x++)
+                            {
+                                s1[x] = S[x][i[x]][j[x]];
+                                k1[x] = t1(x, i[x], j[x]);
+                            }
+                            for (int x = 0; x < 3; x++)
+                            {
+                                const int y = (x + 1) % 3, z = (x + 2) % 3;
+                                const double dsx = ds(x, i[x], j[x]);
+                                dS[x] += v * dsx * s1[y] * s1[z];
+                                dT[x] += v * (dt(x, i[x], j[x]) * s1[y] * s1[z] +
+                                              dsx * (k1[y] * s1[z] + s1[y] * k1[z]));
+                            }
+                        }
+                    }
+                    const double sc = nrm[mu] * nrm[SA + nu] * pre;
+                    for (int x = 0; x < 3; x++)
+                        g[x] += sc * (wd * dT[x] - ww * dS[x]);
+                }
+            const int ia = atom_a[pr.pair], ib = atom_b[pr.pair];
+            for (int x = 0; x < 3; x++)
+            {
+                unsafeAtomicAdd(&grad[3 * ia + x], g[x]);
+                unsafeAtomicAdd(&grad[3 * ib + x], -g[x]);
+            }
+        }
+
+        // Point-charge attraction, step 1: the pair's E matrix and its A - B derivatives
+        // contracted with D over the AO block, ed[q][h][ppair] (q = 0: E, 1..3: dE/dX_x), so that
+        // step 2 touches one number per Hermite index and charge.
+        template <int LA, int LB>
+        __global__ __launch_bounds__(256) void vContractKernel(int nppairs, const PrimIn *__restrict__ prim,
+                                                               const double *__restrict__ eb,
+                                                               const double *__restrict__ e1b,
+                                                               const int *__restrict__ ao_a,
+                                                               const int *__restrict__ ao_b,
+                                                               const int *__restrict__ sh_a,
+                                                               const int *__restrict__ sh_b, long long n,
+                                                               const double *__restrict__ D, double *ed)
+        {
+            constexpr int SA = nSph(LA), SB = nSph(LB), SP = SA * SB, HP = nHerm(LA + LB);
+            const long long gid = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+            if (gid >= static_cast<long long>(nppairs) * HP)
+                return;
+            const int pp = static_cast<int>(gid / HP), h = static_cast<int>(gid % HP);
+            const int pair = prim[pp].pair;
+            const bool same = sh_a[pair] == sh_b[pair];
+            const long long r0 = ao_a[pair], c0 = ao_b[pair];
+            const double *e = eb + (static_cast<size_t>(pp) * HP + h) * SP;
+            const double *e1 = e1b + static_cast<size_t>(pp) * 3 * HP * SP + static_cast<size_t>(h) * SP;
+            double s[4] = {0.0, 0.0, 0.0, 0.0};
+            for (int mu = 0; mu < SA; mu++)
+                for (int nu = 0; nu < SB; nu++)
+                {
+                    const long long r = r0 + mu, c = c0 + nu;
+                    const double w = same ? D[r * n + c] : D[r * n + c] + D[c * n + r];
+                    const int mn = mu * SB + nu;
+                    s[0] = fma(w, e[mn], s[0]);
+                    for (int x = 0; x < 3; x++)
+                        s[1 + x] = fma(w, e1[static_cast<size_t>(x) * HP * SP + mn], s[1 + x]);
+                }
+            for (int q = 0; q < 4; q++)
+                ed[(static_cast<size_t>(q) * HP + h) * nppairs + pp] = s[q];
+        }
+
+        // Step 2, one thread per primitive pair, charges in a uniform loop.  With
+        // V = -(2 pi / p) q sum_h E_h R_h(p, P - C), X = A - B and P = (a A + b B) / p:
+        //   dV/dX_x = -(2 pi / p) q sum_h (dE_h/dX_x) R_h,   dV/dP_x = -(2 pi / p) q sum_h E_h R_{h+1_x}
+        //   d/dA = d/dX + (a/p) d/dP,   d/dB = -d/dX + (b/p) d/dP,   d/dC = -d/dP
+        // (the orbital part, externalChargesD1Kernel, and the operator part,
+        // externalChargesOperatorD1Kernel, oneelints.cpp).  The operator part of each charge is
+        // summed over the wave before one atomic add.
+        template <int LA, int LB>
+        __global__ __launch_bounds__(256) void vGradKernel(int nppairs, const PrimIn *__restrict__ prim,
+                                                           const double4 *__restrict__ ppd,
+                                                           const double *__restrict__ ed,
+                                                           const double4 *__restrict__ charges, int ncharges,
+                                                           const double *__restrict__ boys,
+                                                           const int *__restrict__ atom_a,
+                                                           const int *__restrict__ atom_b, double *grad,
+                                                           double *charge_grad)
+        {
+            constexpr int L = LA + LB, L1 = L + 1, HP = nHerm(L);
+            const int pp0 = static_cast<int>(blockIdx.x) * blockDim.x + threadIdx.x;
+            const bool valid = pp0 < nppairs;
+            const int pp = valid ? pp0 : 0;
+            const PrimIn pr = prim[pp];
+            const double4 P = ppd[pp];
+            const double ra = pr.a / P.x;
+            double gA[3] = {0.0, 0.0, 0.0}, gB[3] = {0.0, 0.0, 0.0};
+            for (int k = 0; k < ncharges; k++)
+            {
+                const double4 q = charges[k];
+                const double X = P.y - q.x, Y = P.z - q.y, Z = P.w - q.z;
+                double R[nHerm(L1) + L1];
+                boysF<L1>(P.x * (X * X + Y * Y + Z * Z), boys, R);
+                double sc = -2.0 * P.x;
+#pragma unroll
+                for (int m = 1; m <= L1; m++)
+                {
+                    R[m] *= sc;
+                    sc *= -2.0 * P.x;
+                }
+                hermiteR<L1, double>(R, X, Y, Z);
+                double vP[3] = {0.0, 0.0, 0.0}, vX[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+                for (int h = 0; h < HP; h++)
+                {
+                    const int tuv = hermTuv(h);
+                    const int t = tuv & 255, u = (tuv >> 8) & 255, v = tuv >> 16;
+                    const double e0 = ed[static_cast<size_t>(h) * nppairs + pp];
+                    const double rh = R[h == 0 ? 0 : L1 + h];
+                    vP[0] = fma(e0, R[L1 + hermIdx(t + 1, u, v)], vP[0]);
+                    vP[1] = fma(e0, R[L1 + hermIdx(t, u + 1, v)], vP[1]);
+                    vP[2] = fma(e0, R[L1 + hermIdx(t, u, v + 1)], vP[2]);
+                    for (int x = 0; x < 3; x++)
+                        vX[x] = fma(ed[(static_cast<size_t>(1 + x) * HP + h) * nppairs + pp], rh, vX[x]);
+                }
+                const double pre = valid ? -2.0 * M_PI / P.x * q.w : 0.0;
+                for (int x = 0; x < 3; x++)
+                {
+                    const double dP = pre * vP[x], dX = pre * vX[x];
+                    gA[x] += dX + ra * dP;
+                    gB[x] += -dX + (1.0 - ra) * dP;
+                    if (charge_grad)
+                    {
+                        const double cg = waveSum(-dP);
+                        if ((threadIdx.x & 63) == 0)
+                            unsafeAtomicAdd(&charge_grad[3 * k + x], cg);
+                    }
+                }
+            }
+            if (!valid)
+                return;
+            const int ia = atom_a[pr.pair], ib = atom_b[pr.pair];
+            for (int x = 0; x < 3; x++)
+            {
+                unsafeAtomicAdd(&grad[3 * ia + x], gA[x]);
+                unsafeAtomicAdd(&grad[3 * ib + x], gB[x]);
+            }
+        }
+
         using StFn = void (*)(const PairClass &, long long, double *, double *, hipStream_t);
         using VFn = void (*)(const PairClass &, const double4 *, int, const double *, long long, double *,
                              hipStream_t);
@@ -462,6 +665,42 @@ namespace lbx
             hipLaunchKernelGGL((vKernel<LA, LB>), dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256), 0, s,
                                pc.n_ppairs, pc.d_prim, pc.d_pp, pc.d_eb, charges, ncharges, boys, pc.d_ao_a,
                                pc.d_ao_b, pc.d_sh_a, pc.d_sh_b, n, V);
+        }
+
+        using GradFn = void (*)(const PairClass &, long long, const double *, const double *, const double4 *, int,
+                                const double *, double *, double *, double *, hipStream_t);
+
+        template <int LA, int LB>
+        void launchOneGrad(const PairClass &pc, long long n, const double *D, const double *W, const double4 *charges,
+                           int ncharges, const double *boys, double *ed, double *grad, double *charge_grad,
+                           hipStream_t s)
+        {
+            const int npp = pc.n_ppairs;
+            if (npp == 0)
+                return;
+            const unsigned g1 = static_cast<unsigned>((npp + 255) / 256);
+            hipLaunchKernelGGL((stGradKernel<LA, LB>), dim3(g1), dim3(256), 0, s, npp, pc.d_prim, pc.d_centers,
+                               pc.d_norms, pc.d_ao_a, pc.d_ao_b, pc.d_sh_a, pc.d_sh_b, pc.d_atom_a, pc.d_atom_b, n, D,
+                               W, grad);
+            if (ncharges == 0)
+                return;
+            const long long tot = static_cast<long long>(npp) * nHerm(LA + LB);
+            hipLaunchKernelGGL((vContractKernel<LA, LB>), dim3(static_cast<unsigned>((tot + 255) / 256)), dim3(256), 0,
+                               s, npp, pc.d_prim, pc.d_eb, pc.d_e1b, pc.d_ao_a, pc.d_ao_b, pc.d_sh_a, pc.d_sh_b, n, D,
+                               ed);
+            hipLaunchKernelGGL((vGradKernel<LA, LB>), dim3(g1), dim3(256), 0, s, npp, pc.d_prim, pc.d_pp, ed, charges,
+                               ncharges, boys, pc.d_atom_a, pc.d_atom_b, grad, charge_grad);
+        }
+
+        GradFn oneGradFn(int la, int lb)
+        {
+#define LBX_E(A, B)                                                                                \
+    if (la == A && lb == B)                                                                        \
+        return &launchOneGrad<A, B>;
+            LBX_E(0, 0) LBX_E(1, 0) LBX_E(1, 1) LBX_E(2, 0) LBX_E(2, 1) LBX_E(2, 2) LBX_E(3, 0)
+            LBX_E(3, 1) LBX_E(3, 2) LBX_E(3, 3)
+#undef LBX_E
+            throw Error(LBX_ERR_INVALID, "no one-electron gradient kernel for this angular momentum pair");
         }
 
         StFn stFn(int la, int lb)
@@ -618,5 +857,41 @@ namespace lbx
                 vFn(pc.la, pc.lb)(pc, charges, ncharges, c.d_boys, n, V, c.stream);
         }
         LBX_HIP(hipGetLastError());
+    }
+
+    // Nuclear gradient of tr(D (T + V)) - tr(W S) (W may be null), natoms x 3, and the
+    // derivative with respect to each charge position (charge_grad, ncharges x 3, may be null);
+    // grad and charge_grad are added into (zeroed by the caller).
+    void computeOneElectronGrad(Context &c, const double *D, const double *W, const double4 *charges, int ncharges,
+                                double *grad, double *charge_grad)
+    {
+        size_t ed_doubles = 0;
+        for (PairClass &pc : c.cls)
+            if (pc.n_ppairs > 0)
+            {
+                computeEcoeffsD1(c, pc);
+                ed_doubles = std::max(ed_doubles, static_cast<size_t>(4) * nHerm(pc.la + pc.lb) * pc.n_ppairs);
+            }
+        double *ed = nullptr;
+        if (ncharges > 0 && ed_doubles > 0)
+            LBX_HIP(hipMalloc(&ed, ed_doubles * sizeof(double)));
+        try
+        {
+            const long long n = static_cast<long long>(c.nbf);
+            for (const PairClass &pc : c.cls)
+                if (pc.n_ppairs > 0)
+                    oneGradFn(pc.la, pc.lb)(pc, n, D, W, charges, ncharges, c.d_boys, ed, grad, charge_grad,
+                                            c.stream);
+            LBX_HIP(hipGetLastError());
+            LBX_HIP(hipStreamSynchronize(c.stream));
+        }
+        catch (...)
+        {
+            (void)hipStreamSynchronize(c.stream);
+            (void)hipFree(ed);
+            throw;
+        }
+        if (ed)
+            LBX_HIP(hipFree(ed));
     }
 }

@@ -502,6 +502,43 @@ namespace lible::ints
         return g;
     }
 
+    // ------------------------------------------------------------------ one-electron integrals
+    vec2d overlap(const Structure &s)
+    {
+        const size_t n = s.getDimAO();
+        return fetch(s, vec2d(n, n), [](lbx_context *c, double *o)
+                     { return lbx_one_electron(c, o, nullptr, 0, nullptr, nullptr); });
+    }
+
+    vec2d kineticEnergy(const Structure &s)
+    {
+        const size_t n = s.getDimAO();
+        return fetch(s, vec2d(n, n), [](lbx_context *c, double *o)
+                     { return lbx_one_electron(c, nullptr, o, 0, nullptr, nullptr); });
+    }
+
+    vec2d externalCharges(const std::vector<std::array<double, 4>> &point_charges, const Structure &s)
+    {
+        const size_t n = s.getDimAO();
+        std::vector<double> q;
+        for (const auto &c : point_charges)
+            q.insert(q.end(), c.begin(), c.end());
+        const int nq = static_cast<int>(point_charges.size());
+        return fetch(s, vec2d(n, n), [&](lbx_context *c, double *o)
+                     { return lbx_one_electron(c, nullptr, nullptr, nq, nq ? q.data() : nullptr, o); });
+    }
+
+    vec2d nuclearAttraction(const Structure &s)
+    {
+        std::vector<std::array<double, 4>> nuclei(s.getNAtoms());
+        for (size_t a = 0; a < s.getNAtoms(); a++)
+        {
+            const auto xyz = s.getCoordsAtom(a);
+            nuclei[a] = {xyz[0], xyz[1], xyz[2], static_cast<double>(s.getZ(a))};
+        }
+        return externalCharges(nuclei, s);
+    }
+
     // ------------------------------------------------------------------ Boys, catalogue, benchmark
     BoysGrid::BoysGrid(int n) : n_(n), max_n_(n + 6)
     {

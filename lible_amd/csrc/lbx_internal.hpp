@@ -176,6 +176,11 @@ namespace lbx
     // (x, y, z, q) each) over the main basis, added into zeroed nbf x nbf device buffers (any
     // may be null)
     void computeOneElectron(Context &c, double *S, double *T, const double4 *charges, int ncharges, double *V);
+    // nuclear gradient of tr(D (T + V)) - tr(W S) into grad (natoms x 3; W may be null) and the
+    // derivatives with respect to the charge positions into charge_grad (ncharges x 3, may be
+    // null); both added into
+    void computeOneElectronGrad(Context &c, const double *D, const double *W, const double4 *charges, int ncharges,
+                                double *grad, double *charge_grad);
 
     // schwarz / task tables (capi.cpp)
     void ensureSchwarz(Context &c);

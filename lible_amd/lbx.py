@@ -25,7 +25,7 @@ SYMBOLS = ["lbx_create", "lbx_destroy", "lbx_last_error", "lbx_dims", "lbx_set_s
            "lbx_eri2d1_batch", "lbx_eri4_batch_static", "lbx_eri4soc_batch", "lbx_eri3soc_batch", "lbx_eri2d2_batch", "lbx_jk", "lbx_create_batch", "lbx_ri_sums", "lbx_jk_grad",
            "lbx_tune", "lbx_set_variant", "lbx_profile_enable",
            "lbx_profile_read", "lbx_set_class_filter", "lbx_set_bound_window", "lbx_ri_jk",
-           "lbx_one_electron"]
+           "lbx_one_electron", "lbx_one_electron_grad"]
 VARIANTS = ("valu-bra", "valu-ket", "mfma", "valu-bra-f32", "valu-ket-f32", "mfma-f32", "block",
             "block-f32")
 
@@ -98,6 +98,7 @@ def lib():
         L.lbx_set_bound_window.argtypes = [p, d, d]
         L.lbx_ri_jk.argtypes = [p, p, p, p]
         L.lbx_one_electron.argtypes = [p, p, p, i, p, p]
+        L.lbx_one_electron_grad.argtypes = [p, p, p, i, p, p, p]
         L.lbx_profile_enable.argtypes = [p, i]
         L.lbx_profile_read.argtypes = [p, ctypes.POINTER(ClassProfile), i, ctypes.POINTER(i)]
         _LIB = L
@@ -348,6 +349,30 @@ class Context:
                                       q.ctypes.data if q is not None and q.size else None,
                                       V.data_ptr() if V is not None else None))
         return S, T, V
+
+    def one_electron_grad(self, D, W=None, charges=None):
+        """Nuclear gradient of tr(D (T + V)) - tr(W S) (lbx_one_electron_grad) -> (grad
+        (natoms, 3) over the basis-function centres, charge_grad (k, 3) over the charge
+        positions, None without charges)."""
+        torch = _torch()
+        n = self.nbf
+        D = D.to(device=self.device, dtype=torch.float64).contiguous()
+        if D.shape != (n, n):
+            raise LbxError(f"density must be ({n}, {n})")
+        if W is not None:
+            W = W.to(device=self.device, dtype=torch.float64).contiguous()
+            if W.shape != (n, n):
+                raise LbxError(f"energy-weighted density must be ({n}, {n})")
+        q = None
+        if charges is not None:
+            q = np.ascontiguousarray(charges, np.float64).reshape(-1, 4)
+        nq = 0 if q is None else q.shape[0]
+        g = self._out((self.natoms, 3))
+        cg = self._out((nq, 3)) if nq else None
+        _check(lib().lbx_one_electron_grad(self.h, D.data_ptr(), W.data_ptr() if W is not None else None, nq,
+                                           q.ctypes.data if nq else None, g.data_ptr(),
+                                           cg.data_ptr() if cg is not None else None))
+        return g, cg
 
     def tune(self, D, schwarz: float = 1e-12):
         """Times every kernel variant of every class pair on this density and keeps the

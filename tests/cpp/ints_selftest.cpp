@@ -75,6 +75,14 @@ namespace
     const std::vector<std::array<double, 3>> xyz_o3{
         {0, 0, 0}, {0, 1, 1}, {0, 0, 2}};                               // tests_ints.cpp:115-120
 
+    const std::vector<int> nrs_c2h4{6, 6, 1, 1, 1, 1};
+    const std::vector<std::array<double, 3>> xyz_c2h4{
+        {3.402, 0.773, -9.252}, {4.697, 0.791, -8.909}, {2.933, -0.150, -9.521},
+        {2.837, 1.682, -9.258}, {5.262, -0.118, -8.904}, {5.167, 1.714, -8.641}};   // :51-58
+    const std::vector<int> nrs_co2{6, 8, 8};
+    const std::vector<std::array<double, 3>> xyz_co2{
+        {0.0, 0.0, 0.32942}, {0.0, 1.15362, -0.12353}, {0.0, -1.15362, -0.12353}};  // :18-23
+
     void cpuChecks()
     {
         guarded("vectormd_layout", [] {
@@ -213,6 +221,27 @@ namespace
                 }
             report("coulomb_exchange", err <= 1e-12 * scale,
                    "max err " + std::to_string(err / scale) + " of scale");
+        });
+        guarded("one_electron", [] {
+            // the reference's one-electron known answers (tests_ints.cpp:167-183, 236-252,
+            // 303-319) and nuclearAttraction = externalCharges at the nuclei
+            lints::Structure s("cc-pvdz", nrs_c2h4, xyz_c2h4);
+            checkSum("one_electron_overlap", absSum(lints::overlap(s)), 226.099184393591);
+            checkSum("one_electron_kinetic", absSum(lints::kineticEnergy(s)), 252.755644635021);
+            lints::Structure s2("6-31+g", nrs_co2, xyz_co2);
+            vec2d V = lints::nuclearAttraction(s2);
+            checkSum("one_electron_nuclear", absSum(V), 2141.927982294058);
+            std::vector<std::array<double, 4>> q;
+            for (size_t a = 0; a < s2.getNAtoms(); a++)
+            {
+                const auto c = s2.getCoordsAtom(a);
+                q.push_back({c[0], c[1], c[2], double(s2.getZ(a))});
+            }
+            vec2d V2 = lints::externalCharges(q, s2);
+            double d = 0;
+            for (size_t i = 0; i < V.size(); i++)
+                d = std::max(d, std::fabs(V.memptr()[i] - V2.memptr()[i]));
+            report("one_electron_external_charges", d == 0.0, "max diff " + std::to_string(d));
         });
         guarded("coulomb_exchange_gradient", [] {
             // the backward entry against central differences (h = 1e-4 bohr) of the energy of

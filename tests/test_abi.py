@@ -46,6 +46,7 @@ def test_null_arguments_rejected():
         assert getattr(L, name)(None, None) == -1
     assert L.lbx_ri_jk(None, None, None, None) == -1
     assert L.lbx_one_electron(None, None, None, 0, None, None) == -1
+    assert L.lbx_one_electron_grad(None, None, None, 0, None, None, None) == -1
     assert L.lbx_set_bound_window(None, 0.0, 1.0) == -1
     for name in ("lbx_eri4_batch", "lbx_eri3_batch", "lbx_eri2_batch", "lbx_eri4d1_batch",
                  "lbx_eri3d1_batch", "lbx_eri2d1_batch", "lbx_eri4soc_batch", "lbx_eri3soc_batch",

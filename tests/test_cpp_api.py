@@ -50,6 +50,8 @@ def test_cpp_interface_known_answers():
     lines = run("gpu", 300)
     names = {l.split()[1].rstrip(":") for l in lines}
     assert {"eri4", "eri4_symmetry", "coulomb_exchange", "coulomb_exchange_gradient",
+            "one_electron_overlap", "one_electron_kinetic", "one_electron_nuclear",
+            "one_electron_external_charges",
             "eri4_diagonal", "eri3", "eri2",
             "eri2_diagonal", "ghost_atoms", "deploy_eri4_kernel", "deploy_eri3_kernel",
             "deploy_eri2_kernel", "deploy_eri4d1_kernel", "deploy_eri3d1_kernel",
