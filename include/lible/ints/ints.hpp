@@ -47,6 +47,28 @@ namespace lible::ints
 
     using basis_atoms_t = std::vector<BasisAtom>;
 
+    /// A normalised shell of atomic orbitals of a structure (shell.hpp:36-67).
+    struct Shell
+    {
+        int l_{};
+        int z_{};                 // atomic number of the shell's atom (0 for a ghost atom)
+        size_t dim_cart_{};
+        size_t dim_sph_{};
+        size_t ofs_cart_{};
+        size_t ofs_sph_{};
+        size_t idx_{};            // position in the list of all shells (of its basis)
+        size_t idx_atom_{};
+        std::array<double, 3> xyz_coords_{};   // bohr
+        std::vector<double> exps_;
+        std::vector<double> coeffs_;            // raw contraction coefficients
+        std::vector<double> norms_;             // spherical AO norms
+        std::vector<double> norms_prim_;        // primitive norms
+        /// (not in the reference) the structure the shell belongs to, so that shell-pair data
+        /// built from shells can feed the kernel functors
+        std::shared_ptr<detail::Tensors> tensors_;
+        bool aux_ = false;                      // (not in the reference) auxiliary basis shell
+    };
+
     /// Bohr <-> angstrom (defs.hpp:9-12).
     constexpr double _bohr_to_ang_ = 0.5291772083;
     constexpr double _ang_to_bohr_ = 1.0 / _bohr_to_ang_;
@@ -117,6 +139,13 @@ namespace lible::ints
         int getZ(size_t iatom) const { return atomic_nrs_.at(iatom); }
         std::array<double, 3> getCoordsAtom(size_t iatom) const { return coords_.at(iatom); }
 
+        /// Normalised shells in AO order, all or of one angular momentum, of the main and the
+        /// auxiliary basis (structure.hpp:97-106).
+        std::vector<Shell> getShells() const;
+        std::vector<Shell> getShellsAux() const;
+        std::vector<Shell> getShellsL(int l) const;
+        std::vector<Shell> getShellsLAux(int l) const;
+
         /// The lbx context of this structure (created on the current device on first use).
         lbx_context *context() const;
         /// The link from shell data and kernel functors back to this structure (shared by copies
@@ -173,6 +202,13 @@ namespace lible::ints
     /// primitive pairs kept where |N_a d_a N_b d_b exp(-ab/(a+b) R_AB^2)| >= primitives_thrs_.
     struct ShellPairData
     {
+        ShellPairData() = default;
+        /// The pairs of shells_a (angular momentum la) with shells_b (lb) (shell_pair_data.hpp:
+        /// 50-57): with use_symm and la == lb only b <= a; the shells must come from one
+        /// structure's getShells / getShellsL.
+        ShellPairData(bool use_symm, int la, int lb, const std::vector<Shell> &shells_a,
+                      const std::vector<Shell> &shells_b, double primitives_thrs = 1e-15);
+
         bool uses_symm_{};
         double primitives_thrs_{};
         int la_{};

@@ -777,77 +777,119 @@ namespace lible::ints
         return *slot;
     }
 
-    std::vector<ShellPairData> shellPairData(bool use_symm, const Structure &s)
+    namespace
     {
-        const auto &sh = s.shells(false);
-        std::vector<std::pair<int, int>> lp;
-        for (int la = 0; la <= s.getMaxL(); la++)
-            for (int lb = 0; lb <= (use_symm ? la : s.getMaxL()); lb++)
-                lp.emplace_back(la, lb);
-        std::vector<ShellPairData> out;
-        for (auto [la, lb] : lp)
+        std::vector<Shell> shellsOf(const Structure &s, bool aux, int l)
         {
-            std::vector<size_t> A, B;
+            const auto &sh = s.shells(aux);
+            const size_t nreal = s.getNAtoms();
+            std::vector<Shell> out;
             for (size_t i = 0; i < sh.size(); i++)
             {
-                if (sh[i].l == la)
-                    A.push_back(i);
-                if (sh[i].l == lb)
-                    B.push_back(i);
+                const auto &h = sh[i];
+                if (l >= 0 && h.l != l)
+                    continue;
+                Shell x;
+                x.l_ = h.l;
+                x.z_ = h.atom < nreal ? s.getZ(h.atom) : 0;
+                x.dim_cart_ = static_cast<size_t>((h.l + 1) * (h.l + 2) / 2);
+                x.dim_sph_ = static_cast<size_t>(2 * h.l + 1);
+                x.ofs_cart_ = h.ofs_cart;
+                x.ofs_sph_ = h.ofs_sph;
+                x.idx_ = i;
+                x.idx_atom_ = h.atom;
+                x.xyz_coords_ = h.xyz;
+                x.exps_ = h.exps;
+                x.coeffs_ = h.coeffs;
+                x.norms_ = h.norms;
+                x.norms_prim_ = h.norms_prim;
+                x.tensors_ = s.tensors();
+                x.aux_ = aux;
+                out.push_back(std::move(x));
             }
-            ShellPairData d;
-            d.uses_symm_ = use_symm;
-            d.primitives_thrs_ = 1e-15;
-            d.la_ = la;
-            d.lb_ = lb;
-            d.tensors_ = s.tensors();
-            const size_t sa = 2 * la + 1, sb = 2 * lb + 1;
-            const size_t ne = sa * sb * static_cast<size_t>(lbx::nHerm(la + lb));
-            size_t nprim = 0, nnorm = 0;
-            for (size_t ia = 0; ia < A.size(); ia++)
-                for (size_t ib = 0; ib < ((use_symm && la == lb) ? ia + 1 : B.size()); ib++)
-                {
-                    const auto &a = sh[A[ia]], &b = sh[B[ib]];
-                    double r2 = 0;
-                    for (int k = 0; k < 3; k++)
-                        r2 += (a.xyz[k] - b.xyz[k]) * (a.xyz[k] - b.xyz[k]);
-                    d.n_pairs_total_++;
-                    size_t surv = 0;
-                    for (size_t x = 0; x < a.exps.size(); x++)
-                        for (size_t y = 0; y < b.exps.size(); y++)
-                        {
-                            d.n_ppairs_total_++;
-                            const double ea = a.exps[x], eb = b.exps[y];
-                            const double da = a.norms_prim[x] * a.coeffs[x], db = b.norms_prim[y] * b.coeffs[y];
-                            if (std::fabs(da * db * std::exp(-(ea * eb / (ea + eb)) * r2)) >= d.primitives_thrs_)
-                            {
-                                d.exps_.insert(d.exps_.end(), {ea, eb});
-                                d.coeffs_.insert(d.coeffs_.end(), {da, db});
-                                surv++;
-                            }
-                        }
-                    if (surv == 0)
-                        continue;
-                    d.coords_.insert(d.coords_.end(), a.xyz.begin(), a.xyz.end());
-                    d.coords_.insert(d.coords_.end(), b.xyz.begin(), b.xyz.end());
-                    d.norms_.insert(d.norms_.end(), a.norms.begin(), a.norms.end());
-                    d.norms_.insert(d.norms_.end(), b.norms.begin(), b.norms.end());
-                    d.nrs_ppairs_.push_back(surv);
-                    d.offsets_primitives_.push_back(nprim);
-                    d.offsets_ecoeffs_.push_back(nprim * ne);
-                    d.offsets_ecoeffs_deriv1_.push_back(3 * nprim * ne);
-                    d.offsets_norms_.push_back(nnorm);
-                    d.offsets_sph_.insert(d.offsets_sph_.end(), {a.ofs_sph, b.ofs_sph});
-                    d.offsets_cart_.insert(d.offsets_cart_.end(), {a.ofs_cart, b.ofs_cart});
-                    d.atomic_idxs_.insert(d.atomic_idxs_.end(), {a.atom, b.atom});
-                    d.shell_idxs_.insert(d.shell_idxs_.end(), {A[ia], B[ib]});
-                    nprim += surv;
-                    nnorm += sa + sb;
-                    d.n_pairs_++;
-                }
-            d.n_ppairs_ = nprim;
-            out.push_back(std::move(d));
+            return out;
         }
+    }
+
+    std::vector<Shell> Structure::getShells() const { return shellsOf(*this, false, -1); }
+    std::vector<Shell> Structure::getShellsAux() const { return shellsOf(*this, true, -1); }
+    std::vector<Shell> Structure::getShellsL(int l) const { return shellsOf(*this, false, l); }
+    std::vector<Shell> Structure::getShellsLAux(int l) const { return shellsOf(*this, true, l); }
+
+    // shell_pair_data.cpp:7-202: primitive pairs kept where
+    // |N_a d_a N_b d_b exp(-ab/(a+b) R_AB^2)| >= primitives_thrs; pairs without any are dropped
+    ShellPairData::ShellPairData(bool use_symm, int la, int lb, const std::vector<Shell> &A,
+                                 const std::vector<Shell> &B, double primitives_thrs)
+        : uses_symm_(use_symm), primitives_thrs_(primitives_thrs), la_(la), lb_(lb)
+    {
+        for (const auto *v : {&A, &B})
+            for (const Shell &x : *v)
+            {
+                if (x.l_ != (v == &A ? la : lb))
+                    throw std::runtime_error("ShellPairData: a shell's angular momentum differs from la / lb");
+                if (x.aux_)
+                    throw std::runtime_error("ShellPairData: auxiliary-basis shell");
+                if (!tensors_)
+                    tensors_ = x.tensors_;
+                else if (x.tensors_ != tensors_)
+                    throw std::runtime_error("ShellPairData: shells of different structures");
+            }
+        const size_t sa = 2 * la + 1, sb = 2 * lb + 1;
+        const size_t ne = sa * sb * static_cast<size_t>(lbx::nHerm(la + lb));
+        size_t nprim = 0, nnorm = 0;
+        for (size_t ia = 0; ia < A.size(); ia++)
+            for (size_t ib = 0; ib < ((use_symm && la == lb) ? std::min(ia + 1, B.size()) : B.size()); ib++)
+            {
+                const Shell &a = A[ia], &b = B[ib];
+                double r2 = 0;
+                for (int k = 0; k < 3; k++)
+                    r2 += (a.xyz_coords_[k] - b.xyz_coords_[k]) * (a.xyz_coords_[k] - b.xyz_coords_[k]);
+                n_pairs_total_++;
+                size_t surv = 0;
+                for (size_t x = 0; x < a.exps_.size(); x++)
+                    for (size_t y = 0; y < b.exps_.size(); y++)
+                    {
+                        n_ppairs_total_++;
+                        const double ea = a.exps_[x], eb = b.exps_[y];
+                        const double da = a.norms_prim_[x] * a.coeffs_[x], db = b.norms_prim_[y] * b.coeffs_[y];
+                        if (std::fabs(da * db * std::exp(-(ea * eb / (ea + eb)) * r2)) >= primitives_thrs)
+                        {
+                            exps_.insert(exps_.end(), {ea, eb});
+                            coeffs_.insert(coeffs_.end(), {da, db});
+                            surv++;
+                        }
+                    }
+                if (surv == 0)
+                    continue;
+                coords_.insert(coords_.end(), a.xyz_coords_.begin(), a.xyz_coords_.end());
+                coords_.insert(coords_.end(), b.xyz_coords_.begin(), b.xyz_coords_.end());
+                norms_.insert(norms_.end(), a.norms_.begin(), a.norms_.end());
+                norms_.insert(norms_.end(), b.norms_.begin(), b.norms_.end());
+                nrs_ppairs_.push_back(surv);
+                offsets_primitives_.push_back(nprim);
+                offsets_ecoeffs_.push_back(nprim * ne);
+                offsets_ecoeffs_deriv1_.push_back(3 * nprim * ne);
+                offsets_norms_.push_back(nnorm);
+                offsets_sph_.insert(offsets_sph_.end(), {a.ofs_sph_, b.ofs_sph_});
+                offsets_cart_.insert(offsets_cart_.end(), {a.ofs_cart_, b.ofs_cart_});
+                atomic_idxs_.insert(atomic_idxs_.end(), {a.idx_atom_, b.idx_atom_});
+                shell_idxs_.insert(shell_idxs_.end(), {a.idx_, b.idx_});
+                nprim += surv;
+                nnorm += sa + sb;
+                n_pairs_++;
+            }
+        n_ppairs_ = nprim;
+    }
+
+    std::vector<ShellPairData> shellPairData(bool use_symm, const Structure &s)
+    {
+        std::vector<ShellPairData> out;
+        for (int la = 0; la <= s.getMaxL(); la++)
+            for (int lb = 0; lb <= (use_symm ? la : s.getMaxL()); lb++)
+            {
+                out.emplace_back(use_symm, la, lb, s.getShellsL(la), s.getShellsL(lb));
+                out.back().tensors_ = s.tensors();   // also for a class without shells
+            }
         return out;
     }
 

@@ -42,7 +42,8 @@ def test_header_compiles_standalone(tmp_path):
 
 def test_cpp_interface_host_side():
     lines = run("cpu", 60)
-    assert len(lines) == 10
+    assert len(lines) == 11
+    assert any(l.startswith("PASS shell_pair_data_ctor") for l in lines)
 
 
 @pytest.mark.gpu

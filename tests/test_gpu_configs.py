@@ -84,14 +84,19 @@ def test_c2_jk_full_size(lbx):
     check_c2(J.cpu().numpy(), K.cpu().numpy(), st, Dh, fx)
 
 
+C3_SPLIT = 1e-6
+
+
 @pytest.mark.timeout(600)
 def test_c3_jk_mixed_full_size(lbx):
-    """120 atoms, 3600 functions: FP64 and mixed builds (FP32 below Q_ab Q_cd = 1e-5).  The
+    """120 atoms, 3600 functions: FP64 and mixed builds (FP32 below Q_ab Q_cd = 1e-6).  The
     FP32 tier's own error is isolated with a bound window: the quartets below the split built in
     FP64 and in FP32, whose partial J and K are small enough that FP64 rounding does not hide the
-    difference; it must move the energy by at most 1e-10 Eh.  The whole mixed build differs from
-    the FP64 build also by the FP64 rounding of ~1e10 atomic additions in another order (both
-    builds' energies are ~4e5 Eh, one ulp 6e-11), so its energy change is checked to 1e-9 Eh."""
+    difference; it must move the energy by at most 1e-10 Eh.  (At this size the split of 1e-5
+    puts 8.7e9 quartets in FP32 and their rounding adds up to ~3e-10 Eh, so the split is 1e-6
+    here.)  The whole mixed build differs from the FP64 build also by the FP64 rounding of ~1e10
+    atomic additions in another order (both builds' energies are ~4e5 Eh, one ulp 6e-11), so its
+    energy change is checked to 1e-9 Eh."""
     import torch
     cfg = B.CONFIGS["c3_jk_mixed"]
     m = B.config_molecule("c3_jk_mixed")
@@ -100,7 +105,7 @@ def test_c3_jk_mixed_full_size(lbx):
                      device="cuda")
     J, K, st = ctx.jk(D, schwarz=SCHWARZ)
     print(f"c3 FP64: {st['quartets']:.3e} quartets", flush=True)
-    Jm, Km, sm = ctx.jk(D, schwarz=SCHWARZ, precision=lbx.LBX_MIXED, mixed_thresh=1e-5)
+    Jm, Km, sm = ctx.jk(D, schwarz=SCHWARZ, precision=lbx.LBX_MIXED, mixed_thresh=C3_SPLIT)
     print(f"c3 mixed: {sm['quartets_fp32'] / sm['quartets']:.3f} FP32", flush=True)
     assert sm["quartets"] == st["quartets"] and sm["quartets_fp32"] > 0.5 * sm["quartets"]
     dE = 0.5 * float((D * (Jm - J)).sum()) - 0.25 * float((D * (Km - K)).sum())
@@ -111,9 +116,9 @@ def test_c3_jk_mixed_full_size(lbx):
     assert dJ <= 1e-9 and dK <= 1e-9
     assert float((J - J.T).abs().max()) == 0.0 and float((K - K.T).abs().max()) == 0.0
     del Jm, Km
-    ctx.set_bound_window(0.0, 1e-5)
+    ctx.set_bound_window(0.0, C3_SPLIT)
     Jw, Kw, sw = ctx.jk(D, schwarz=SCHWARZ)
-    Jw32, Kw32, sw32 = ctx.jk(D, schwarz=SCHWARZ, precision=lbx.LBX_MIXED, mixed_thresh=1e-5)
+    Jw32, Kw32, sw32 = ctx.jk(D, schwarz=SCHWARZ, precision=lbx.LBX_MIXED, mixed_thresh=C3_SPLIT)
     ctx.set_bound_window()
     assert sw["quartets"] == sw32["quartets_fp32"] == sm["quartets_fp32"]
     dE32 = 0.5 * float((D * (Jw32 - Jw)).sum()) - 0.25 * float((D * (Kw32 - Kw)).sum())

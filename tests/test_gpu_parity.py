@@ -642,6 +642,45 @@ def test_jk_medium_fp32(lbx, jk_medium_case, variant):
     close(K.cpu().numpy(), Kr, rel=0.0, floor=1e-5)
 
 
+def test_jk_mixed_vs_oracle_per_class(lbx, jk_medium_case):
+    """The mixed build at the 1e-5 split against the oracle's FP64 J and K (energy change from
+    the matrix differences <= 1e-10 Eh, elements <= 1e-9), and the FP32 tier's error class pair by
+    class pair: the quartets with Schwarz bound in [1e-12, 1e-5) of one class pair at a time
+    (class filter + bound window) in FP32 against the same quartets in FP64 (which the tests
+    above pin to the oracle per class pair); each class pair's energy change <= 1e-11 Eh and the
+    per-class changes add up to the whole tier's."""
+    import torch
+    m, D, Jr, Kr, sr = jk_medium_case
+    ctx = lbx.Context(m)
+    Dt = torch.tensor(D, device="cuda")
+    Jm, Km, sm = ctx.jk(Dt, schwarz=1e-12, precision=lbx.LBX_MIXED, mixed_thresh=1e-5)
+    assert sm["quartets"] == sr["quartets"]
+    assert sm["quartets_fp32"] > 0.2 * sm["quartets"]
+    Jm, Km = Jm.cpu().numpy(), Km.cpu().numpy()
+    dE = 0.5 * float((D * (Jm - Jr)).sum()) - 0.25 * float((D * (Km - Kr)).sum())
+    assert abs(dE) <= 1e-10, dE
+    assert np.abs(Jm - Jr).max() <= 1e-9 and np.abs(Km - Kr).max() <= 1e-9
+
+    def tier(cls):
+        ctx.set_class_filter(cls)
+        J64, K64, _ = ctx.jk(Dt, schwarz=1e-12)
+        J32, K32, s32 = ctx.jk(Dt, schwarz=1e-12, precision=lbx.LBX_MIXED, mixed_thresh=1e-5)
+        assert s32["quartets_fp32"] == s32["quartets"]
+        return (0.5 * float((Dt * (J32 - J64)).sum()) - 0.25 * float((Dt * (K32 - K64)).sum()),
+                s32["quartets"])
+
+    ctx.set_bound_window(0.0, 1e-5)
+    whole, nq = tier(None)
+    per = {c: tier(c) for c in CLASS_PAIRS}
+    ctx.set_class_filter(None)
+    ctx.set_bound_window()
+    assert sum(q for _, q in per.values()) == nq
+    worst = max(per.items(), key=lambda kv: abs(kv[1][0]))
+    print(f"FP32 tier: {nq} quartets, dE {whole:.3e}; worst class {worst[0]}: {worst[1][0]:.3e}")
+    assert abs(worst[1][0]) <= 1e-11
+    assert abs(sum(e for e, _ in per.values()) - whole) <= 1e-12
+
+
 @pytest.mark.parametrize("thresh", [0.0, 1e-12])
 def test_jk_small(lbx, oracle, thresh):
     import torch
