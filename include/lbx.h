@@ -146,9 +146,11 @@ int lbx_jk(lbx_context *ctx, const double *D, double *J, double *K, double schwa
  *   J_mn = sum_PQ (mn|P) (V^-1)_PQ sum_kl (Q|kl) D_kl
  *   K_mk = sum_P sum_nl W_P(mn) D_nl W_P(kl),   W_P = sum_Q (L^-1)_PQ (Q|..)
  * D, J, K: device (nbf, nbf) row-major, D symmetric; K may be NULL (J only).  Needs an
- * auxiliary basis.  Works on the dense 3-index tensor: about 8 nbf^2 naux bytes of device memory
- * for J, twice that with K, allocated for the call (LBX_ERR_NOMEM when it does not fit);
- * LBX_ERR_STATE when (P|Q) is not numerically positive definite. */
+ * auxiliary basis.  The first call builds the fitted 3-index tensor W (8 nbf^2 naux bytes of
+ * device memory, plus a work buffer of the same size while it is built or once K is asked for)
+ * and the context keeps both until lbx_destroy, so later calls (an SCF loop) cost only the
+ * products with D.  LBX_ERR_NOMEM when they do not fit; LBX_ERR_STATE when (P|Q) is not
+ * numerically positive definite. */
 int lbx_ri_jk(lbx_context *ctx, const double *D, double *J, double *K);
 
 /* One-electron integrals over the main basis, dense (nbf, nbf) row-major device matrices, each

@@ -2125,7 +2125,7 @@ extern "C"
                                naux = static_cast<long long>(ctx->nbf_aux);
                                s = ctx->stream;
                            }
-                           riJK(ctx, s, n, naux, D, J, K, nullptr);
+                           riJK(ctx, *ctx, s, n, naux, D, J, K);
                        });
     }
 

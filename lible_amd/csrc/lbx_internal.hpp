@@ -130,6 +130,12 @@ namespace lbx
         std::vector<int> atom_mol;
         int nmol = 1;
         std::vector<void *> allocations;
+        // density fitting (ri.hip): the fitted 3-index tensor W (nbf^2 naux doubles, P-major),
+        // built on the first lbx_ri_jk call and kept, and the exchange work buffer (same size,
+        // made on the first call with K); both in allocations
+        double *ri_w = nullptr, *ri_t = nullptr;
+        bool ri_ready = false;
+        std::mutex ri_mu;
 
         template <class T>
         T *alloc(size_t n)
@@ -212,9 +218,9 @@ namespace lbx
     };
     void launchD1(const D1Args &a, hipStream_t s);
 
-    // ri.hip: density-fitted J (and K when K != nullptr) from the dense eri3 / eri2 of the
-    // context (called without the context's lock: it calls lbx_eri3 / lbx_eri2); gamma_out, if
-    // not null, receives g_P = sum_mn (mn|P) D_mn
-    void riJK(lbx_context *ctx, hipStream_t stream, long long nbf, long long naux, const double *D, double *J,
-              double *K, double *gamma_out);
+    // ri.hip: density-fitted J (and K when K != nullptr) from the fitted 3-index tensor that the
+    // first call builds from the dense eri3 / eri2 of the context and keeps (called without the
+    // context's lock, since it calls lbx_eri3 / lbx_eri2; serialised by c.ri_mu)
+    void riJK(lbx_context *ctx, Context &c, hipStream_t stream, long long nbf, long long naux, const double *D,
+              double *J, double *K);
 }

@@ -1,16 +1,19 @@
 // Density-fitted (RI) Coulomb and exchange matrices from the dense 3-index and 2-index
 // integrals of the library (the natural consumer of eri3()/eri2(), eri3.cpp:7-46,
-// eri2.cpp:27-72; the reference has no J/K build).  With B_P(mn) = (mn|P), V = (P|Q) = L L^T:
-//   J = sum_P c_P B_P,                c = V^-1 g,   g_P = sum_mn B_P(mn) D_mn
-//   K = sum_P W_P D W_P,              W_P = sum_Q (L^-1)_PQ B_Q
-// The 3-index tensor is held P-major (one nbf x nbf block per auxiliary function) so that the
-// exchange is one batched GEMM W_P D and one GEMM over k = nbf * naux; the Cholesky
-// factorisation and solves are rocSOLVER's, the products rocBLAS's (FP64 matrix cores).
+// eri2.cpp:27-72; the reference has no J/K build).  With B_P(mn) = (mn|P), V = (P|Q) = L L^T
+// and the fitted tensor W_P = sum_Q (L^-1)_PQ B_Q (so that sum_P W_P W_P^T = B V^-1 B^T):
+//   J = sum_P W_P h_P,   h_P = sum_mn W_P(mn) D_mn
+//   K = sum_P W_P D W_P
+// W depends on the geometry only: the first call builds it (eri3, a transpose to P-major, eri2,
+// the Cholesky factorisation, one triangular solve) and the context keeps it, so every later
+// call (an SCF iteration) costs two matrix-vector products for J and, for K, one batched GEMM
+// W_P D and one GEMM over k = nbf * naux.  Factorisation by rocSOLVER, products by rocBLAS.
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
 #include "lbx_internal.hpp"
 
+#include <memory>
 #include <string>
 
 namespace lbx
@@ -79,69 +82,79 @@ namespace lbx
         };
     }
 
-    void riJK(lbx_context *ctx, hipStream_t stream, long long n, long long naux, const double *D, double *J,
-              double *K, double *gamma_out)
+    void riJK(lbx_context *ctx, Context &c, hipStream_t stream, long long n, long long naux, const double *D,
+              double *J, double *K)
     {
+        std::lock_guard<std::mutex> lk(c.ri_mu);
         const long long n2 = n * n;
-        // dense (mn|P), mn-major as the eri3 entry writes it, then P-major
-        Buf raw(sizeof(double) * n2 * naux);
-        {
-            const int rc = lbx_eri3(ctx, raw.d());
-            if (rc != LBX_OK)
-                throw Error(rc, std::string("lbx_ri_jk: ") + lbx_last_error());
-        }
-        Buf xp(sizeof(double) * n2 * naux);
-        {
-            const long long tiles = ((n2 + 31) / 32) * ((naux + 31) / 32);
-            const int grid = static_cast<int>(tiles < 65536 ? tiles : 65536);
-            hipLaunchKernelGGL(transposeKernel, dim3(grid), dim3(256), 0, stream, raw.d(), xp.d(), n2, naux);
-            LBX_HIP(hipGetLastError());
-        }
-        Buf V(sizeof(double) * naux * naux);
-        {
-            const int rc = lbx_eri2(ctx, V.d());
-            if (rc != LBX_OK)
-                throw Error(rc, std::string("lbx_ri_jk: ") + lbx_last_error());
-        }
         Handle h(stream);
-        // the 3-index products run through rocBLAS's 64-bit-index entries (nbf^2 naux exceeds
-        // 2^31 elements at a few hundred functions); the metric's factorisation needs 32-bit
-        const int NA = static_cast<int>(naux);
-        // V = L L^T (V symmetric: its row-major storage is also its column-major storage)
-        Buf info(sizeof(rocblas_int));
-        rb(rocsolver_dpotrf(h.h, rocblas_fill_lower, NA, V.d(), NA, static_cast<rocblas_int *>(info.p)),
-           "rocsolver_dpotrf");
-        rocblas_int hinfo = 0;
-        LBX_HIP(hipMemcpyAsync(&hinfo, info.p, sizeof(hinfo), hipMemcpyDeviceToHost, stream));
-        LBX_HIP(hipStreamSynchronize(stream));
-        if (hinfo != 0)
-            throw Error(LBX_ERR_STATE, "lbx_ri_jk: the auxiliary metric (P|Q) is not positive definite (pivot " +
-                                               std::to_string(hinfo) + ")");
+        if (K && !c.ri_t)
+            c.ri_t = c.alloc<double>(static_cast<size_t>(n2 * naux));
+        if (!c.ri_ready)
+        {
+            if (!c.ri_w)
+                c.ri_w = c.alloc<double>(static_cast<size_t>(n2 * naux));
+            // dense (mn|P), mn-major as the eri3 entry writes it, then P-major into W's buffer
+            std::unique_ptr<Buf> tmp;
+            double *raw = c.ri_t;
+            if (!raw)
+            {
+                tmp = std::make_unique<Buf>(sizeof(double) * n2 * naux);
+                raw = tmp->d();
+            }
+            {
+                const int rc = lbx_eri3(ctx, raw);
+                if (rc != LBX_OK)
+                    throw Error(rc, std::string("lbx_ri_jk: ") + lbx_last_error());
+            }
+            {
+                const long long tiles = ((n2 + 31) / 32) * ((naux + 31) / 32);
+                const int grid = static_cast<int>(tiles < 65536 ? tiles : 65536);
+                hipLaunchKernelGGL(transposeKernel, dim3(grid), dim3(256), 0, stream, raw, c.ri_w, n2, naux);
+                LBX_HIP(hipGetLastError());
+            }
+            Buf V(sizeof(double) * naux * naux);
+            {
+                const int rc = lbx_eri2(ctx, V.d());
+                if (rc != LBX_OK)
+                    throw Error(rc, std::string("lbx_ri_jk: ") + lbx_last_error());
+            }
+            // V = L L^T (V symmetric: its row-major storage is also its column-major storage);
+            // the factorisation takes 32-bit sizes, the 3-index products rocBLAS's 64-bit
+            // entries (nbf^2 naux exceeds 2^31 elements at a few hundred functions)
+            const int NA = static_cast<int>(naux);
+            Buf info(sizeof(rocblas_int));
+            rb(rocsolver_dpotrf(h.h, rocblas_fill_lower, NA, V.d(), NA, static_cast<rocblas_int *>(info.p)),
+               "rocsolver_dpotrf");
+            rocblas_int hinfo = 0;
+            LBX_HIP(hipMemcpyAsync(&hinfo, info.p, sizeof(hinfo), hipMemcpyDeviceToHost, stream));
+            LBX_HIP(hipStreamSynchronize(stream));
+            if (hinfo != 0)
+                throw Error(LBX_ERR_STATE, "lbx_ri_jk: the auxiliary metric (P|Q) is not positive definite (pivot " +
+                                                   std::to_string(hinfo) + ")");
+            // W = X L^-T with X column-major n2 x naux (column P = the P-major block B_P)
+            const double one = 1.0;
+            rb(rocblas_dtrsm_64(h.h, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose,
+                                rocblas_diagonal_non_unit, n2, naux, &one, V.d(), naux, c.ri_w, n2),
+               "rocblas_dtrsm (W = X L^-T)");
+            LBX_HIP(hipStreamSynchronize(stream));
+            c.ri_ready = true;
+        }
         const double one = 1.0, zero = 0.0;
-        // X (column-major N2 x NA, ld N2): column P = the P-major block B_P
         Buf g(sizeof(double) * naux);
-        rb(rocblas_dgemv_64(h.h, rocblas_operation_transpose, n2, naux, &one, xp.d(), n2, D, 1, &zero, g.d(), 1),
-           "rocblas_dgemv (g = X^T d)");
-        if (gamma_out)
-            LBX_HIP(hipMemcpyAsync(gamma_out, g.d(), sizeof(double) * naux, hipMemcpyDeviceToDevice, stream));
-        rb(rocsolver_dpotrs(h.h, rocblas_fill_lower, NA, 1, V.d(), NA, g.d(), NA), "rocsolver_dpotrs");
-        rb(rocblas_dgemv_64(h.h, rocblas_operation_none, n2, naux, &one, xp.d(), n2, g.d(), 1, &zero, J, 1),
-           "rocblas_dgemv (J = X c)");
+        rb(rocblas_dgemv_64(h.h, rocblas_operation_transpose, n2, naux, &one, c.ri_w, n2, D, 1, &zero, g.d(), 1),
+           "rocblas_dgemv (h = W^T d)");
+        rb(rocblas_dgemv_64(h.h, rocblas_operation_none, n2, naux, &one, c.ri_w, n2, g.d(), 1, &zero, J, 1),
+           "rocblas_dgemv (J = W h)");
         if (K)
         {
-            // W = X L^-T: column P of W is W_P = sum_Q (L^-1)_PQ B_Q
-            rb(rocblas_dtrsm_64(h.h, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose,
-                                rocblas_diagonal_non_unit, n2, naux, &one, V.d(), naux, xp.d(), n2),
-               "rocblas_dtrsm (W = X L^-T)");
-            // T_P = W_P D (W_P and D symmetric, so row- and column-major readings agree), into
-            // the buffer of the mn-major tensor
-            double *T = raw.d();
+            // T_P = W_P D (W_P and D symmetric, so row- and column-major readings agree)
             rb(rocblas_dgemm_strided_batched_64(h.h, rocblas_operation_none, rocblas_operation_none, n, n, n, &one,
-                                                xp.d(), n, n2, D, n, 0, &zero, T, n, n2, naux),
+                                                c.ri_w, n, n2, D, n, 0, &zero, c.ri_t, n, n2, naux),
                "rocblas_dgemm_strided_batched (T_P = W_P D)");
             // K = sum_P T_P W_P^T = [T_1 .. T_naux] [W_1 .. W_naux]^T, k = nbf * naux
-            rb(rocblas_dgemm_64(h.h, rocblas_operation_none, rocblas_operation_transpose, n, n, n * naux, &one, T,
-                                n, xp.d(), n, &zero, K, n),
+            rb(rocblas_dgemm_64(h.h, rocblas_operation_none, rocblas_operation_transpose, n, n, n * naux, &one,
+                                c.ri_t, n, c.ri_w, n, &zero, K, n),
                "rocblas_dgemm (K)");
         }
         LBX_HIP(hipStreamSynchronize(stream));

@@ -86,9 +86,15 @@ class RHF:
     def jk(self, D):
         if self.density_fit:
             return self.ctx.ri_jk(D)
-        J, K, _ = self.ctx.jk(D, schwarz=self.schwarz, precision=self.precision,
-                              mixed_thresh=self.mixed_thresh)
-        return J, K
+        # the same D, J and K buffers every iteration: lbx_jk captures its launch sequence into a
+        # HIP graph on the first call and replays it on the later ones
+        if not hasattr(self, "_bufs"):
+            self._bufs = tuple(self.torch.empty_like(D) for _ in range(3))
+        Db, Jb, Kb = self._bufs
+        Db.copy_(D)
+        self.ctx.jk(Db, schwarz=self.schwarz, precision=self.precision, mixed_thresh=self.mixed_thresh,
+                    J=Jb, K=Kb)
+        return Jb.clone(), Kb.clone()
 
     def _density(self, F):
         torch = self.torch
