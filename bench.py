@@ -166,18 +166,26 @@ def leg_c1() -> dict:
 
 
 C3_SPLIT = 1e-6   # at 120 atoms the FP32 rounding of the quartets below 1e-5 adds up to ~3e-10 Eh
+TUNED = {}        # kernel per class pair chosen by the autotuner on the main workload
 
 
 def leg_c3() -> dict:
-    """120 atoms, 3600 functions: the first FP64 call (Schwarz factors, task tables), the
-    autotuner, one FP64 and one mixed build (split C3_SPLIT, after its first call), the energy
-    change of the mixed build, and the FP32 tier's own error (bound window)."""
+    """120 atoms, 3600 functions, with the kernel choices the autotuner made on the 40-atom
+    workload (same shell recipe; tuning here would spend minutes building task tables of the
+    larger lane sides): the first FP64 call (Schwarz factors, task tables), one FP64 and one
+    mixed build (split C3_SPLIT, after its first call), the energy change of the mixed build, and
+    the FP32 tier's own error (bound window)."""
     import torch
     from lible_amd import basis as B
     from lible_amd import lbx
     cfg = B.CONFIGS["c3_jk_mixed"]
     mol = B.config_molecule("c3_jk_mixed")
     ctx = lbx.Context(mol)
+    applied = 0
+    for tier in ("fp64", "fp32"):
+        for cls, kernel in TUNED.get(tier, {}).items():
+            applied += bool(ctx.set_variant(cls, lbx.VARIANTS.index(kernel)))
+    log(f"[bench] c3: {applied} tuned kernel choices of the 40-atom workload applied")
     D = torch.tensor(B.density_from_orbitals(cfg["seed"] + 100, ctx.nbf, cfg["nocc"], cfg["sigma"]),
                      device="cuda")
     J, K = torch.empty_like(D), torch.empty_like(D)
@@ -186,11 +194,6 @@ def leg_c3() -> dict:
     torch.cuda.synchronize()
     first = time.perf_counter() - t0
     log(f"[bench] c3 first FP64 call {first:.1f} s")
-    t0 = time.perf_counter()
-    ctx.tune(D, schwarz=SCHWARZ)
-    torch.cuda.synchronize()
-    t_tune = time.perf_counter() - t0
-    log(f"[bench] c3 tuning {t_tune:.1f} s")
     t64, (_, _, st) = timed(lambda: ctx.jk(D, schwarz=SCHWARZ, J=J, K=K), 1, "c3 fp64")
     Jm, Km = torch.empty_like(D), torch.empty_like(D)
     ctx.jk(D, schwarz=SCHWARZ, precision=lbx.LBX_MIXED, mixed_thresh=C3_SPLIT, J=Jm, K=Km)
@@ -203,7 +206,8 @@ def leg_c3() -> dict:
     res = {"workload": "c3_jk_mixed", "nbf": ctx.nbf, "atoms": mol.natoms,
            "quartets": st["quartets"], "fp64_ms": t64 * 1e3, "mixed_ms": tm * 1e3, "split": C3_SPLIT,
            "speedup_vs_fp64": t64 / tm, "fp32_share": sm["quartets_fp32"] / sm["quartets"],
-           "fp64_tflops": st["flops"] / t64 / 1e12, "first_call_s": first, "tune_s": t_tune,
+           "fp64_tflops": st["flops"] / t64 / 1e12, "first_call_s": first,
+           "kernel_choices": f"{applied} class-pair kernels as tuned on the 40-atom workload",
            "energy_2e": E, "delta_E": dE,
            "max_abs_dJ": float((Jm - J).abs().max()), "max_abs_dK": float((Km - K).abs().max())}
     del Jm, Km, J, K
@@ -223,10 +227,17 @@ def leg_c4() -> dict:
     out = torch.empty((len(mols), 4), dtype=torch.float64, device="cuda")
     ctx.ri_sums(out)
     torch.cuda.synchronize()
-    t, (_, st) = timed(lambda: ctx.ri_sums(out), 2, "c4 ri")
+    t_untuned, _ = timed(lambda: ctx.ri_sums(out), 2, "c4 ri (default kernels)")
+    t0 = time.perf_counter()
+    ctx.tune()
+    t_tune = time.perf_counter() - t0
+    ctx.ri_sums(out)
+    torch.cuda.synchronize()
+    t, (_, st) = timed(lambda: ctx.ri_sums(out), 2, "c4 ri (tuned)")
     ctx.close()
     log("[bench] c4 done")
     return {"workload": "c4_ri_batch", "molecules": len(mols), "ms": t * 1e3,
+            "ms_default_kernels": t_untuned * 1e3, "tune_s": t_tune,
             "triplets_per_s": st["triplets"] / t, "doublets_per_s": st["doublets"] / t,
             "tflops": st["flops"] / t / 1e12}
 
@@ -488,10 +499,11 @@ def main(argv=None):
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--mixed-steps", type=int, default=1,
                     help="timed mixed-precision builds after the FP64 ones (0: skip)")
-    ap.add_argument("--legs", default="c1,c3,c4",
+    ap.add_argument("--legs", default="c1,c3,c4,ri,scf",
                     help="other workloads measured after the main one, one pass each: c1 (dense "
                          "eri4, 84 functions), c3 (120-atom FP64 and mixed J/K), c4 (256-molecule "
-                         "RI batch), c5 (60-atom gradient)")
+                         "RI batch), c5 (60-atom gradient), ri (density-fitted J/K), scf (RHF "
+                         "energy and gradient of (H2O)8)")
     ap.add_argument("--pmc-calibration", action="store_true",
                     help="end with a 1 GiB device-to-device copy, the reference reading for "
                          "rocprofv3's FETCH_SIZE / WRITE_SIZE counters")
@@ -552,6 +564,9 @@ def main(argv=None):
     elapsed = time.perf_counter() - t0
     prof = ctx.profile_read()
     ctx.profile(False)
+    # the tuned kernel per class pair, reused by the 120-atom leg (same shell recipe; tuning
+    # there would cost minutes of task-table building for its larger lane sides)
+    TUNED["fp64"] = {p["class"]: p["kernel"] for p in prof}
 
     step_s = elapsed / a.steps
     value = st["quartets"] / step_s
@@ -570,6 +585,8 @@ def main(argv=None):
         ctx.jk(D, schwarz=SCHWARZ, precision=lbx.LBX_MIXED, mixed_thresh=MIXED_SPLIT, J=Jm, K=Km)
         torch.cuda.synchronize()
         log("[bench] first mixed call (FP32 task tables) done")
+        ctx.profile(True)
+        torch.cuda.synchronize()
         t0 = time.perf_counter()
         for k in range(a.mixed_steps):
             _, _, sm = ctx.jk(D, schwarz=SCHWARZ, precision=lbx.LBX_MIXED,
@@ -577,6 +594,8 @@ def main(argv=None):
             log(f"[bench] mixed step {k} done")
         torch.cuda.synchronize()
         m_s = (time.perf_counter() - t0) / a.mixed_steps
+        TUNED["fp32"] = {p["class"]: p["kernel"] for p in ctx.profile_read() if p["kernel"].endswith("f32")}
+        ctx.profile(False)
         dE = energy_difference(D, J, K, Jm, Km)
         tier = fp32_tier_error(ctx, D, per_class=True)
         log("[bench] FP32-tier error per class pair done")

@@ -220,7 +220,10 @@ int lbx_jk_grad(lbx_context *ctx, const double *D, double *grad, double schwarz_
  * class pair on this density and threshold (on a sample of ~2M quartets of each class pair:
  * every k-th pair of one side with all its partners) and keeps the fastest per quartet: the
  * fastest FP64 form for FP64 work and the fastest of all forms for the FP32 tier of mixed
- * builds; without it lbx_jk picks by a work estimate from the primitive-pair counts. */
+ * builds; without it lbx_jk picks by a work estimate from the primitive-pair counts.  On a
+ * batched context (lbx_create_batch) it tunes lbx_ri_sums instead: every eri3 / eri2 class pair's
+ * FP64 sum kernel, one-lane or MFMA form, timed over its whole task table (D and the threshold
+ * are not used; D may be NULL). */
 int lbx_tune(lbx_context *ctx, const double *D, double schwarz_thresh);
 /* Fixes the variant of class pair (la lb|lc ld), la >= lb, lc >= ld, bra class >= ket class in
  * the order (0,0),(1,0),(1,1),(2,0),...: variants 0-2 and 6 for FP64 work, 3-5 and 7 for the

@@ -751,6 +751,7 @@ struct lbx_context : Context
         const TaskTable &t = it->second;
         KArgs a = baseArgs(*this, bra, ket, t, i == j);
         a.bra_major = braMajor ? 1 : 0;
+        a.per_q = s.kind == KIND_WAVE ? 1 : 0;
         a.D = D;
         a.J = J;
         a.K = K;
@@ -865,6 +866,91 @@ struct lbx_context : Context
     std::vector<std::pair<Launcher, KArgs>> ri_launches;
     std::map<double, std::vector<std::pair<Launcher, KArgs>>> ri_prepared;
     std::map<double, std::array<double, 5>> ri_counts;   // triplets, doublets, flops, FP32 ones
+    // FP64 variant of each RI class pair chosen by lbx_tune on a batched context, keyed by
+    // (kind, lane class, uniform class): kind 3 (main pair class | aux l), 2 (aux l | aux l')
+    std::map<std::tuple<int, int, int>, int> tuned_sum;
+
+    int sumVariant(int kind, int il, int iu, const PairClass &lane, const PairClass &unif) const
+    {
+        auto it = tuned_sum.find({kind, il, iu});
+        if (it != tuned_sum.end())
+            return it->second;
+        return kernelSlot(MODE_SUM, lane.la, lane.lb, unif.la, unif.lb, VAR_VALU_BRA).fn ? VAR_VALU_BRA : VAR_MFMA;
+    }
+
+    // Times the FP64 sum kernels of every RI class pair (the one-lane form and the MFMA form,
+    // each over the class pair's whole task table into a scratch output) and keeps the faster.
+    void tuneRiSums()
+    {
+        DevBuf scratch(4LL * nmol);
+        hipEvent_t e0, e1;
+        LBX_HIP(hipEventCreate(&e0));
+        LBX_HIP(hipEventCreate(&e1));
+        auto one = [&](int kind, int il, int iu, const PairClass &lane, const PairClass &unif, bool same)
+        {
+            if (lane.n_pairs == 0 || unif.n_pairs == 0)
+                return;
+            int best = -1;
+            float best_ms = 0.0f;
+            for (int v : {static_cast<int>(VAR_VALU_BRA), static_cast<int>(VAR_MFMA)})
+            {
+                const KernelSlot &k = kernelSlot(MODE_SUM, lane.la, lane.lb, unif.la, unif.lb, v);
+                if (!k.fn)
+                    continue;
+                TaskTable t = buildTasksMol(*this, lane, unif, same, k.width);
+                KArgs a = baseArgs(*this, lane, unif, t, same);
+                a.kind = kind;
+                a.out = scratch.p;
+                k.fn(a, stream);   // loads the code
+                LBX_HIP(hipEventRecord(e0, stream));
+                k.fn(a, stream);
+                LBX_HIP(hipEventRecord(e1, stream));
+                LBX_HIP(hipGetLastError());
+                LBX_HIP(hipEventSynchronize(e1));
+                float ms = 0.0f;
+                LBX_HIP(hipEventElapsedTime(&ms, e0, e1));
+                if (best < 0 || ms < best_ms)
+                {
+                    best = v;
+                    best_ms = ms;
+                }
+                // the table's device arrays were allocated from the context; free them now
+                for (void *p : {static_cast<void *>(t.d_prefix), static_cast<void *>(t.d_plo),
+                                static_cast<void *>(t.d_phi), static_cast<void *>(t.d_bprefix),
+                                static_cast<void *>(t.d_bq0)})
+                {
+                    auto it = std::find(allocations.begin(), allocations.end(), p);
+                    if (p && it != allocations.end())
+                    {
+                        (void)hipFree(p);
+                        allocations.erase(it);
+                    }
+                }
+            }
+            if (best >= 0)
+                tuned_sum[{kind, il, iu}] = best;
+        };
+        try
+        {
+            for (int i = 0; i < static_cast<int>(cls.size()); i++)
+                for (int l = 0; l < static_cast<int>(aux.size()); l++)
+                    one(3, i, l, cls[i], aux[l], false);
+            for (int la = 0; la < static_cast<int>(aux.size()); la++)
+                for (int lc = 0; lc <= la; lc++)
+                    one(2, la, lc, aux[la], aux[lc], la == lc);
+        }
+        catch (...)
+        {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            throw;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        dropRiGraph();
+        ri_prepared.clear();
+        ri_counts.clear();
+    }
 
     void prepareRiSums(double split)
     {
@@ -873,13 +959,11 @@ struct lbx_context : Context
         {
             std::vector<std::pair<Launcher, KArgs>> launches;
             std::array<double, 5> n{0, 0, 0, 0, 0};
-            auto add = [&](const PairClass &lane, const PairClass &unif, bool same, int kind)
+            auto add = [&](const PairClass &lane, const PairClass &unif, bool same, int kind, int il, int iu)
             {
                 if (lane.n_pairs == 0 || unif.n_pairs == 0)
                     return;
-                int var = VAR_VALU_BRA;
-                if (!kernelSlot(MODE_SUM, lane.la, lane.lb, unif.la, unif.lb, var).fn)
-                    var = VAR_MFMA;
+                const int var = sumVariant(kind, il, iu, lane, unif);
                 const KernelSlot &ks = slotFor(MODE_SUM, lane, unif, var);
                 const KernelSlot &k32 = kernelSlot(MODE_SUM, lane.la, lane.lb, unif.la, unif.lb, VAR_VALU_BRA32);
                 const bool tiers = split > 0.0 && k32.fn;
@@ -902,12 +986,12 @@ struct lbx_context : Context
                 else
                     one(ks, 0.0, kNoUpper, false);
             };
-            for (auto &pc : cls)
-                for (auto &ax : aux)
-                    add(pc, ax, false, 3);
+            for (int i = 0; i < static_cast<int>(cls.size()); i++)
+                for (int l = 0; l < static_cast<int>(aux.size()); l++)
+                    add(cls[i], aux[l], false, 3, i, l);
             for (int la = 0; la < static_cast<int>(aux.size()); la++)
                 for (int lc = 0; lc <= la; lc++)
-                    add(aux[la], aux[lc], la == lc, 2);
+                    add(aux[la], aux[lc], la == lc, 2, la, lc);
             it = ri_prepared.emplace(split, std::move(launches)).first;
             ri_counts[split] = n;
         }
@@ -1987,13 +2071,23 @@ extern "C"
     {
         return guarded([&]
                        {
-                           if (!ctx || !D)
+                           if (!ctx)
+                               throw Error(LBX_ERR_INVALID, "lbx_tune: null argument");
+                           if (!ctx->atom_mol.empty())
+                           {
+                               // a batched context: the RI sums' kernels (D is not used)
+                               std::lock_guard<std::mutex> lk(ctx->mu);
+                               if (ctx->shells_aux.empty())
+                                   throw Error(LBX_ERR_STATE, "lbx_tune: the batched context has no auxiliary basis");
+                               ctx->tuneRiSums();
+                               return;
+                           }
+                           if (!D)
                                throw Error(LBX_ERR_INVALID, "lbx_tune: null argument");
                            if (!(schwarz_thresh >= 0))
                                throw Error(LBX_ERR_INVALID, "lbx_tune: negative Schwarz threshold");
                            std::lock_guard<std::mutex> lk(ctx->mu);
                            Context &c = *ctx;
-                           requireSingle(c, "lbx_tune");
                            ensureSchwarz(c);
                            const long long n = static_cast<long long>(c.nbf);
                            DevBuf J(n * n), K(n * n);

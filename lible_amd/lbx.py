@@ -374,10 +374,13 @@ class Context:
                                            cg.data_ptr() if cg is not None else None))
         return g, cg
 
-    def tune(self, D, schwarz: float = 1e-12):
+    def tune(self, D=None, schwarz: float = 1e-12):
         """Times every kernel variant of every class pair on this density and keeps the
-        fastest (lbx_tune)."""
+        fastest (lbx_tune); on a batched context the RI sums' kernels (D not used)."""
         torch = _torch()
+        if D is None:
+            _check(lib().lbx_tune(self.h, None, schwarz))
+            return
         D = D.to(device=self.device, dtype=torch.float64).contiguous()
         _check(lib().lbx_tune(self.h, D.data_ptr(), schwarz))
 

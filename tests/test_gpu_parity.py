@@ -376,6 +376,11 @@ def test_ri_sums_batch(lbx, oracle):
     close(other.cpu().numpy(), first)
     with pytest.raises(lbx.LbxError):
         ctx.eri3()
+    # the tuner's choice of sum kernel per class pair (one-lane or MFMA form) gives the same sums
+    ctx.tune()
+    tuned, st2 = ctx.ri_sums()
+    assert st2["triplets"] == st["triplets"] and st2["doublets"] == st["doublets"]
+    close(tuned.cpu().numpy(), first, rel=1e-13, floor=0.0)
 
 
 def test_ri_sums_mixed(lbx, oracle):
