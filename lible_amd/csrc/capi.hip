@@ -751,7 +751,6 @@ struct lbx_context : Context
         const TaskTable &t = it->second;
         KArgs a = baseArgs(*this, bra, ket, t, i == j);
         a.bra_major = braMajor ? 1 : 0;
-        a.per_q = s.kind == KIND_WAVE ? 1 : 0;
         a.D = D;
         a.J = J;
         a.K = K;

@@ -50,7 +50,6 @@ namespace lbx
         long long ntasks;
         int same_class;            // bra class == ket class: quartet p == q is its own image
         int bra_major;             // quartetM: q indexes bra pairs, p ket pairs (classes differ)
-        int per_q;                 // quartetM: each workgroup takes whole uniform positions q
         // quartetV lane-block-major order: lane block b = lane positions 64b .. 64b+63; group g =
         // blocks g G .. g G + G - 1 (G = 2^gshift).  Group g's tasks pair its uniform positions
         // q = bq0[g], bq0[g] + 1, ... with its G blocks in turn (bprefix[g+1] - bprefix[g] = G x
@@ -750,315 +749,308 @@ namespace lbx
             (&sR[0][0][0])[e] = T(0);
         __syncthreads();
 
-        // Task ranges.  With per_q, workgroup b takes whole uniform positions q = b, b + grid, ...:
-        // every workgroup then walks the other side's pairs from the start of the same
-        // Schwarz-sorted list, so the workgroups resident on one XCD read those pairs' E matrices
-        // at nearly the same time and share them in L2.  Otherwise one contiguous task range.
-        const long long nchunk = A.per_q ? A.nket : 1;
-        for (long long chunk = A.per_q ? blockIdx.x : 0; chunk < nchunk; chunk += A.per_q ? gridDim.x : 1)
+        // contiguous task range per workgroup
+        const long long t0 = A.ntasks * blockIdx.x / gridDim.x;
+        const long long t1 = A.ntasks * (blockIdx.x + 1) / gridDim.x;
+        if (t0 >= t1)
+            return;
+        int q = uniform(findKet(A.prefix, A.nket, t0));
+        long long base = A.prefix[q], next = A.prefix[q + 1];
+        for (long long t = t0; t < t1; t++)
         {
-            const long long t0 = A.per_q ? A.prefix[chunk] : A.ntasks * blockIdx.x / gridDim.x;
-            const long long t1 = A.per_q ? A.prefix[chunk + 1] : A.ntasks * (blockIdx.x + 1) / gridDim.x;
-            if (t0 >= t1)
-                continue;
-            int q = uniform(A.per_q ? static_cast<int>(chunk) : findKet(A.prefix, A.nket, t0));
-            long long base = A.prefix[q], next = A.prefix[q + 1];
-            for (long long t = t0; t < t1; t++)
+            while (t >= next)
             {
-                while (t >= next)
+                q++;
+                base = next;
+                next = A.prefix[q + 1];
+            }
+            const int p = uniform(A.plo[q] + static_cast<int>(t - base));
+            // task tables run over the ket pairs of one bra pair when bra_major is set (the
+            // larger E matrix is then shared by consecutive tasks and stays in cache)
+            const int cd = uniform(A.ket.order[A.bra_major ? p : q]);
+            const int ab = uniform(A.bra.order[A.bra_major ? q : p]);
+            const int kpoff = uniform(A.ket.poff[cd]), kcnt = uniform(A.ket.pcnt[cd]);
+            const int bpoff = uniform(A.bra.poff[ab]), bcnt = uniform(A.bra.pcnt[ab]);
+            const int nrows = bcnt * HP;
+
+            Acc acc2[NQ][NP];
+#pragma unroll
+            for (int a = 0; a < NQ; a++)
+#pragma unroll
+                for (int b = 0; b < NP; b++)
+                    acc2[a][b] = Acc{0, 0, 0, 0};
+
+            for (int rowbase = 0; rowbase < nrows; rowbase += RC)
+            {
+                const int prim0 = rowbase / HP;
+                const int primEnd = min(bcnt, (rowbase + RC - 1) / HP + 1);
+                const int nprim = primEnd - prim0;
+
+                Acc acc1[MT][NQ];
+#pragma unroll
+                for (int a = 0; a < MT; a++)
+#pragma unroll
+                    for (int b = 0; b < NQ; b++)
+                        acc1[a][b] = Acc{0, 0, 0, 0};
+
+                for (int k = 0; k < kcnt; k++)
                 {
-                    q++;
-                    base = next;
-                    next = A.prefix[q + 1];
-                }
-                const int p = uniform(A.plo[q] + static_cast<int>(t - base));
-                // task tables run over the ket pairs of one bra pair when bra_major is set (the
-                // larger E matrix is then shared by consecutive tasks and stays in cache)
-                const int cd = uniform(A.ket.order[A.bra_major ? p : q]);
-                const int ab = uniform(A.bra.order[A.bra_major ? q : p]);
-                const int kpoff = uniform(A.ket.poff[cd]), kcnt = uniform(A.ket.pcnt[cd]);
-                const int bpoff = uniform(A.bra.poff[ab]), bcnt = uniform(A.bra.pcnt[ab]);
-                const int nrows = bcnt * HP;
-
-                Acc acc2[NQ][NP];
-    #pragma unroll
-                for (int a = 0; a < NQ; a++)
-    #pragma unroll
-                    for (int b = 0; b < NP; b++)
-                        acc2[a][b] = Acc{0, 0, 0, 0};
-
-                for (int rowbase = 0; rowbase < nrows; rowbase += RC)
-                {
-                    const int prim0 = rowbase / HP;
-                    const int primEnd = min(bcnt, (rowbase + RC - 1) / HP + 1);
-                    const int nprim = primEnd - prim0;
-
-                    Acc acc1[MT][NQ];
-    #pragma unroll
-                    for (int a = 0; a < MT; a++)
-    #pragma unroll
-                        for (int b = 0; b < NQ; b++)
-                            acc1[a][b] = Acc{0, 0, 0, 0};
-
-                    for (int k = 0; k < kcnt; k++)
+                    const double4 Q = A.ket.pp[kpoff + k];
+                    // Boys + scaled R^{(n)}_000 for the chunk's primitive quartets
+                    if (lane < nprim)
                     {
-                        const double4 Q = A.ket.pp[kpoff + k];
-                        // Boys + scaled R^{(n)}_000 for the chunk's primitive quartets
+                        const double4 P = A.bra.pp[bpoff + prim0 + lane];
+                        const double pq = P.x * Q.x, ps = P.x + Q.x;
+                        const double alpha = pq / ps;
+                        const double X_ = P.y - Q.y, Y_ = P.z - Q.z, Z_ = P.w - Q.w;
+                        const double Tb = alpha * (X_ * X_ + Y_ * Y_ + Z_ * Z_);
+                        double F[L + 1];
+                        boysF<L>(Tb, A.boys, F);
+                        double sc = kTwoPi52 / (pq * sqrt(ps));
+#pragma unroll
+                        for (int m = 0; m <= L; m++)
+                        {
+                            sInit[lane][m] = F[m] * sc;
+                            sc *= -2.0 * alpha;
+                        }
+                        sPQ[lane][0] = T(X_);
+                        sPQ[lane][1] = T(Y_);
+                        sPQ[lane][2] = T(Z_);
+                        sR[L & 1][lane][0] = T(sInit[lane][L]);
+                    }
+                    __syncthreads();
+                    // levels n = L-1 .. 0, ping-pong buffers
+#pragma unroll
+                    for (int n = L - 1; n >= 0; n--)
+                    {
+                        const int nh = nHerm(L - n) - 1;
+                        const T(*in)[HL] = sR[(n + 1) & 1];
+                        T(*out)[HL] = sR[n & 1];
+                        const int items = nprim * nh;
+                        for (int it = lane; it < items; it += 64)
+                        {
+                            const int pr = it / nh;
+                            const int h = it - pr * nh + 1;
+                            const unsigned r = kRecTab.v[h];
+                            const int s1 = r & 2047, s2 = (r >> 11) & 2047, ax = (r >> 22) & 3,
+                                      c = (r >> 24);
+                            out[pr][h] = fma(sPQ[pr][ax], in[pr][s1], T(c) * in[pr][s2]);
+                        }
                         if (lane < nprim)
-                        {
-                            const double4 P = A.bra.pp[bpoff + prim0 + lane];
-                            const double pq = P.x * Q.x, ps = P.x + Q.x;
-                            const double alpha = pq / ps;
-                            const double X_ = P.y - Q.y, Y_ = P.z - Q.z, Z_ = P.w - Q.w;
-                            const double Tb = alpha * (X_ * X_ + Y_ * Y_ + Z_ * Z_);
-                            double F[L + 1];
-                            boysF<L>(Tb, A.boys, F);
-                            double sc = kTwoPi52 / (pq * sqrt(ps));
-    #pragma unroll
-                            for (int m = 0; m <= L; m++)
-                            {
-                                sInit[lane][m] = F[m] * sc;
-                                sc *= -2.0 * alpha;
-                            }
-                            sPQ[lane][0] = T(X_);
-                            sPQ[lane][1] = T(Y_);
-                            sPQ[lane][2] = T(Z_);
-                            sR[L & 1][lane][0] = T(sInit[lane][L]);
-                        }
-                        __syncthreads();
-                        // levels n = L-1 .. 0, ping-pong buffers
-    #pragma unroll
-                        for (int n = L - 1; n >= 0; n--)
-                        {
-                            const int nh = nHerm(L - n) - 1;
-                            const T(*in)[HL] = sR[(n + 1) & 1];
-                            T(*out)[HL] = sR[n & 1];
-                            const int items = nprim * nh;
-                            for (int it = lane; it < items; it += 64)
-                            {
-                                const int pr = it / nh;
-                                const int h = it - pr * nh + 1;
-                                const unsigned r = kRecTab.v[h];
-                                const int s1 = r & 2047, s2 = (r >> 11) & 2047, ax = (r >> 22) & 3,
-                                          c = (r >> 24);
-                                out[pr][h] = fma(sPQ[pr][ax], in[pr][s1], T(c) * in[pr][s2]);
-                            }
-                            if (lane < nprim)
-                                out[lane][0] = T(sInit[lane][n]);
-                            __syncthreads();
-                        }
-                        // ket transform on MFMA: acc1[mt][nt] += Rg[rows, k] Ek[k, cols]
-                        const T *ek;
-                        if constexpr (F32)
-                            ek = A.ket.ek32 + static_cast<size_t>(kpoff + k) * HQ * SQ;
-                        else
-                            ek = A.ket.ek + static_cast<size_t>(kpoff + k) * HQ * SQ;
-    #pragma unroll
-                        for (int kt = 0; kt < KQ; kt++)
-                        {
-                            const int kk = 4 * kt + g;
-                            T b1[NQ];
-    #pragma unroll
-                            for (int nt = 0; nt < NQ; nt++)
-                            {
-                                const int col = 16 * nt + j;
-                                b1[nt] = (kk < HQ && col < SQ) ? ek[kk * SQ + col] : T(0);
-                            }
-    #pragma unroll
-                            for (int mt = 0; mt < MT; mt++)
-                            {
-                                if (rowbase + 16 * mt >= nrows)
-                                    continue;
-                                const int gr = rowbase + 16 * mt + j;
-                                const int pr = gr / HP;
-                                const int h = gr - pr * HP;
-                                const T a = (pr < bcnt) ? sR[0][pr - prim0][sTab[h][kk]] : T(0);
-    #pragma unroll
-                                for (int nt = 0; nt < NQ; nt++)
-                                    acc1[mt][nt] = mfma16(a, b1[nt], acc1[mt][nt]);
-                            }
-                        }
+                            out[lane][0] = T(sInit[lane][n]);
                         __syncthreads();
                     }
-                    // bra transform: I^T[kl, mn] += X^T[kl, rows] Eb[rows, mn]
-    #pragma unroll
-                    for (int mt = 0; mt < MT; mt++)
+                    // ket transform on MFMA: acc1[mt][nt] += Rg[rows, k] Ek[k, cols]
+                    const T *ek;
+                    if constexpr (F32)
+                        ek = A.ket.ek32 + static_cast<size_t>(kpoff + k) * HQ * SQ;
+                    else
+                        ek = A.ket.ek + static_cast<size_t>(kpoff + k) * HQ * SQ;
+#pragma unroll
+                    for (int kt = 0; kt < KQ; kt++)
                     {
-                        if (rowbase + 16 * mt >= nrows)
-                            continue;
-    #pragma unroll
-                        for (int c = 0; c < 4; c++)
+                        const int kk = 4 * kt + g;
+                        T b1[NQ];
+#pragma unroll
+                        for (int nt = 0; nt < NQ; nt++)
                         {
-                            // K-chunk c of X^T = accumulator element c of every lane: rows
-                            // 16 mt + 4 c + (0..3) in FP64 (element c holds row 4 c + g), rows
-                            // 16 mt + c + 4 (0..3) in FP32 (element c holds row 4 g + c)
-                            const int gr = rowbase + 16 * mt + (F32 ? 4 * g + c : 4 * c + g);
+                            const int col = 16 * nt + j;
+                            b1[nt] = (kk < HQ && col < SQ) ? ek[kk * SQ + col] : T(0);
+                        }
+#pragma unroll
+                        for (int mt = 0; mt < MT; mt++)
+                        {
+                            if (rowbase + 16 * mt >= nrows)
+                                continue;
+                            const int gr = rowbase + 16 * mt + j;
                             const int pr = gr / HP;
                             const int h = gr - pr * HP;
-                            const bool rv = pr < bcnt;
-                            const T *eb;
-                            if constexpr (F32)
-                                eb = A.bra.eb32 + (static_cast<size_t>(bpoff + pr) * HP + h) * SP;
-                            else
-                                eb = A.bra.eb + (static_cast<size_t>(bpoff + pr) * HP + h) * SP;
-    #pragma unroll
-                            for (int np = 0; np < NP; np++)
-                            {
-                                const int col = 16 * np + j;
-                                const T b2 = (rv && col < SP) ? eb[col] : T(0);
-    #pragma unroll
-                                for (int nq = 0; nq < NQ; nq++)
-                                    acc2[nq][np] = mfma16(acc1[mt][nq][c], b2, acc2[nq][np]);
-                            }
+                            const T a = (pr < bcnt) ? sR[0][pr - prim0][sTab[h][kk]] : T(0);
+#pragma unroll
+                            for (int nt = 0; nt < NQ; nt++)
+                                acc1[mt][nt] = mfma16(a, b1[nt], acc1[mt][nt]);
+                        }
+                    }
+                    __syncthreads();
+                }
+                // bra transform: I^T[kl, mn] += X^T[kl, rows] Eb[rows, mn]
+#pragma unroll
+                for (int mt = 0; mt < MT; mt++)
+                {
+                    if (rowbase + 16 * mt >= nrows)
+                        continue;
+#pragma unroll
+                    for (int c = 0; c < 4; c++)
+                    {
+                        // K-chunk c of X^T = accumulator element c of every lane: rows
+                        // 16 mt + 4 c + (0..3) in FP64 (element c holds row 4 c + g), rows
+                        // 16 mt + c + 4 (0..3) in FP32 (element c holds row 4 g + c)
+                        const int gr = rowbase + 16 * mt + (F32 ? 4 * g + c : 4 * c + g);
+                        const int pr = gr / HP;
+                        const int h = gr - pr * HP;
+                        const bool rv = pr < bcnt;
+                        const T *eb;
+                        if constexpr (F32)
+                            eb = A.bra.eb32 + (static_cast<size_t>(bpoff + pr) * HP + h) * SP;
+                        else
+                            eb = A.bra.eb + (static_cast<size_t>(bpoff + pr) * HP + h) * SP;
+#pragma unroll
+                        for (int np = 0; np < NP; np++)
+                        {
+                            const int col = 16 * np + j;
+                            const T b2 = (rv && col < SP) ? eb[col] : T(0);
+#pragma unroll
+                            for (int nq = 0; nq < NQ; nq++)
+                                acc2[nq][np] = mfma16(acc1[mt][nq][c], b2, acc2[nq][np]);
                         }
                     }
                 }
-                // accumulator element r of lane (g, j) holds D[4 r + g][j] (FP64) or D[4 g + r][j]
-                // (FP32); here D = I^T[16 nq + row][16 np + j]
-    #pragma unroll
-                for (int nq = 0; nq < NQ; nq++)
-    #pragma unroll
-                    for (int np = 0; np < NP; np++)
-    #pragma unroll
-                        for (int r = 0; r < 4; r++)
-                        {
-                            const int kl = 16 * nq + (F32 ? 4 * g + r : 4 * r + g), mn = 16 * np + j;
-                            if (kl < SQ && mn < SP)
-                                sI[mn * SQ + kl] = double(acc2[nq][np][r]);
-                        }
-                __syncthreads();
+            }
+            // accumulator element r of lane (g, j) holds D[4 r + g][j] (FP64) or D[4 g + r][j]
+            // (FP32); here D = I^T[16 nq + row][16 np + j]
+#pragma unroll
+            for (int nq = 0; nq < NQ; nq++)
+#pragma unroll
+                for (int np = 0; np < NP; np++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++)
+                    {
+                        const int kl = 16 * nq + (F32 ? 4 * g + r : 4 * r + g), mn = 16 * np + j;
+                        if (kl < SQ && mn < SP)
+                            sI[mn * SQ + kl] = double(acc2[nq][np][r]);
+                    }
+            __syncthreads();
 
-                const int a0 = A.bra.ao_a[ab], b0 = A.bra.ao_b[ab];
-                const int c0 = A.ket.ao_a[cd], d0 = A.ket.ao_b[cd];
-                if constexpr (MODE == MODE_JK)
+            const int a0 = A.bra.ao_a[ab], b0 = A.bra.ao_b[ab];
+            const int c0 = A.ket.ao_a[cd], d0 = A.ket.ao_b[cd];
+            if constexpr (MODE == MODE_JK)
+            {
+                double f = 1.0;
+                if (A.bra.sh_a[ab] == A.bra.sh_b[ab])
+                    f *= 0.5;
+                if (A.ket.sh_a[cd] == A.ket.sh_b[cd])
+                    f *= 0.5;
+                if (A.same_class && p == q)
+                    f *= 0.5;
+                const long long n = A.nbf;
+                const double *D = A.D;
+                // D blocks: 0 ab, 1 cd, 2 bd, 3 ad, 4 bc, 5 ac
+                for (int e = lane; e < SA * SB; e += 64)
+                    sD[0][e] = D[(a0 + e / SB) * n + b0 + e % SB];
+                for (int e = lane; e < SC * SD; e += 64)
+                    sD[1][e] = D[(c0 + e / SD) * n + d0 + e % SD];
+                for (int e = lane; e < SB * SD; e += 64)
+                    sD[2][e] = D[(b0 + e / SD) * n + d0 + e % SD];
+                for (int e = lane; e < SA * SD; e += 64)
+                    sD[3][e] = D[(a0 + e / SD) * n + d0 + e % SD];
+                for (int e = lane; e < SB * SC; e += 64)
+                    sD[4][e] = D[(b0 + e / SC) * n + c0 + e % SC];
+                for (int e = lane; e < SA * SC; e += 64)
+                    sD[5][e] = D[(a0 + e / SC) * n + c0 + e % SC];
+                __syncthreads();
+                for (int mn = lane; mn < SP; mn += 64)
                 {
-                    double f = 1.0;
-                    if (A.bra.sh_a[ab] == A.bra.sh_b[ab])
-                        f *= 0.5;
-                    if (A.ket.sh_a[cd] == A.ket.sh_b[cd])
-                        f *= 0.5;
-                    if (A.same_class && p == q)
-                        f *= 0.5;
-                    const long long n = A.nbf;
-                    const double *D = A.D;
-                    // D blocks: 0 ab, 1 cd, 2 bd, 3 ad, 4 bc, 5 ac
-                    for (int e = lane; e < SA * SB; e += 64)
-                        sD[0][e] = D[(a0 + e / SB) * n + b0 + e % SB];
-                    for (int e = lane; e < SC * SD; e += 64)
-                        sD[1][e] = D[(c0 + e / SD) * n + d0 + e % SD];
-                    for (int e = lane; e < SB * SD; e += 64)
-                        sD[2][e] = D[(b0 + e / SD) * n + d0 + e % SD];
-                    for (int e = lane; e < SA * SD; e += 64)
-                        sD[3][e] = D[(a0 + e / SD) * n + d0 + e % SD];
-                    for (int e = lane; e < SB * SC; e += 64)
-                        sD[4][e] = D[(b0 + e / SC) * n + c0 + e % SC];
-                    for (int e = lane; e < SA * SC; e += 64)
-                        sD[5][e] = D[(a0 + e / SC) * n + c0 + e % SC];
-                    __syncthreads();
+                    double s = 0.0;
+                    for (int kl = 0; kl < SQ; kl++)
+                        s = fma(sI[mn * SQ + kl], sD[1][kl], s);
+                    unsafeAtomicAdd(&A.J[(a0 + mn / SB) * n + b0 + mn % SB], 2.0 * f * s);
+                }
+                for (int kl = lane; kl < SQ; kl += 64)
+                {
+                    double s = 0.0;
+                    for (int mn = 0; mn < SP; mn++)
+                        s = fma(sI[mn * SQ + kl], sD[0][mn], s);
+                    unsafeAtomicAdd(&A.J[(c0 + kl / SD) * n + d0 + kl % SD], 2.0 * f * s);
+                }
+                // K_ac[mu,ka] += f sum_{nu,la} I D_bd[nu,la]
+                for (int e = lane; e < SA * SC; e += 64)
+                {
+                    const int mu = e / SC, ka = e % SC;
+                    double s = 0.0;
+                    for (int nu = 0; nu < SB; nu++)
+                        for (int la = 0; la < SD; la++)
+                            s = fma(sI[(mu * SB + nu) * SQ + ka * SD + la], sD[2][nu * SD + la], s);
+                    unsafeAtomicAdd(&A.K[(c0 + ka) * n + a0 + mu], f * s);
+                }
+                // K_bc[nu,ka] += f sum_{mu,la} I D_ad[mu,la]
+                for (int e = lane; e < SB * SC; e += 64)
+                {
+                    const int nu = e / SC, ka = e % SC;
+                    double s = 0.0;
+                    for (int mu = 0; mu < SA; mu++)
+                        for (int la = 0; la < SD; la++)
+                            s = fma(sI[(mu * SB + nu) * SQ + ka * SD + la], sD[3][mu * SD + la], s);
+                    unsafeAtomicAdd(&A.K[(c0 + ka) * n + b0 + nu], f * s);
+                }
+                // K_ad[mu,la] += f sum_{nu,ka} I D_bc[nu,ka]
+                for (int e = lane; e < SA * SD; e += 64)
+                {
+                    const int mu = e / SD, la = e % SD;
+                    double s = 0.0;
+                    for (int nu = 0; nu < SB; nu++)
+                        for (int ka = 0; ka < SC; ka++)
+                            s = fma(sI[(mu * SB + nu) * SQ + ka * SD + la], sD[4][nu * SC + ka], s);
+                    unsafeAtomicAdd(&A.K[(d0 + la) * n + a0 + mu], f * s);
+                }
+                // K_bd[nu,la] += f sum_{mu,ka} I D_ac[mu,ka]
+                for (int e = lane; e < SB * SD; e += 64)
+                {
+                    const int nu = e / SD, la = e % SD;
+                    double s = 0.0;
+                    for (int mu = 0; mu < SA; mu++)
+                        for (int ka = 0; ka < SC; ka++)
+                            s = fma(sI[(mu * SB + nu) * SQ + ka * SD + la], sD[5][mu * SC + ka], s);
+                    unsafeAtomicAdd(&A.K[(d0 + la) * n + b0 + nu], f * s);
+                }
+            }
+            else if constexpr (MODE == MODE_DENSE)
+            {
+                for (int e = lane; e < SP * SQ; e += 64)
+                    denseStore<C>(A, sI[e], e / SQ, e % SQ, a0, b0, c0, d0);
+            }
+            else if constexpr (MODE == MODE_SUM)
+            {
+                double sa = 0.0, ss = 0.0;
+                for (int e = lane; e < SP * SQ; e += 64)
+                {
+                    sa += fabs(sI[e]);
+                    ss += sI[e];
+                }
+                const bool one = A.kind == 3 ? A.bra.sh_a[ab] == A.bra.sh_b[ab] : (A.same_class && p == q);
+                const double w = one ? 1.0 : 2.0;
+                sa = waveSum(w * sa);
+                ss = waveSum(w * ss);
+                if (lane == 0)
+                {
+                    double *o = A.out + 4 * static_cast<long long>(A.ket.mol[cd]) + (A.kind == 3 ? 0 : 2);
+                    unsafeAtomicAdd(o, sa);
+                    unsafeAtomicAdd(o + 1, ss);
+                }
+            }
+            else
+            {
+                if (lane == 0)
+                {
+                    double m = 0.0;
+                    for (int mn = 0; mn < SP; mn++)
+                        m = fmax(m, fabs(sI[mn * SQ + mn]));
+                    A.qout[ab] = sqrt(m);
+                }
+                if (A.dout)
                     for (int mn = lane; mn < SP; mn += 64)
                     {
-                        double s = 0.0;
-                        for (int kl = 0; kl < SQ; kl++)
-                            s = fma(sI[mn * SQ + kl], sD[1][kl], s);
-                        unsafeAtomicAdd(&A.J[(a0 + mn / SB) * n + b0 + mn % SB], 2.0 * f * s);
-                    }
-                    for (int kl = lane; kl < SQ; kl += 64)
-                    {
-                        double s = 0.0;
-                        for (int mn = 0; mn < SP; mn++)
-                            s = fma(sI[mn * SQ + kl], sD[0][mn], s);
-                        unsafeAtomicAdd(&A.J[(c0 + kl / SD) * n + d0 + kl % SD], 2.0 * f * s);
-                    }
-                    // K_ac[mu,ka] += f sum_{nu,la} I D_bd[nu,la]
-                    for (int e = lane; e < SA * SC; e += 64)
-                    {
-                        const int mu = e / SC, ka = e % SC;
-                        double s = 0.0;
-                        for (int nu = 0; nu < SB; nu++)
-                            for (int la = 0; la < SD; la++)
-                                s = fma(sI[(mu * SB + nu) * SQ + ka * SD + la], sD[2][nu * SD + la], s);
-                        unsafeAtomicAdd(&A.K[(c0 + ka) * n + a0 + mu], f * s);
-                    }
-                    // K_bc[nu,ka] += f sum_{mu,la} I D_ad[mu,la]
-                    for (int e = lane; e < SB * SC; e += 64)
-                    {
-                        const int nu = e / SC, ka = e % SC;
-                        double s = 0.0;
-                        for (int mu = 0; mu < SA; mu++)
-                            for (int la = 0; la < SD; la++)
-                                s = fma(sI[(mu * SB + nu) * SQ + ka * SD + la], sD[3][mu * SD + la], s);
-                        unsafeAtomicAdd(&A.K[(c0 + ka) * n + b0 + nu], f * s);
-                    }
-                    // K_ad[mu,la] += f sum_{nu,ka} I D_bc[nu,ka]
-                    for (int e = lane; e < SA * SD; e += 64)
-                    {
-                        const int mu = e / SD, la = e % SD;
-                        double s = 0.0;
-                        for (int nu = 0; nu < SB; nu++)
-                            for (int ka = 0; ka < SC; ka++)
-                                s = fma(sI[(mu * SB + nu) * SQ + ka * SD + la], sD[4][nu * SC + ka], s);
-                        unsafeAtomicAdd(&A.K[(d0 + la) * n + a0 + mu], f * s);
-                    }
-                    // K_bd[nu,la] += f sum_{mu,ka} I D_ac[mu,ka]
-                    for (int e = lane; e < SB * SD; e += 64)
-                    {
-                        const int nu = e / SD, la = e % SD;
-                        double s = 0.0;
-                        for (int mu = 0; mu < SA; mu++)
-                            for (int ka = 0; ka < SC; ka++)
-                                s = fma(sI[(mu * SB + nu) * SQ + ka * SD + la], sD[5][mu * SC + ka], s);
-                        unsafeAtomicAdd(&A.K[(d0 + la) * n + b0 + nu], f * s);
-                    }
-                }
-                else if constexpr (MODE == MODE_DENSE)
-                {
-                    for (int e = lane; e < SP * SQ; e += 64)
-                        denseStore<C>(A, sI[e], e / SQ, e % SQ, a0, b0, c0, d0);
-                }
-                else if constexpr (MODE == MODE_SUM)
-                {
-                    double sa = 0.0, ss = 0.0;
-                    for (int e = lane; e < SP * SQ; e += 64)
-                    {
-                        sa += fabs(sI[e]);
-                        ss += sI[e];
-                    }
-                    const bool one = A.kind == 3 ? A.bra.sh_a[ab] == A.bra.sh_b[ab] : (A.same_class && p == q);
-                    const double w = one ? 1.0 : 2.0;
-                    sa = waveSum(w * sa);
-                    ss = waveSum(w * ss);
-                    if (lane == 0)
-                    {
-                        double *o = A.out + 4 * static_cast<long long>(A.ket.mol[cd]) + (A.kind == 3 ? 0 : 2);
-                        unsafeAtomicAdd(o, sa);
-                        unsafeAtomicAdd(o + 1, ss);
-                    }
-                }
-                else
-                {
-                    if (lane == 0)
-                    {
-                        double m = 0.0;
-                        for (int mn = 0; mn < SP; mn++)
-                            m = fmax(m, fabs(sI[mn * SQ + mn]));
-                        A.qout[ab] = sqrt(m);
-                    }
-                    if (A.dout)
-                        for (int mn = lane; mn < SP; mn += 64)
+                        const int mu = mn / SB, nu = mn % SB;
+                        const double v = sI[mn * SQ + mn];
+                        if (A.kind == 2)
+                            A.dout[a0 + mu] = v;
+                        else
                         {
-                            const int mu = mn / SB, nu = mn % SB;
-                            const double v = sI[mn * SQ + mn];
-                            if (A.kind == 2)
-                                A.dout[a0 + mu] = v;
-                            else
-                            {
-                                A.dout[(a0 + mu) * A.nbf + b0 + nu] = v;
-                                A.dout[(b0 + nu) * A.nbf + a0 + mu] = v;
-                            }
+                            A.dout[(a0 + mu) * A.nbf + b0 + nu] = v;
+                            A.dout[(b0 + nu) * A.nbf + a0 + mu] = v;
                         }
-                }
-                __syncthreads();
+                    }
             }
+            __syncthreads();
         }
     }
 
@@ -1086,20 +1078,12 @@ namespace lbx
         hipLaunchKernelGGL((quartetV<LA, LB, LC, LD, MODE, SWAP, T>), dim3(grid), dim3(256), 0, s, A);
     }
 
-    // grid: up to 16384 one-wave workgroups, no more than there are tasks (or, with per_q,
-    // uniform positions)
-    inline int mfmaGrid(const KArgs &A)
-    {
-        const long long n = A.per_q ? A.nket : A.ntasks;
-        return static_cast<int>(n < 16384 ? n : 16384);
-    }
-
     template <int LA, int LB, int LC, int LD, int MODE>
     void launchMfma(const KArgs &A, hipStream_t s)
     {
         if (A.ntasks <= 0)
             return;
-        int grid = mfmaGrid(A);
+        int grid = static_cast<int>(A.ntasks < 16384 ? A.ntasks : 16384);
         hipLaunchKernelGGL((quartetM<LA, LB, LC, LD, MODE>), dim3(grid), dim3(64), 0, s, A);
     }
 
@@ -1108,7 +1092,7 @@ namespace lbx
     {
         if (A.ntasks <= 0)
             return;
-        int grid = mfmaGrid(A);
+        int grid = static_cast<int>(A.ntasks < 16384 ? A.ntasks : 16384);
         hipLaunchKernelGGL((quartetM<LA, LB, LC, LD, MODE_JK, float>), dim3(grid), dim3(64), 0, s, A);
     }
 
