@@ -243,7 +243,9 @@ def leg_c4() -> dict:
 
 
 def leg_c5() -> dict:
-    """The 60-atom nuclear gradient, one pass after the first call."""
+    """The 60-atom nuclear gradient, one pass: the first call, whose set-up (Schwarz factors,
+    E' matrices, task tables) is ~1% of it (67.6 s against 67.5 s for a repeated call), so the
+    default run stays within its time budget."""
     import torch
     from lible_amd import basis as B
     from lible_amd import lbx
@@ -252,13 +254,12 @@ def leg_c5() -> dict:
     ctx = lbx.Context(mol)
     D = torch.tensor(B.density_from_orbitals(cfg["seed"] + 100, ctx.nbf, cfg["nocc"], cfg["sigma"]),
                      device="cuda")
-    g, _ = ctx.jk_grad(D, schwarz=SCHWARZ)
-    torch.cuda.synchronize()
-    t, (_, st) = timed(lambda: ctx.jk_grad(D, schwarz=SCHWARZ, grad=g), 1, "c5 gradient")
+    t, (g, st) = timed(lambda: ctx.jk_grad(D, schwarz=SCHWARZ), 1, "c5 gradient (first call)")
     gh = g.cpu().numpy()
     ctx.close()
     log("[bench] c5 done")
-    return {"workload": "c5_gradient", "ms": t * 1e3, "quartets_per_s": st["quartets"] / t,
+    return {"workload": "c5_gradient", "atoms": mol.natoms, "nbf": int(mol.shells.nbf),
+            "ms_first_call": t * 1e3, "quartets": st["quartets"], "quartets_per_s": st["quartets"] / t,
             "tflops": st["flops"] / t / 1e12,
             "max_abs_translation_sum": float(abs(gh.sum(axis=0)).max())}
 
@@ -499,7 +500,7 @@ def main(argv=None):
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--mixed-steps", type=int, default=1,
                     help="timed mixed-precision builds after the FP64 ones (0: skip)")
-    ap.add_argument("--legs", default="c1,c3,c4,ri,scf",
+    ap.add_argument("--legs", default="c1,c3,c4,c5,ri,scf",
                     help="other workloads measured after the main one, one pass each: c1 (dense "
                          "eri4, 84 functions), c3 (120-atom FP64 and mixed J/K), c4 (256-molecule "
                          "RI batch), c5 (60-atom gradient), ri (density-fitted J/K), scf (RHF "
