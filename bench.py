@@ -224,6 +224,7 @@ def leg_c4() -> dict:
     from lible_amd import lbx
     mols = B.batch_molecules()
     ctx = lbx.Context.batch(mols)
+    ctx.set_graphs(True)
     out = torch.empty((len(mols), 4), dtype=torch.float64, device="cuda")
     ctx.ri_sums(out)
     torch.cuda.synchronize()
@@ -348,6 +349,7 @@ def bench_ri_batch(a):
     mols = B.batch_molecules()
     t0 = time.perf_counter()
     ctx = lbx.Context.batch(mols)
+    ctx.set_graphs(True)
     torch.cuda.synchronize()
     t_setup = time.perf_counter() - t0
     log(f"[bench] c4_ri_batch: {len(mols)} molecules, {sum(m.natoms for m in mols)} atoms, "

@@ -181,9 +181,10 @@ int lbx_one_electron_grad(lbx_context *ctx, const double *D, const double *W, in
  * lbx_ri_sums evaluates every molecule's (mn|P) and (P|Q) and reduces them on the fly: out is a
  * device (nmol, 4) array of the sums of |value| and of value over the dense eri3 tensor
  * (nbf, nbf, naux) and over the dense eri2 tensor (naux, naux) of each molecule, i.e. the
- * checksums the reference's tests take of eri3() and eri2() (tests_ints.cpp:694-746).  The
- * whole launch sequence is captured into a HIP graph on the first call (per out pointer) and
- * replayed afterwards.  Also valid on an ordinary context with an auxiliary basis (nmol = 1). */
+ * checksums the reference's tests take of eri3() and eri2() (tests_ints.cpp:694-746).  With
+ * lbx_set_graphs on, the whole launch sequence is captured into a HIP graph on the first call
+ * (per out pointer) and replayed afterwards.  Also valid on an ordinary context with an auxiliary
+ * basis (nmol = 1). */
 typedef struct lbx_ri_stats {
     double triplets;        /* (ab|P) shell triplets evaluated */
     double doublets;        /* (P|Q) shell doublets evaluated */
@@ -195,7 +196,8 @@ int lbx_create_batch(lbx_context **ctx, int nmol, const lbx_basis *main_bases,
                      const lbx_basis *aux_bases);
 /* precision LBX_FP64, or LBX_MIXED: the triplets (doublets) whose Schwarz bound Q_ab Q_P
  * (Q_P Q_Q), Q = sqrt(max |(xx|xx)|), is below mixed_thresh in FP32 arithmetic with FP64 Boys
- * values and sums; nothing is screened out.  One graph per (out, precision split). */
+ * values and sums; nothing is screened out.  With graphs on, one graph per (out, precision
+ * split). */
 int lbx_ri_sums(lbx_context *ctx, double *out, int precision, double mixed_thresh,
                 lbx_ri_stats *stats);
 
@@ -261,6 +263,16 @@ int lbx_set_class_filter(lbx_context *ctx, int la, int lb, int lc, int ld);
  * [threshold, split): the same quartets in FP64 and in FP32, with partial sums small enough
  * that FP64 rounding does not hide the difference). */
 int lbx_set_bound_window(lbx_context *ctx, double lo, double hi);
+
+/* HIP graphs for repeated calls, no reference counterpart; off by default.  On: lbx_jk captures
+ * the launch sequence of a (D, J, K, thresholds) set into a graph after its first call and
+ * replays it on later calls with the same pointers (the kernels read the buffers' current
+ * contents, so an SCF loop that reuses its buffers replays); lbx_ri_sums captures its launches
+ * per output buffer and precision split.  The capture runs on a private non-blocking stream, but
+ * while it is in progress the HIP runtime rejects synchronous calls that other threads of the
+ * process make on the legacy (null) stream, so a program that drives the GPU from several
+ * threads keeps graphs off.  Turning graphs off drops the captured ones. */
+int lbx_set_graphs(lbx_context *ctx, int on);
 
 #ifdef __cplusplus
 }

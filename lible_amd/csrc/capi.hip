@@ -646,6 +646,10 @@ struct lbx_context : Context
     std::map<std::pair<int, int>, int> tuned, tuned32;
     // lbx_set_class_filter: the one class pair (bra class, ket class) lbx_jk evaluates; -1: all
     int filter_i = -1, filter_j = -1;
+    // lbx_set_graphs: capture the J/K launch sequences and the RI sums into HIP graphs and
+    // replay them (off by default: a capture in progress makes synchronous HIP calls of other
+    // threads on the legacy stream fail)
+    bool graphs = false;
     // lbx_set_bound_window: lbx_jk evaluates only quartets with window_lo <= Q_ab Q_cd < window_hi
     double window_lo = 0.0, window_hi = kNoUpper;
     // lbx_eri*_batch: (shell a, shell b) -> pair index per main pair class
@@ -1912,8 +1916,8 @@ extern "C"
                            // Replays: the launch sequence of a (D, J, K, thresholds) set is
                            // captured into a HIP graph after its first (eager) build and replayed
                            // on later calls; per-class timing and the class filter run eagerly.
-                           const bool graphs = !ctx->profiling && ctx->filter_i < 0 && ctx->window_lo <= 0.0 &&
-                                               ctx->window_hi == kNoUpper;
+                           const bool graphs = ctx->graphs && !ctx->profiling && ctx->filter_i < 0 &&
+                                               ctx->window_lo <= 0.0 && ctx->window_hi == kNoUpper;
                            const lbx_context::JKKey gkey{D, J, K, schwarz_thresh, split, mixed ? 1 : 0};
                            if (graphs)
                            {
@@ -1954,6 +1958,33 @@ extern "C"
                            if (c.shells_aux.empty())
                                throw Error(LBX_ERR_STATE, "lbx_ri_sums: the context has no auxiliary basis");
                            const double split = precision == LBX_MIXED ? mixed_thresh : 0.0;
+                           auto report = [&]
+                           {
+                               if (stats)
+                               {
+                                   stats->triplets = ctx->ri_triplets;
+                                   stats->doublets = ctx->ri_doublets;
+                                   stats->flops = ctx->ri_flops;
+                                   stats->triplets_fp32 = ctx->ri_triplets32;
+                                   stats->doublets_fp32 = ctx->ri_doublets32;
+                               }
+                           };
+                           if (!ctx->graphs)
+                           {
+                               // eager launches on the context stream
+                               if (split > 0.0)
+                               {
+                                   ensureSchwarz(c);
+                                   ensureSchwarzAux(c);
+                                   ensureF32(c);
+                                   ensureF32Aux(c);
+                               }
+                               ctx->prepareRiSums(split);
+                               ctx->queueRiSums(out, c.stream);
+                               LBX_HIP(hipStreamSynchronize(c.stream));
+                               report();
+                               return;
+                           }
                            if (!ctx->ri_exec || ctx->ri_out != out || ctx->ri_split != split)
                            {
                                ctx->dropRiGraph();
@@ -1999,13 +2030,22 @@ extern "C"
                            }
                            LBX_HIP(hipGraphLaunch(ctx->ri_exec, c.stream));
                            LBX_HIP(hipStreamSynchronize(c.stream));
-                           if (stats)
+                           report();
+                       });
+    }
+
+    int lbx_set_graphs(lbx_context *ctx, int on)
+    {
+        return guarded([&]
+                       {
+                           if (!ctx)
+                               throw Error(LBX_ERR_INVALID, "lbx_set_graphs: null context");
+                           std::lock_guard<std::mutex> lk(ctx->mu);
+                           ctx->graphs = on != 0;
+                           if (!ctx->graphs)
                            {
-                               stats->triplets = ctx->ri_triplets;
-                               stats->doublets = ctx->ri_doublets;
-                               stats->flops = ctx->ri_flops;
-                               stats->triplets_fp32 = ctx->ri_triplets32;
-                               stats->doublets_fp32 = ctx->ri_doublets32;
+                               ctx->dropJKGraphs();
+                               ctx->dropRiGraph();
                            }
                        });
     }

@@ -25,7 +25,7 @@ SYMBOLS = ["lbx_create", "lbx_destroy", "lbx_last_error", "lbx_dims", "lbx_set_s
            "lbx_eri2d1_batch", "lbx_eri4_batch_static", "lbx_eri4soc_batch", "lbx_eri3soc_batch", "lbx_eri2d2_batch", "lbx_jk", "lbx_create_batch", "lbx_ri_sums", "lbx_jk_grad",
            "lbx_tune", "lbx_set_variant", "lbx_profile_enable",
            "lbx_profile_read", "lbx_set_class_filter", "lbx_set_bound_window", "lbx_ri_jk",
-           "lbx_one_electron", "lbx_one_electron_grad"]
+           "lbx_one_electron", "lbx_one_electron_grad", "lbx_set_graphs"]
 VARIANTS = ("valu-bra", "valu-ket", "mfma", "valu-bra-f32", "valu-ket-f32", "mfma-f32", "block",
             "block-f32")
 
@@ -99,6 +99,7 @@ def lib():
         L.lbx_ri_jk.argtypes = [p, p, p, p]
         L.lbx_one_electron.argtypes = [p, p, p, i, p, p]
         L.lbx_one_electron_grad.argtypes = [p, p, p, i, p, p, p]
+        L.lbx_set_graphs.argtypes = [p, i]
         L.lbx_profile_enable.argtypes = [p, i]
         L.lbx_profile_read.argtypes = [p, ctypes.POINTER(ClassProfile), i, ctypes.POINTER(i)]
         _LIB = L
@@ -387,6 +388,11 @@ class Context:
     def set_class_filter(self, cls: tuple | None):
         """Restricts the following jk() calls to one class pair (la, lb, lc, ld); None: all."""
         _check(lib().lbx_set_class_filter(self.h, *(cls if cls is not None else (-1, -1, -1, -1))))
+
+    def set_graphs(self, on: bool):
+        """Capture repeated jk() / ri_sums() launch sequences into HIP graphs and replay them
+        (lbx_set_graphs; off by default, keep off when several threads drive the GPU)."""
+        _check(lib().lbx_set_graphs(self.h, int(bool(on))))
 
     def set_bound_window(self, lo: float = 0.0, hi: float = float("inf")):
         """Restricts the following jk() calls to quartets with lo <= Q_ab Q_cd < hi."""

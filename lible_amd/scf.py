@@ -74,6 +74,7 @@ class RHF:
             raise ValueError("density fitting needs an auxiliary basis")
         self.schwarz, self.precision, self.mixed_thresh = schwarz, precision, mixed_thresh
         self.ctx = L.Context(mol)
+        self.ctx.set_graphs(True)   # the J/K buffers are reused every iteration: replays
         self.nuclei = np.column_stack([mol.coords, self.Z])
         S, T, V = self.ctx.one_electron(charges=self.nuclei)
         self.S, self.H = S, T + V
@@ -149,7 +150,6 @@ class RHF:
 
     def energy_weighted_density(self):
         """W = 2 sum_occ eps_i C_i C_i^T of the converged orbitals."""
-        r = self.result
         eF, C, _ = self._density(self._F)
         Cocc = C[:, :self.nocc]
         return 2.0 * (Cocc * eF[:self.nocc]) @ Cocc.T

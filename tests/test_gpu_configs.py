@@ -134,6 +134,7 @@ def test_c4_ri_batch_full_size(lbx):
     mols = B.batch_molecules()
     assert [m.natoms for m in mols] == fx["natoms"]
     ctx = lbx.Context.batch(mols)
+    ctx.set_graphs(True)
     out, st = ctx.ri_sums()
     got = out.cpu().numpy()
     ref = np.asarray(fx["sums"])

@@ -41,6 +41,7 @@ def test_nothing_survives_the_screen(lbx):
     import torch
     m = B.synthetic_molecule(5, 3, 4.0, B.RECIPE_SMALL)
     ctx = lbx.Context(m)
+    ctx.set_graphs(True)
     D = torch.tensor(B.density_from_orbitals(1, ctx.nbf, 4, 0.1), device="cuda")
     J, K = torch.full_like(D, 7.0), torch.full_like(D, 7.0)
     for _ in range(2):
@@ -92,3 +93,48 @@ def test_state_errors(lbx):
         bctx.one_electron()
     sums, _ = bctx.ri_sums()
     assert np.allclose(sums[0].cpu().numpy(), sums[1].cpu().numpy(), rtol=1e-13)
+
+
+def test_threads_share_contexts(lbx):
+    """Calls on one context from several threads are serialised by the context's lock; calls on
+    two contexts interleave; every result equals the single-threaded one (J/K to the rounding of
+    the atomic accumulation order)."""
+    import threading
+    import torch
+    ma = B.synthetic_molecule(41, 3, 4.0, B.RECIPE_SMALL)
+    mb = B.synthetic_molecule(42, 2, 3.0, B.RECIPE_SMALL)
+    ca, cb = lbx.Context(ma), lbx.Context(mb)
+    D = torch.tensor(B.density_from_orbitals(6, ca.nbf, 6, 0.1), device="cuda")
+    J0, K0, _ = ca.jk(D, schwarz=0.0)
+    J0, K0 = J0.cpu().numpy(), K0.cpu().numpy()
+    E0 = cb.eri4().cpu().numpy()
+    errors, results = [], []
+
+    def jk_worker():
+        try:
+            for _ in range(3):
+                J, K, _ = ca.jk(D, schwarz=0.0)
+                results.append(("jk", J.cpu().numpy(), K.cpu().numpy()))
+        except Exception as e:   # reported below
+            errors.append(e)
+
+    def eri_worker():
+        try:
+            for _ in range(3):
+                results.append(("eri4", cb.eri4().cpu().numpy(), None))
+        except Exception as e:
+            errors.append(e)
+
+    threads = [threading.Thread(target=f) for f in (jk_worker, jk_worker, eri_worker, eri_worker)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert not errors, errors
+    assert len(results) == 12
+    for kind, x, y in results:
+        if kind == "jk":
+            assert np.abs(x - J0).max() <= 1e-13 * np.abs(J0).max()
+            assert np.abs(y - K0).max() <= 1e-13 * np.abs(K0).max()
+        else:
+            assert np.array_equal(x, E0)

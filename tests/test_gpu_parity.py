@@ -362,8 +362,11 @@ def test_ri_sums_batch(lbx, oracle):
     mols = [B.synthetic_molecule(300 + k, 2 + k % 4, 3.0 + k % 3, B.RECIPE_MAIN, B.RECIPE_AUX)
             for k in range(7)]
     ctx = lbx.Context.batch(mols)
+    eager, _ = ctx.ri_sums()          # graphs off: eager launches
+    ctx.set_graphs(True)
     out, st = ctx.ri_sums()
     first = out.cpu().numpy().copy()
+    close(eager.cpu().numpy(), first, rel=1e-13, floor=0.0)
     for k, m in enumerate(mols):
         s = oracle.Structure(m)
         close(first[k], dense_sums(s.eri3(), s.eri2()))
@@ -543,6 +546,7 @@ def test_jk_graph_replays(lbx, jk_case):
     import torch
     m, D, Jr, Kr, sr = jk_case
     ctx = lbx.Context(m)
+    ctx.set_graphs(True)
     Dt = torch.tensor(D, device="cuda")
     J, K = torch.empty_like(Dt), torch.empty_like(Dt)
     ctx.jk(Dt, schwarz=1e-12, J=J, K=K)
@@ -564,6 +568,23 @@ def test_jk_graph_replays(lbx, jk_case):
     ctx.jk(Dt, schwarz=1e-12, J=J, K=K)
     close(J.cpu().numpy(), Jr, floor=1e-13)
     close(K.cpu().numpy(), Kr, floor=1e-13)
+    # new contents in the same density buffer (an SCF iteration): the replay reads them
+    Dt.mul_(0.5)
+    ctx.jk(Dt, schwarz=1e-12, J=J, K=K)
+    close(J.cpu().numpy(), 0.5 * Jr, floor=1e-13)
+    close(K.cpu().numpy(), 0.5 * Kr, floor=1e-13)
+    Dt.mul_(2.0)
+    # the mixed-precision launch sequence (two tiers per class pair) replayed as well
+    Jm, Km = torch.empty_like(Dt), torch.empty_like(Dt)
+    ctx.jk(Dt, schwarz=1e-12, precision=lbx.LBX_MIXED, mixed_thresh=1e-5, J=Jm, K=Km)
+    Jm0, Km0 = Jm.cpu().numpy().copy(), Km.cpu().numpy().copy()
+    close(Jm0, Jr, rel=0.0, floor=1e-9)
+    close(Km0, Kr, rel=0.0, floor=1e-9)
+    for _ in range(2):
+        Jm.fill_(1.0)
+        ctx.jk(Dt, schwarz=1e-12, precision=lbx.LBX_MIXED, mixed_thresh=1e-5, J=Jm, K=Km)
+        close(Jm.cpu().numpy(), Jm0, rel=1e-13, floor=1e-15)
+        close(Km.cpu().numpy(), Km0, rel=1e-13, floor=1e-15)
 
 
 def test_jk_energy_autograd(lbx, oracle):
