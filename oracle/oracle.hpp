@@ -2,7 +2,8 @@
 //
 // TEST INFRASTRUCTURE ONLY.  Nothing in the product (lible_amd/, include/) may include, link or
 // call this code; only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it,
-// and only as the checker.  Each function names the reference file:line it follows.
+// and only as the reference results are compared against (or, in bench.py, as the timed CPU
+// baseline).  Each function names the reference file:line it follows.
 //
 // Parity pins (what fixes each part independently of retyping the formula):
 //   Boys ............ KAT calcBoysF (tests_ints.cpp:1296-1313), F_m(0)=1/(2m+1), the erf closed

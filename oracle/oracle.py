@@ -1,7 +1,8 @@
 """ctypes binding of the CPU FP64 oracle (oracle/liblbx_oracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, `__graft_entry__.smoke()` and bench.py's
-cpu_baseline leg, always as the checker.  The product never imports this module.
+cpu_baseline leg, always as the results compared against (or the timed CPU baseline).  The
+product never imports this module.
 """
 from __future__ import annotations
 
