@@ -17,6 +17,11 @@
 //                     8-fold symmetry; Schwarz inequality.
 //   D1 integrals .... KATs deployERI4D1/ERI3D1/ERI2D1Kernel (tests_ints.cpp:961-1053); central
 //                     finite differences; translational invariance.
+//   D2, SOC ......... KATs deployERI2D2/ERI4SOC/ERI3SOCKernel (tests_ints.cpp:1055-1148); ERI2D2
+//                     = difference quotient of ERI2D1; ERI4SOC = d_Ay d_Bz - d_Az d_By of the
+//                     integrals (one-primitive shells, finite differences).
+//   One-electron .... KATs overlap, kineticEnergy (C2H4/cc-pVDZ), nuclearAttraction (CO2/6-31+G)
+//                     (tests_ints.cpp:167-319); unit diagonal of the normalised overlap.
 //   J/K/E/gradient .. contraction of the dense oracle eri4 tensor; J, K symmetric; E invariant
 //                     under rigid translation; gradient = central finite difference of E.
 #pragma once
