@@ -32,7 +32,7 @@ typedef struct lbx_basis {
     const double *coords;      /* host, natoms*3, bohr */
     int nshells;
     const int *shell_atom;     /* host, nshells */
-    const int *shell_l;        /* host, nshells; main basis l <= 3, auxiliary l <= 5 */
+    const int *shell_l;        /* host, nshells; main basis l <= 4, auxiliary l <= 5 */
     const int *shell_nprim;    /* host, nshells */
     const double *exps;        /* host, sum(nprim) */
     const double *coeffs;      /* host, sum(nprim) */
@@ -57,7 +57,7 @@ int lbx_set_stream(lbx_context *ctx, void *stream);
 int lbx_synchronize(lbx_context *ctx);
 
 /* Boys function F_0..F_n at count points (device x[count] -> device out[count*(n+1)]).
- * Replaces calcBoysF (boys_function.cpp:71-117); n <= 16. */
+ * Replaces calcBoysF (boys_function.cpp:71-117); n <= 17. */
 int lbx_boys(int n, long count, const double *x, double *out, void *stream);
 
 /* Dense spherical ERIs, device outputs, row-major:

@@ -352,7 +352,7 @@ namespace lible::ints
     };
 
     /// Tabulated Boys function (boys_function.hpp:10-59): F_0 .. F_{n+6} on x = 0, 0.01, .., 30,
-    /// the table behind the GPU kernels' Taylor expansions; n <= 16.
+    /// the table behind the GPU kernels' Taylor expansions; n <= 17.
     class BoysGrid
     {
     public:

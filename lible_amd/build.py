@@ -107,6 +107,7 @@ def build(jobs: int | None = None, verbose: bool = True, incremental: bool = Fal
         (os.path.join(CSRC, "ints_api.cpp"), os.path.join(BUILD, "ints_api.o"), []),
         (os.path.join(CSRC, "d1.hip"), os.path.join(BUILD, "d1.o"), []),
         (os.path.join(CSRC, "ri.hip"), os.path.join(BUILD, "ri.o"), []),
+        (os.path.join(CSRC, "generic.hip"), os.path.join(BUILD, "generic.o"), []),
     ]
     for u in range(N_UNITS):
         work.append((os.path.join(CSRC, "instances.hip"), os.path.join(BUILD, f"inst_{u}.o"),

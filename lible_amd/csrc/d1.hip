@@ -20,55 +20,20 @@
 // for every class pair with the angular momenta at run time, one wave per quartet, Hermite R and
 // the ket transform staged in LDS, the output block of one component accumulated in registers
 // (lanes own output elements), one component per pass over the primitive quartets.
-#include "device_common.hpp"
 #include "lbx_internal.hpp"
+#include "runtime_l.hpp"
 
 namespace lbx
 {
     namespace
     {
-        constexpr int kD1MaxL1 = 4 * kMaxLMain + 1;             // highest R level, (ff|ff) + 1
+        // shells up to f on every index (the LDS and register budgets below)
+        constexpr int kD1MaxL = 3;
+        constexpr int kD1MaxL1 = 4 * kD1MaxL + 1;               // highest R level, (ff|ff) + 1
         constexpr int kD1MaxHerm = nHerm(kD1MaxL1);             // 560
-        constexpr int kD1MaxY = nHerm(2 * kMaxLMain) * 49;      // H(ff) x S(ff)
+        constexpr int kD1MaxY = nHerm(2 * kD1MaxL) * 49;        // H(ff) x S(ff)
         constexpr int kD1MaxOut = 49 * 49;                      // S(ff) x S(ff)
         constexpr int kD1Acc = ceilDiv(kD1MaxOut, 64);          // output elements per lane
-
-        __device__ __forceinline__ void boysRuntime(double x, const double *__restrict__ tab, int L,
-                                                    double *F)
-        {
-            if (x == 0.0)
-            {
-                F[0] = 1.0;
-                for (int k = 1; k <= L; k++)
-                    F[k] = 1.0 / (2 * k + 1);
-            }
-            else if (x > kBoysLargeX)
-            {
-                const double rx = 1.0 / x;
-                F[0] = 0.5 * sqrt(M_PI * rx);
-                for (int k = 1; k <= L; k++)
-                    F[k] = F[k - 1] * (k - 0.5) * rx;
-            }
-            else
-            {
-                const int iv = static_cast<int>(x * 100.0);
-                const double dx = x - iv * kBoysStep;
-                const double *row = tab + iv * kBoysCols + L;
-                const double m = -dx;
-                double s = row[6] * (1.0 / 720.0);
-                s = fma(s, m, row[5] * (1.0 / 120.0));
-                s = fma(s, m, row[4] * (1.0 / 24.0));
-                s = fma(s, m, row[3] * (1.0 / 6.0));
-                s = fma(s, m, row[2] * 0.5);
-                s = fma(s, m, row[1]);
-                s = fma(s, m, row[0]);
-                F[L] = s;
-                const double ex = exp(-x);
-                const double x2 = 2.0 * x;
-                for (int k = L - 1; k >= 0; k--)
-                    F[k] = fma(x2, F[k + 1], ex) * (1.0 / (2 * k + 1));
-            }
-        }
 
         // image permutations: image position j holds original index perm[j]
         // second derivatives of (a|b): the six distinct (i, j), i <= j, of the A block
@@ -132,37 +97,8 @@ namespace lbx
                             if (lane == 0)
                                 boysRuntime(alpha * (X * X + Y * Y + Z * Z), A.boys, L1, sF);
                             __syncthreads();
-                            // Hermite R: level m from level m + 1, m = L1 .. 0 (HJO 9.9.18-20)
-                            const double fac = kTwoPi52 / (pq * sqrt(ps));
-                            int cur = 0;
-                            if (lane == 0)
-                                sR[0][0] = sF[L1] * fac * pow(-2.0 * alpha, L1);
-                            __syncthreads();
-                            for (int m = L1 - 1; m >= 0; m--)
-                            {
-                                const int nxt = cur ^ 1, top = nHerm(L1 - m);
-                                for (int e = lane; e < top; e += 64)
-                                {
-                                    const int tuv = sTuv[e];
-                                    const int t = tuv & 255, u = (tuv >> 8) & 255, v = tuv >> 16;
-                                    double r;
-                                    if (t > 0)
-                                        r = (t > 1 ? (t - 1) * sR[cur][hermIdx(t - 2, u, v)] : 0.0) +
-                                            X * sR[cur][hermIdx(t - 1, u, v)];
-                                    else if (u > 0)
-                                        r = (u > 1 ? (u - 1) * sR[cur][hermIdx(t, u - 2, v)] : 0.0) +
-                                            Y * sR[cur][hermIdx(t, u - 1, v)];
-                                    else if (v > 0)
-                                        r = (v > 1 ? (v - 1) * sR[cur][hermIdx(t, u, v - 2)] : 0.0) +
-                                            Z * sR[cur][hermIdx(t, u, v - 1)];
-                                    else
-                                        r = sF[m] * fac * pow(-2.0 * alpha, m);
-                                    sR[nxt][e] = r;
-                                }
-                                __syncthreads();
-                                cur = nxt;
-                            }
-                            const double *R = sR[cur];
+                            const double *R = hermiteRLds(sR[0], sR[1], sTuv, sF, L1, kTwoPi52 / (pq * sqrt(ps)),
+                                                          alpha, X, Y, Z, lane);
 
                             if (extra)
                             {

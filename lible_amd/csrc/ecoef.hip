@@ -278,7 +278,7 @@ namespace lbx
     if (la == A && lb == B)                                                                        \
         return &launchEcoefD1<A, B>;
             LBX_E(0, 0) LBX_E(1, 0) LBX_E(1, 1) LBX_E(2, 0) LBX_E(2, 1) LBX_E(2, 2) LBX_E(3, 0)
-            LBX_E(3, 1) LBX_E(3, 2) LBX_E(3, 3)
+            LBX_E(3, 1) LBX_E(3, 2) LBX_E(3, 3) LBX_E(4, 0) LBX_E(4, 1) LBX_E(4, 2) LBX_E(4, 3) LBX_E(4, 4)
 #undef LBX_E
             throw Error(LBX_ERR_INVALID, "no derivative E-matrix kernel for this angular momentum pair");
         }
@@ -289,7 +289,8 @@ namespace lbx
     if (la == A && lb == B)                                                                        \
         return &launchEcoef<A, B>;
             LBX_E(0, 0) LBX_E(1, 0) LBX_E(1, 1) LBX_E(2, 0) LBX_E(2, 1) LBX_E(2, 2) LBX_E(3, 0)
-            LBX_E(3, 1) LBX_E(3, 2) LBX_E(3, 3) LBX_E(4, 0) LBX_E(5, 0)
+            LBX_E(3, 1) LBX_E(3, 2) LBX_E(3, 3) LBX_E(4, 0) LBX_E(4, 1) LBX_E(4, 2) LBX_E(4, 3) LBX_E(4, 4)
+            LBX_E(5, 0)
 #undef LBX_E
             throw Error(LBX_ERR_INVALID, "no E-matrix kernel for this angular momentum pair");
         }
@@ -698,7 +699,7 @@ namespace lbx
     if (la == A && lb == B)                                                                        \
         return &launchOneGrad<A, B>;
             LBX_E(0, 0) LBX_E(1, 0) LBX_E(1, 1) LBX_E(2, 0) LBX_E(2, 1) LBX_E(2, 2) LBX_E(3, 0)
-            LBX_E(3, 1) LBX_E(3, 2) LBX_E(3, 3)
+            LBX_E(3, 1) LBX_E(3, 2) LBX_E(3, 3) LBX_E(4, 0) LBX_E(4, 1) LBX_E(4, 2) LBX_E(4, 3) LBX_E(4, 4)
 #undef LBX_E
             throw Error(LBX_ERR_INVALID, "no one-electron gradient kernel for this angular momentum pair");
         }
@@ -709,7 +710,7 @@ namespace lbx
     if (la == A && lb == B)                                                                        \
         return &launchST<A, B>;
             LBX_E(0, 0) LBX_E(1, 0) LBX_E(1, 1) LBX_E(2, 0) LBX_E(2, 1) LBX_E(2, 2) LBX_E(3, 0)
-            LBX_E(3, 1) LBX_E(3, 2) LBX_E(3, 3)
+            LBX_E(3, 1) LBX_E(3, 2) LBX_E(3, 3) LBX_E(4, 0) LBX_E(4, 1) LBX_E(4, 2) LBX_E(4, 3) LBX_E(4, 4)
 #undef LBX_E
             throw Error(LBX_ERR_INVALID, "no one-electron kernel for this angular momentum pair");
         }
@@ -720,7 +721,7 @@ namespace lbx
     if (la == A && lb == B)                                                                        \
         return &launchV<A, B>;
             LBX_E(0, 0) LBX_E(1, 0) LBX_E(1, 1) LBX_E(2, 0) LBX_E(2, 1) LBX_E(2, 2) LBX_E(3, 0)
-            LBX_E(3, 1) LBX_E(3, 2) LBX_E(3, 3)
+            LBX_E(3, 1) LBX_E(3, 2) LBX_E(3, 3) LBX_E(4, 0) LBX_E(4, 1) LBX_E(4, 2) LBX_E(4, 3) LBX_E(4, 4)
 #undef LBX_E
             throw Error(LBX_ERR_INVALID, "no one-electron kernel for this angular momentum pair");
         }

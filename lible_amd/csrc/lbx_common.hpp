@@ -7,9 +7,9 @@
 
 namespace lbx
 {
-    constexpr int kMaxLMain = 3;    // main-basis shells: s, p, d, f
+    constexpr int kMaxLMain = 4;    // main-basis shells: s to g (compiled kernels to f, g at run-time L)
     constexpr int kMaxLAux = 5;     // auxiliary shells: up to h
-    constexpr int kMaxLBoys = 16;   // highest Boys order any kernel asks for
+    constexpr int kMaxLBoys = 17;   // highest Boys order any kernel asks for: (gg|gg) gradient
     constexpr int kBoysCols = kMaxLBoys + 7;   // table columns F_0..F_{n+6}
     constexpr int kBoysRows = 3001;            // x = 0, 0.01, ..., 30
     constexpr double kBoysStep = 0.01;

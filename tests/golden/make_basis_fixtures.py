@@ -5,7 +5,8 @@ The reference ships its basis sets as plain text (`element=Z`, `l=<l> n=<K>`, K 
 (`test/tests_ints.cpp:694-1148`) use O and H from def2-svp, def2-tzvp, def2-qzvp-rifit and
 def2-universal-jkfit.  This script copies exactly those element blocks, unchanged, into
 `tests/golden/basis/<name>.txt` so that the tests run without the reference tree (plus
-C and H of cc-pVDZ and C and O of 6-31+G for the one-electron known answers).
+C and H of cc-pVDZ and C and O of 6-31+G for the one-electron known answers, and H and O of
+def2-QZVP, a main basis with g shells).
 
 Run once in the build container:  python tests/golden/make_basis_fixtures.py
 """
@@ -23,6 +24,8 @@ WANTED = {
     # on CO2 in 6-31+G (tests_ints.cpp:167-318)
     ("lible_basis/dunning/cc-pvdz.txt", "cc-pvdz"): [1, 6],
     ("lible_basis/pople/6-31+g.txt", "6-31+g"): [6, 8],
+    # a main basis with g functions (O 7s4p3d2f1g, H 4s3p2d1f) for the g-shell parity tests
+    ("lible_basis/ahlrichs/def2-qzvp.txt", "def2-qzvp"): [1, 8],
 }
 
 

@@ -50,7 +50,7 @@ def test_boys(lbx, oracle):
                         [0.0, 1e-12, 0.005, 0.01, 29.995, 30.0, 30.0 + 1e-9, 45.0, 1e3]])
     xt = torch.tensor(x, dtype=torch.float64, device="cuda")
     # Both follow the reference's 7-term Taylor + downward recursion (accurate to ~1e-13
-    # relative); the table columns above n differ (one table to order 22 on the GPU, one per n in
+    # relative); the table columns above n differ (one table to order 23 on the GPU, one per n in
     # the oracle, as boys_function.cpp:6-10), so the two agree to a few 1e-13.
     for n in (0, 5, 13, 16):
         got = lbx.boys(n, xt).cpu().numpy()

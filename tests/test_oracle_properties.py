@@ -248,3 +248,73 @@ def test_eri2d2_is_derivative_of_eri2d1(oracle):
             fd = (p - q) / (2 * h)
             scale = np.abs(fd).max()
             assert np.abs(d2[:, j] - fd).max() <= 1e-6 * max(scale, 1e-12), (j, np.abs(d2[:, j] - fd).max())
+
+
+# ---- g shells in the main basis.  The reference's checksums reach g and h only in auxiliary
+# shells (the single-centre E of (g|, (h|) and the l = 8 spherical table); the main (gg| pair
+# E matrices, R to order 16 and the Boys function to order 16 are pinned here by invariants.
+RECIPE_G1 = [(0, 2, 1), (1, 1, 1), (2, 1, 1), (3, 1, 1), (4, 1, 1)]
+
+
+def _rotation(seed):
+    q, r = np.linalg.qr(np.random.default_rng(seed).normal(size=(3, 3)))
+    q = q * np.sign(np.diag(r))
+    return q if np.linalg.det(q) > 0 else -q
+
+
+def _shell_blocks(m):
+    l = np.asarray(m.shells.l)
+    off = np.concatenate([[0], np.cumsum(2 * l + 1)])
+    return [slice(off[i], off[i + 1]) for i in range(len(l))]
+
+
+def test_g_rotation_invariance(oracle):
+    """Rotating the molecule multiplies each shell's functions by an orthogonal matrix (real
+    spherical harmonics), so the Frobenius norm of every shell-quartet block of (ab|cd), and of
+    every shell-pair block of S, T and V (charges rotated with the atoms), is unchanged; a wrong
+    spherical coefficient, E term or R index for l = 4 changes them."""
+    m = B.synthetic_molecule(401, 2, 3.0, RECIPE_G1)
+    Q = _rotation(3)
+    centre = m.coords.mean(axis=0)
+    mr = B.Molecule((m.coords - centre) @ Q.T + centre, m.shells)
+    s, sr = oracle.Structure(m), oracle.Structure(mr)
+    e, er = s.eri4(), sr.eri4()
+    blk = _shell_blocks(m)
+    scale = np.abs(e).max()
+    for a, b, c, d in itertools.product(range(len(blk)), repeat=4):
+        n0 = np.linalg.norm(e[blk[a], blk[b], blk[c], blk[d]])
+        n1 = np.linalg.norm(er[blk[a], blk[b], blk[c], blk[d]])
+        assert abs(n0 - n1) <= 1e-12 * n0 + 1e-14 * scale, (a, b, c, d, n0, n1)
+    ch = np.column_stack([m.coords, [3.0, 5.0]])
+    chr_ = np.column_stack([mr.coords, [3.0, 5.0]])
+    for x, xr in ((s.overlap(), sr.overlap()), (s.kinetic(), sr.kinetic()),
+                  (s.external_charges(ch), sr.external_charges(chr_))):
+        xs = np.abs(x).max()
+        for a, b in itertools.product(range(len(blk)), repeat=2):
+            n0, n1 = np.linalg.norm(x[blk[a], blk[b]]), np.linalg.norm(xr[blk[a], blk[b]])
+            assert abs(n0 - n1) <= 1e-12 * n0 + 1e-14 * xs, (a, b, n0, n1)
+    # the g functions are unit-normalised; the (gg|gg) block of one shell, a matrix over the
+    # function pairs, is a Gram matrix of the pair densities: positive semidefinite (its
+    # antisymmetric pair combinations give zero)
+    assert np.abs(np.diag(s.overlap()) - 1.0).max() < 1e-13
+    g = blk[-1]
+    n = g.stop - g.start
+    ev = np.linalg.eigvalsh(e[g, g, g, g].reshape(n * n, n * n))
+    assert ev.min() >= -1e-13 * ev.max() and (ev > 1e-8 * ev.max()).sum() == n * (n + 1) // 2
+
+
+def test_g_gradient_finite_difference(oracle):
+    """The oracle's J/K gradient with g shells against central differences of its energy."""
+    m = B.synthetic_molecule(404, 2, 3.0, RECIPE_G1)
+    s = oracle.Structure(m)
+    D = B.density_from_orbitals(10, s.nbf, 8, 0.2)
+    g = s.jk_grad(D, thresh=0.0)
+    h = 1e-4
+    for ia, x in ((0, 0), (1, 2)):
+        cp, cm = m.coords.copy(), m.coords.copy()
+        cp[ia, x] += h
+        cm[ia, x] -= h
+        fd = (energy(oracle.Structure(B.Molecule(cp, m.shells)), D) -
+              energy(oracle.Structure(B.Molecule(cm, m.shells)), D)) / (2 * h)
+        assert abs(g[ia, x] - fd) < 1e-7 * max(1.0, abs(fd)), (ia, x, g[ia, x], fd)
+    assert np.abs(g.sum(axis=0)).max() < 1e-10 * np.abs(g).max()
