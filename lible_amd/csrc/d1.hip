@@ -98,7 +98,7 @@ namespace lbx
                                 boysRuntime(alpha * (X * X + Y * Y + Z * Z), A.boys, L1, sF);
                             __syncthreads();
                             const double *R = hermiteRLds(sR[0], sR[1], sTuv, sF, L1, kTwoPi52 / (pq * sqrt(ps)),
-                                                          alpha, X, Y, Z, lane);
+                                                          alpha, X, Y, Z, lane, 64);
 
                             if (extra)
                             {

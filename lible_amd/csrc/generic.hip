@@ -4,8 +4,9 @@
 //
 // One wave per shell quartet (the task tables of the one-wave MFMA form, so every J/K, dense,
 // diagonal, per-molecule-sum and gradient driver runs these classes unchanged).  Per quartet,
-// the ket columns are taken in chunks of whole c functions sized so that everything stays in
-// 64 KiB of LDS; per chunk and bra primitive pair i
+// the ket columns are taken in chunks of whole c functions sized to about 24 KiB of LDS (at
+// most 64 KiB), so that several workgroups stay resident per CU; per chunk and bra primitive
+// pair i (the chunk's block I in registers, then in LDS for the epilogue)
 //     Y[h][cd] = sum_k sum_g R^{ik}_{h+g} Ek_k[g][cd]         (R of each primitive quartet in LDS)
 //     I[ab][cd] += sum_h Eb_i[h][ab] Y[h][cd]
 // and the chunk's block I goes through the epilogue of the mode (the contractions of the MFMA
@@ -29,7 +30,11 @@ namespace lbx
 {
     namespace
     {
-        constexpr int kGenLds = 64 * 1024 / 8;   // doubles of LDS per workgroup
+        constexpr int kGenThreads = 256;         // four waves share one quartet of a large class
+        constexpr int kGenLds = 64 * 1024 / 8;   // doubles of LDS per workgroup, at most
+        // below this many doubles the column chunk grows no further, so that four workgroups
+        // (16 waves) stay resident per CU (R is rebuilt per chunk, a small cost)
+        constexpr int kGenLdsTarget = 5 * 1024;
 
         // LDS layout of one class pair (offsets in doubles) and its column chunk
         struct GenPlan
@@ -41,6 +46,8 @@ namespace lbx
             int oDab, oDbd, oDad;    // density blocks fixed per quartet
             int oDcd, oDbc, oDac;    // density blocks of the chunk
             int oY, oX, oW, oI;      // chunk work arrays
+            int oRed;                // one double per wave: cross-wave maximum
+            int nt;                  // threads per workgroup: 64 or kGenThreads
             int doubles;
         };
 
@@ -72,8 +79,8 @@ namespace lbx
             const int fixed = o;
             // per c function of the chunk
             const int perC = (dens ? Sd + Sb + Sa : 0) + (grad ? (Sab + 2 * Hab) * Sd : (Hab + Sab) * Sd);
-            g.nc = std::min(Sc, (kGenLds - fixed) / perC);
-            if (g.nc < 1)
+            g.nc = std::min(Sc, std::max(1, (kGenLdsTarget - fixed - 4) / perC));
+            if (g.nc < 1 || fixed + 4 + g.nc * perC > kGenLds)
                 throw Error(LBX_ERR_INVALID, "class pair too large for the run-time-L kernel");
             const int cw = g.nc * Sd;
             g.oDcd = take(dens ? cw : 0);
@@ -83,7 +90,12 @@ namespace lbx
             g.oX = take(grad ? Hab * cw : 0);
             g.oW = take(grad ? Sab * cw : 0);
             g.oI = take(grad ? 0 : Sab * cw);
+            g.oRed = take(kGenThreads / 64);
             g.doubles = o;
+            // one wave when the widest loop per primitive quartet (Y, or the gradient's (h, g)
+            // pairs) has little work, four otherwise
+            const int widest = grad ? Hab * nHerm(lc + ld) : Hab * cw;
+            g.nt = widest < 512 ? 64 : kGenThreads;
             return g;
         }
 
@@ -92,7 +104,8 @@ namespace lbx
             return hermIdx((a & 255) + (b & 255), ((a >> 8) & 255) + ((b >> 8) & 255), (a >> 16) + (b >> 16));
         }
 
-        __global__ __launch_bounds__(64) void genericQuartet(const KArgs A, const GenPlan G, const int mode)
+        template <int NT>
+        __global__ __launch_bounds__(NT) void genericQuartet(const KArgs A, const GenPlan G, const int mode)
         {
             extern __shared__ double smem[];
             double *const sR0 = smem + G.oR0, *const sR1 = smem + G.oR1, *const sF = smem + G.oF;
@@ -105,7 +118,7 @@ namespace lbx
             const int Sa = nSph(G.la), Sb = nSph(G.lb), Sc = nSph(G.lc), Sd = nSph(G.ld);
             const int Sab = Sa * Sb, Scd = Sc * Sd;
             const long long n = A.nbf;
-            for (int h = lane; h < nHerm(G.L); h += 64)
+            for (int h = lane; h < nHerm(G.L); h += NT)
                 sTuv[h] = hermTuv(h);
             __syncthreads();
 
@@ -140,11 +153,11 @@ namespace lbx
                         f *= 0.5;
                     if (A.same_class && p == q)
                         f *= 0.5;
-                    for (int e = lane; e < Sab; e += 64)
+                    for (int e = lane; e < Sab; e += NT)
                         dab[e] = A.D[(a0 + e / Sb) * n + b0 + e % Sb];
-                    for (int e = lane; e < Sb * Sd; e += 64)
+                    for (int e = lane; e < Sb * Sd; e += NT)
                         dbd[e] = A.D[(b0 + e / Sd) * n + d0 + e % Sd];
-                    for (int e = lane; e < Sa * Sd; e += 64)
+                    for (int e = lane; e < Sa * Sd; e += NT)
                         dad[e] = A.D[(a0 + e / Sd) * n + d0 + e % Sd];
                 }
                 double dmax = 0.0, sabs = 0.0, ssum = 0.0;
@@ -152,15 +165,15 @@ namespace lbx
                 for (int kc0 = 0; kc0 < Sc; kc0 += G.nc)
                 {
                     const int ncw = min(G.nc, Sc - kc0), cw = ncw * Sd, col0 = kc0 * Sd;
-                    for (int e = lane; e < Sab * cw; e += 64)
+                    for (int e = lane; e < Sab * cw; e += NT)
                         sI[e] = 0.0;
                     if (mode == MODE_JK)
                     {
-                        for (int e = lane; e < cw; e += 64)
+                        for (int e = lane; e < cw; e += NT)
                             dcd[e] = A.D[(c0 + kc0 + e / Sd) * n + d0 + e % Sd];
-                        for (int e = lane; e < Sb * ncw; e += 64)
+                        for (int e = lane; e < Sb * ncw; e += NT)
                             dbc[e] = A.D[(b0 + e / ncw) * n + c0 + kc0 + e % ncw];
-                        for (int e = lane; e < Sa * ncw; e += 64)
+                        for (int e = lane; e < Sa * ncw; e += NT)
                             dac[e] = A.D[(a0 + e / ncw) * n + c0 + kc0 + e % ncw];
                     }
                     __syncthreads();
@@ -168,20 +181,19 @@ namespace lbx
                     {
                         const double4 P = A.bra.pp[i];
                         // the lane that zeroes an element of Y is the lane that accumulates it
-                        for (int e = lane; e < Hab * cw; e += 64)
+                        for (int e = lane; e < Hab * cw; e += NT)
                             sY[e] = 0.0;
                         for (int k = kpoff; k < kpoff + kcnt; k++)
                         {
                             const double4 Q = A.ket.pp[k];
                             const double pq = P.x * Q.x, ps = P.x + Q.x, alpha = pq / ps;
                             const double X = P.y - Q.y, Y = P.z - Q.z, Z = P.w - Q.w;
-                            if (lane == 0)
-                                boysRuntime(alpha * (X * X + Y * Y + Z * Z), A.boys, G.L, sF);
+                            boysLane(alpha * (X * X + Y * Y + Z * Z), A.boys, G.L, sF, lane);
                             __syncthreads();
                             const double *R =
-                                    hermiteRLds(sR0, sR1, sTuv, sF, G.L, kTwoPi52 / (pq * sqrt(ps)), alpha, X, Y, Z, lane);
+                                    hermiteRLds(sR0, sR1, sTuv, sF, G.L, kTwoPi52 / (pq * sqrt(ps)), alpha, X, Y, Z, lane, NT);
                             const double *ek = A.ket.ek + static_cast<size_t>(k) * Hcd * Scd + col0;
-                            for (int e = lane; e < Hab * cw; e += 64)
+                            for (int e = lane; e < Hab * cw; e += NT)
                             {
                                 const int h = e / cw, s = e - h * cw;
                                 const int th = sTuv[h];
@@ -193,7 +205,7 @@ namespace lbx
                             __syncthreads();
                         }
                         const double *eb = A.bra.eb + static_cast<size_t>(i) * Hab * Sab;
-                        for (int e = lane; e < Sab * cw; e += 64)
+                        for (int e = lane; e < Sab * cw; e += NT)
                         {
                             const int sab = e / cw, s = e - sab * cw;
                             double a = sI[e];
@@ -209,14 +221,14 @@ namespace lbx
                     if (mode == MODE_JK)
                     {
                         double *J = A.J, *K = A.K;
-                        for (int mn = lane; mn < Sab; mn += 64)
+                        for (int mn = lane; mn < Sab; mn += NT)
                         {
                             double s = 0.0;
                             for (int c = 0; c < cw; c++)
                                 s = fma(sI[mn * cw + c], dcd[c], s);
                             unsafeAtomicAdd(&J[(a0 + mn / Sb) * n + b0 + mn % Sb], 2.0 * f * s);
                         }
-                        for (int c = lane; c < cw; c += 64)
+                        for (int c = lane; c < cw; c += NT)
                         {
                             double s = 0.0;
                             for (int mn = 0; mn < Sab; mn++)
@@ -224,7 +236,7 @@ namespace lbx
                             unsafeAtomicAdd(&J[(c0 + kc0 + c / Sd) * n + d0 + c % Sd], 2.0 * f * s);
                         }
                         // K_ac[mu, ka] += f sum_{nu, la} I D_bd[nu, la]
-                        for (int e = lane; e < Sa * ncw; e += 64)
+                        for (int e = lane; e < Sa * ncw; e += NT)
                         {
                             const int mu = e / ncw, kk = e % ncw;
                             double s = 0.0;
@@ -234,7 +246,7 @@ namespace lbx
                             unsafeAtomicAdd(&K[(c0 + kc0 + kk) * n + a0 + mu], f * s);
                         }
                         // K_bc[nu, ka] += f sum_{mu, la} I D_ad[mu, la]
-                        for (int e = lane; e < Sb * ncw; e += 64)
+                        for (int e = lane; e < Sb * ncw; e += NT)
                         {
                             const int nu = e / ncw, kk = e % ncw;
                             double s = 0.0;
@@ -244,7 +256,7 @@ namespace lbx
                             unsafeAtomicAdd(&K[(c0 + kc0 + kk) * n + b0 + nu], f * s);
                         }
                         // K_ad[mu, la] += f sum_{nu, ka} I D_bc[nu, ka] (this chunk's ka)
-                        for (int e = lane; e < Sa * Sd; e += 64)
+                        for (int e = lane; e < Sa * Sd; e += NT)
                         {
                             const int mu = e / Sd, la = e % Sd;
                             double s = 0.0;
@@ -254,7 +266,7 @@ namespace lbx
                             unsafeAtomicAdd(&K[(d0 + la) * n + a0 + mu], f * s);
                         }
                         // K_bd[nu, la] += f sum_{mu, ka} I D_ac[mu, ka] (this chunk's ka)
-                        for (int e = lane; e < Sb * Sd; e += 64)
+                        for (int e = lane; e < Sb * Sd; e += NT)
                         {
                             const int nu = e / Sd, la = e % Sd;
                             double s = 0.0;
@@ -266,7 +278,7 @@ namespace lbx
                     }
                     else if (mode == MODE_DENSE)
                     {
-                        for (int e = lane; e < Sab * cw; e += 64)
+                        for (int e = lane; e < Sab * cw; e += NT)
                         {
                             const int mn = e / cw, kl = col0 + e % cw;
                             const double v = sI[e];
@@ -297,7 +309,7 @@ namespace lbx
                     }
                     else if (mode == MODE_SUM)
                     {
-                        for (int e = lane; e < Sab * cw; e += 64)
+                        for (int e = lane; e < Sab * cw; e += NT)
                         {
                             sabs += fabs(sI[e]);
                             ssum += sI[e];
@@ -306,7 +318,7 @@ namespace lbx
                     else
                     {
                         // (ab|ab) diagonal: element (mn, mn) lies in this chunk when its column does
-                        for (int mn = lane; mn < Sab; mn += 64)
+                        for (int mn = lane; mn < Sab; mn += NT)
                         {
                             const int c = mn - col0;
                             if (c < 0 || c >= cw)
@@ -336,7 +348,7 @@ namespace lbx
                     const double w = one ? 1.0 : 2.0;
                     sabs = waveSum(w * sabs);
                     ssum = waveSum(w * ssum);
-                    if (lane == 0)
+                    if ((lane & 63) == 0)   // each wave adds its part
                     {
                         double *o = A.out + 4 * static_cast<long long>(A.ket.mol[cd]) + (A.kind == 3 ? 0 : 2);
                         unsafeAtomicAdd(o, sabs);
@@ -346,13 +358,23 @@ namespace lbx
                 else if (mode == MODE_DIAG)
                 {
                     dmax = waveMax(dmax);
+                    double *const red = smem + G.oRed;
+                    if ((lane & 63) == 0)
+                        red[lane >> 6] = dmax;
+                    __syncthreads();
                     if (lane == 0)
+                    {
+                        for (int w = 1; w < NT / 64; w++)
+                            dmax = fmax(dmax, red[w]);
                         A.qout[ab] = sqrt(dmax);
+                    }
+                    __syncthreads();   // red of the next task
                 }
             }
         }
 
-        __global__ __launch_bounds__(64) void genericGrad(const GArgs A, const GenPlan G)
+        template <int NT>
+        __global__ __launch_bounds__(NT) void genericGrad(const GArgs A, const GenPlan G)
         {
             extern __shared__ double smem[];
             double *const sR0 = smem + G.oR0, *const sR1 = smem + G.oR1, *const sF = smem + G.oF;
@@ -367,7 +389,7 @@ namespace lbx
             const long long n = A.nbf;
             const double *D = A.D;
             const DPairs &LP = A.lane, &UP = A.unif;
-            for (int h = lane; h < nHerm(G.L); h += 64)
+            for (int h = lane; h < nHerm(G.L); h += NT)
                 sTuv[h] = hermTuv(h);
             __syncthreads();
 
@@ -397,26 +419,26 @@ namespace lbx
                     f *= 0.5;
                 if (A.same_class && p == q)
                     f *= 0.5;
-                for (int e = lane; e < Sab; e += 64)
+                for (int e = lane; e < Sab; e += NT)
                     dab[e] = D[(a0 + e / Sb) * n + b0 + e % Sb];
-                for (int e = lane; e < Sb * Sd; e += 64)
+                for (int e = lane; e < Sb * Sd; e += NT)
                     dbd[e] = D[(b0 + e / Sd) * n + d0 + e % Sd];
-                for (int e = lane; e < Sa * Sd; e += 64)
+                for (int e = lane; e < Sa * Sd; e += NT)
                     dad[e] = D[(a0 + e / Sd) * n + d0 + e % Sd];
                 double gA[3] = {0, 0, 0}, gB[3] = {0, 0, 0}, gC[3] = {0, 0, 0};
 
                 for (int kc0 = 0; kc0 < Sc; kc0 += G.nc)
                 {
                     const int ncw = min(G.nc, Sc - kc0), cw = ncw * Sd, col0 = kc0 * Sd;
-                    for (int e = lane; e < cw; e += 64)
+                    for (int e = lane; e < cw; e += NT)
                         dcd[e] = D[(c0 + kc0 + e / Sd) * n + d0 + e % Sd];
-                    for (int e = lane; e < Sb * ncw; e += 64)
+                    for (int e = lane; e < Sb * ncw; e += NT)
                         dbc[e] = D[(b0 + e / ncw) * n + c0 + kc0 + e % ncw];
-                    for (int e = lane; e < Sa * ncw; e += 64)
+                    for (int e = lane; e < Sa * ncw; e += NT)
                         dac[e] = D[(a0 + e / ncw) * n + c0 + kc0 + e % ncw];
                     __syncthreads();
                     // W[ab][c] = f (4 D_ab D_cd - D_ac D_bd - D_ad D_bc) over this chunk's columns
-                    for (int e = lane; e < Sab * cw; e += 64)
+                    for (int e = lane; e < Sab * cw; e += NT)
                     {
                         const int mn = e / cw, c = e - mn * cw;
                         const int mu = mn / Sb, nu = mn % Sb, kk = c / Sd, la = c % Sd;
@@ -429,7 +451,7 @@ namespace lbx
                         const double4 P = LP.pp[i];
                         const double rai = LP.ra[i];
                         const double *eb = LP.eb + static_cast<size_t>(i) * Hab * Sab;
-                        for (int e = lane; e < Hab * cw; e += 64)
+                        for (int e = lane; e < Hab * cw; e += NT)
                         {
                             const int h = e / cw, c = e - h * cw;
                             double y = 0.0;
@@ -445,16 +467,15 @@ namespace lbx
                             const double rck = UP.ra[k];
                             const double pq = P.x * Q.x, ps = P.x + Q.x, alpha = pq / ps;
                             const double X = P.y - Q.y, Y = P.z - Q.z, Z = P.w - Q.w;
-                            if (lane == 0)
-                                boysRuntime(alpha * (X * X + Y * Y + Z * Z), A.boys, G.L, sF);
+                            boysLane(alpha * (X * X + Y * Y + Z * Z), A.boys, G.L, sF, lane);
                             __syncthreads();
                             const double *R =
-                                    hermiteRLds(sR0, sR1, sTuv, sF, G.L, kTwoPi52 / (pq * sqrt(ps)), alpha, X, Y, Z, lane);
+                                    hermiteRLds(sR0, sR1, sTuv, sF, G.L, kTwoPi52 / (pq * sqrt(ps)), alpha, X, Y, Z, lane, NT);
                             const size_t ecount = static_cast<size_t>(Hcd) * Scd;
                             const double *ek = UP.ek + static_cast<size_t>(k) * ecount + col0;
                             const double *e1k = UP.e1k + static_cast<size_t>(k) * 3 * ecount + col0;
                             // T and V over the (h, g) pairs
-                            for (int e = lane; e < Hab * Hcd; e += 64)
+                            for (int e = lane; e < Hab * Hcd; e += NT)
                             {
                                 const int h = e / Hcd, g = e - h * Hcd;
                                 double z = 0.0, zx = 0.0, zy = 0.0, zz = 0.0;
@@ -482,7 +503,7 @@ namespace lbx
                                     gC[x] += V[x] - rck * T[x];
                                 }
                             }
-                            for (int e = lane; e < Hab * cw; e += 64)
+                            for (int e = lane; e < Hab * cw; e += NT)
                             {
                                 const int h = e / cw, c = e - h * cw;
                                 const int th = sTuv[h];
@@ -495,7 +516,7 @@ namespace lbx
                         }
                         // U_x = sum W[ab][c] sum_h Eb'_i,x[h][ab] X[h][c]
                         const double *e1b = LP.e1b + static_cast<size_t>(i) * 3 * Hab * Sab;
-                        for (int e = lane; e < Sab * cw; e += 64)
+                        for (int e = lane; e < Sab * cw; e += NT)
                         {
                             const int mn = e / cw, c = e - mn * cw;
                             const double w = sW[e];
@@ -520,7 +541,7 @@ namespace lbx
                     gB[x] = waveSum(gB[x]);
                     gC[x] = waveSum(gC[x]);
                 }
-                if (lane == 0)
+                if ((lane & 63) == 0)   // each wave adds its part
                 {
                     const int at[4] = {LP.atom_a[l], LP.atom_b[l], UP.atom_a[u], UP.atom_b[u]};
 #pragma unroll
@@ -543,8 +564,11 @@ namespace lbx
                 return;
             static const GenPlan G = makePlan(LA, LB, LC, LD, MODE == MODE_JK, false);
             const int grid = static_cast<int>(A.ntasks < 16384 ? A.ntasks : 16384);
-            hipLaunchKernelGGL(genericQuartet, dim3(grid), dim3(64), static_cast<size_t>(G.doubles) * 8, s, A, G,
-                               MODE);
+            const size_t lds = static_cast<size_t>(G.doubles) * 8;
+            if (G.nt == 64)
+                hipLaunchKernelGGL(genericQuartet<64>, dim3(grid), dim3(64), lds, s, A, G, MODE);
+            else
+                hipLaunchKernelGGL(genericQuartet<kGenThreads>, dim3(grid), dim3(kGenThreads), lds, s, A, G, MODE);
         }
 
         template <int LA, int LB, int LC, int LD>
@@ -554,7 +578,11 @@ namespace lbx
                 return;
             static const GenPlan G = makePlan(LA, LB, LC, LD, false, true);
             const int grid = static_cast<int>(A.ntasks < 16384 ? A.ntasks : 16384);
-            hipLaunchKernelGGL(genericGrad, dim3(grid), dim3(64), static_cast<size_t>(G.doubles) * 8, s, A, G);
+            const size_t lds = static_cast<size_t>(G.doubles) * 8;
+            if (G.nt == 64)
+                hipLaunchKernelGGL(genericGrad<64>, dim3(grid), dim3(64), lds, s, A, G);
+            else
+                hipLaunchKernelGGL(genericGrad<kGenThreads>, dim3(grid), dim3(kGenThreads), lds, s, A, G);
         }
 
         // The compiled units register, among the classes with a 4 on the bra's first index, only
