@@ -309,13 +309,7 @@ def leg_scf() -> dict:
     import numpy as np
     from lible_amd import basis as B
     from lible_amd import scf
-    nrs, xyz = [], []
-    for i in range(2):
-        for j in range(2):
-            for k in range(2):
-                for z, p in zip(*WATER):
-                    nrs.append(z)
-                    xyz.append([p[0] + 3.2 * i, p[1] + 3.2 * j, p[2] + 3.2 * k])
+    nrs, xyz = water_grid(2)
     path = os.path.join(ROOT, "tests", "golden", "basis", "def2-svp.txt")
     mol = B.molecule_from_basis_files(path, nrs, xyz)
     t0 = time.perf_counter()
@@ -336,7 +330,61 @@ def leg_scf() -> dict:
             "max_abs_translation_sum": float(np.abs(g.sum(axis=0)).max())}
 
 
-LEGS = {"c1": leg_c1, "c3": leg_c3, "c4": leg_c4, "c5": leg_c5, "ri": leg_ri, "scf": leg_scf}
+def water_grid(n: int):
+    """n x n x n water molecules 3.2 angstrom apart: atomic numbers and coordinates (angstrom)."""
+    nrs, xyz = [], []
+    for i in range(n):
+        for j in range(n):
+            for k in range(n):
+                for z, p in zip(*WATER):
+                    nrs.append(z)
+                    xyz.append([p[0] + 3.2 * i, p[1] + 3.2 * j, p[2] + 3.2 * k])
+    return nrs, xyz
+
+
+def leg_qzvp() -> dict:
+    """J/K of (H2O)8 in def2-QZVP (g shells on O: the run-time-L kernels): one build with the
+    default kernel choices, the autotuner, one tuned build, and the share of the g classes."""
+    import torch
+    from lible_amd import basis as B
+    from lible_amd import lbx
+    nrs, xyz = water_grid(2)
+    path = os.path.join(ROOT, "tests", "golden", "basis", "def2-qzvp.txt")
+    mol = B.molecule_from_basis_files(path, nrs, xyz)
+    ctx = lbx.Context(mol)
+    D = torch.tensor(B.density_from_orbitals(3, ctx.nbf, 40, 0.05), device="cuda")
+
+    def profiled():
+        ctx.profile(True)
+        ctx.jk(D, schwarz=SCHWARZ)
+        prof = ctx.profile_read()
+        ctx.profile(False)
+        return prof
+
+    ctx.jk(D, schwarz=SCHWARZ)   # first call: Schwarz factors, task tables
+    t_default, (_, _, st) = timed(lambda: ctx.jk(D, schwarz=SCHWARZ), 1, "qzvp default")
+    p_default = profiled()
+    t0 = time.perf_counter()
+    ctx.tune(D, schwarz=SCHWARZ)
+    t_tune = time.perf_counter() - t0
+    t_tuned, _ = timed(lambda: ctx.jk(D, schwarz=SCHWARZ), 1, "qzvp tuned")
+    p_tuned = profiled()
+    nbf = ctx.nbf
+    ctx.close()
+
+    def g_share(prof):
+        return sum(r["ms"] for r in prof if 4 in r["class"])
+
+    g_flops = sum(r["flops"] for r in p_tuned if 4 in r["class"])
+    log("[bench] qzvp done")
+    return {"workload": "(H2O)8 def2-QZVP J/K", "nbf": nbf, "quartets": st["quartets"],
+            "ms_default_kernels": t_default * 1e3, "ms_g_classes_default": g_share(p_default),
+            "tune_s": t_tune, "ms_tuned": t_tuned * 1e3, "ms_g_classes_tuned": g_share(p_tuned),
+            "g_classes_tflops": g_flops / max(g_share(p_tuned), 1e-9) / 1e9}
+
+
+LEGS = {"c1": leg_c1, "c3": leg_c3, "c4": leg_c4, "c5": leg_c5, "ri": leg_ri, "scf": leg_scf,
+        "qzvp": leg_qzvp}
 
 
 def bench_ri_batch(a):
@@ -506,7 +554,8 @@ def main(argv=None):
                     help="other workloads measured after the main one, one pass each: c1 (dense "
                          "eri4, 84 functions), c3 (120-atom FP64 and mixed J/K), c4 (256-molecule "
                          "RI batch), c5 (60-atom gradient), ri (density-fitted J/K), scf (RHF "
-                         "energy and gradient of (H2O)8)")
+                         "energy and gradient of (H2O)8); also qzvp (J/K of (H2O)8 in def2-QZVP, "
+                         "g shells), not in the default list")
     ap.add_argument("--pmc-calibration", action="store_true",
                     help="end with a 1 GiB device-to-device copy, the reference reading for "
                          "rocprofv3's FETCH_SIZE / WRITE_SIZE counters")

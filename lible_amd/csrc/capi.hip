@@ -415,12 +415,14 @@ namespace lbx
             return 2 * r;
         }
 
-        // Work estimate of a one-lane-per-quartet variant per quartet, from the classes' mean
-        // primitive-pair counts: ket transform and Boys + R per primitive quartet (R once per
-        // column pass), bra transform per lane-side primitive pair.
-        double valuCost(const PairClass &lane, const PairClass &unif, int nc)
+        // Work estimate of a one-lane-per-quartet variant per quartet, from the primitive-pair
+        // counts (the lane side's as its waves see them, `divergent`; else its mean): ket
+        // transform and Boys + R per primitive quartet (R once per column pass), bra transform per
+        // lane-side primitive pair.
+        double valuCost(const PairClass &lane, const PairClass &unif, int nc, bool divergent = false)
         {
-            const double kl = lane.n_pairs ? double(lane.n_ppairs) / lane.n_pairs : 1;
+            const double kmean = lane.n_pairs ? double(lane.n_ppairs) / lane.n_pairs : 1;
+            const double kl = divergent && lane.klane > 0 ? lane.klane : kmean;
             const double ku = unif.n_pairs ? double(unif.n_ppairs) / unif.n_pairs : 1;
             const double hl = nHerm(lane.la + lane.lb), hu = nHerm(unif.la + unif.lb);
             const double sl = nSph(lane.la) * nSph(lane.lb), su = nSph(unif.la) * nSph(unif.lb);
@@ -429,31 +431,51 @@ namespace lbx
             return kl * ku * (hl * hu * su + passes * (60 + rFlopsHost(L))) + kl * sl * hl * su;
         }
 
+        // A one-lane form whose waves wait on their longest lane for more than twice the work the
+        // mean contraction would give (long, uneven contractions on the lane side, as in the
+        // quadruple-zeta bases): for J/K the block form takes the class pair by default, whose
+        // workgroup runs one bra pair's primitives for a block of ket pairs (on def2-QZVP water
+        // clusters 7-25x faster there than the one-lane forms; on the evenly contracted bases
+        // the one-lane forms stay the default).
+        bool laneDivergent(const PairClass &lane, const PairClass &unif, int nc)
+        {
+            return valuCost(lane, unif, nc, true) > 2.0 * valuCost(lane, unif, nc);
+        }
+
         // Default variant before tuning: the cheaper one-lane-per-quartet form by the work
-        // estimate; MFMA when no such form fits the registers.
+        // estimate (with the lane side's longest-lane contraction); MFMA when no such form fits the
+        // registers; for J/K the block form when the cheaper one-lane form diverges.
         int defaultVariant(int mode, const PairClass &bra, const PairClass &ket)
         {
             const KernelSlot &vb = kernelSlot(mode, bra.la, bra.lb, ket.la, ket.lb, VAR_VALU_BRA);
             const KernelSlot &vk = kernelSlot(mode, bra.la, bra.lb, ket.la, ket.lb, VAR_VALU_KET);
-            double cb = vb.fn ? valuCost(bra, ket, vb.nc) : 1e300;
-            double ck = vk.fn ? valuCost(ket, bra, vk.nc) : 1e300;
+            double cb = vb.fn ? valuCost(bra, ket, vb.nc, true) : 1e300;
+            double ck = vk.fn ? valuCost(ket, bra, vk.nc, true) : 1e300;
             if (!vb.fn && !vk.fn)
                 return VAR_MFMA;
-            return cb <= ck ? VAR_VALU_BRA : VAR_VALU_KET;
+            const bool bra_lanes = cb <= ck;
+            if (mode == MODE_JK && kernelSlot(mode, bra.la, bra.lb, ket.la, ket.lb, VAR_BLOCK).fn &&
+                (bra_lanes ? laneDivergent(bra, ket, vb.nc) : laneDivergent(ket, bra, vk.nc)))
+                return VAR_BLOCK;
+            return bra_lanes ? VAR_VALU_BRA : VAR_VALU_KET;
         }
 
         // Default for the FP32 tier of a mixed-precision J/K: the cheaper FP32 one-lane form by
-        // the same estimate, else the FP32 MFMA form, or -1 when the class has neither (its
-        // quartets then stay FP64).
+        // the same estimate (the FP32 block form when it diverges), else the FP32 MFMA form, or
+        // -1 when the class has neither (its quartets then stay FP64).
         int defaultVariant32(const PairClass &bra, const PairClass &ket)
         {
             const KernelSlot &vb = kernelSlot(MODE_JK, bra.la, bra.lb, ket.la, ket.lb, VAR_VALU_BRA32);
             const KernelSlot &vk = kernelSlot(MODE_JK, bra.la, bra.lb, ket.la, ket.lb, VAR_VALU_KET32);
             if (!vb.fn && !vk.fn)
                 return kernelSlot(MODE_JK, bra.la, bra.lb, ket.la, ket.lb, VAR_MFMA32).fn ? VAR_MFMA32 : -1;
-            double cb = vb.fn ? valuCost(bra, ket, vb.nc) : 1e300;
-            double ck = vk.fn ? valuCost(ket, bra, vk.nc) : 1e300;
-            return cb <= ck ? VAR_VALU_BRA32 : VAR_VALU_KET32;
+            double cb = vb.fn ? valuCost(bra, ket, vb.nc, true) : 1e300;
+            double ck = vk.fn ? valuCost(ket, bra, vk.nc, true) : 1e300;
+            const bool bra_lanes = cb <= ck;
+            if (kernelSlot(MODE_JK, bra.la, bra.lb, ket.la, ket.lb, VAR_BLOCK32).fn &&
+                (bra_lanes ? laneDivergent(bra, ket, vb.nc) : laneDivergent(ket, bra, vk.nc)))
+                return VAR_BLOCK32;
+            return bra_lanes ? VAR_VALU_BRA32 : VAR_VALU_KET32;
         }
 
         __global__ void toFloatKernel(const double *__restrict__ in, float *__restrict__ out, long long n)
@@ -610,6 +632,15 @@ namespace lbx
         for (int i = 0; i < pc.n_pairs; i++)
             qs[i] = pc.Q[order[i]];
         pc.order = order;
+        double kl = 0.0;
+        for (int b = 0; b < pc.n_pairs; b += 64)
+        {
+            int m = 0;
+            for (int i = b; i < std::min(b + 64, pc.n_pairs); i++)
+                m = std::max(m, pc.pcnt[order[i]]);
+            kl += m;
+        }
+        pc.klane = kl / ((pc.n_pairs + 63) / 64);
         LBX_HIP(hipMemcpy(pc.d_order, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice));
         LBX_HIP(hipMemcpy(pc.d_q, qs.data(), qs.size() * sizeof(double), hipMemcpyHostToDevice));
     }

@@ -83,6 +83,10 @@ namespace lbx
         std::vector<int> prim_x, prim_y;   // surviving primitive pair -> primitive of a, of b
         std::vector<double> Q;          // Schwarz factor per pair (pair order)
         std::vector<int> order;         // sorted by Q descending
+        // mean over blocks of 64 consecutive sorted pairs of the block's largest primitive-pair
+        // count: the primitive loop a wave of a one-lane form runs when its lanes take this class
+        // (a wave waits for its longest lane); 0 until the pairs are sorted
+        double klane = 0.0;
         // device
         int *d_ao_a = nullptr, *d_ao_b = nullptr, *d_sh_a = nullptr, *d_sh_b = nullptr;
         int *d_poff = nullptr, *d_pcnt = nullptr, *d_order = nullptr;
