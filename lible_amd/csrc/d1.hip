@@ -19,21 +19,21 @@
 // small systems; the nuclear gradient of J/K contracts them on the fly in grad.hpp): one kernel
 // for every class pair with the angular momenta at run time, one wave per quartet, Hermite R and
 // the ket transform staged in LDS, the output block of one component accumulated in registers
-// (lanes own output elements), one component per pass over the primitive quartets.
+// (lanes own output elements) a chunk of ket columns at a time (up to 2432 elements, so (gg|gg)
+// takes three chunks), one pass over the primitive quartets per component and chunk.
 #include "lbx_internal.hpp"
 #include "runtime_l.hpp"
+
+#include <algorithm>
 
 namespace lbx
 {
     namespace
     {
-        // shells up to f on every index (the LDS and register budgets below)
-        constexpr int kD1MaxL = 3;
-        constexpr int kD1MaxL1 = 4 * kD1MaxL + 1;               // highest R level, (ff|ff) + 1
-        constexpr int kD1MaxHerm = nHerm(kD1MaxL1);             // 560
-        constexpr int kD1MaxY = nHerm(2 * kD1MaxL) * 49;        // H(ff) x S(ff)
-        constexpr int kD1MaxOut = 49 * 49;                      // S(ff) x S(ff)
-        constexpr int kD1Acc = ceilDiv(kD1MaxOut, 64);          // output elements per lane
+        // output elements per lane (a block column chunk holds at most 64 kD1Acc elements) and
+        // LDS per workgroup in doubles: R ping-pong, Y of a column chunk, Boys values, (t, u, v)
+        constexpr int kD1Acc = ceilDiv(49 * 49, 64);
+        constexpr int kD1Lds = 64 * 1024 / 8;
 
         // image permutations: image position j holds original index perm[j]
         // second derivatives of (a|b): the six distinct (i, j), i <= j, of the A block
@@ -42,12 +42,10 @@ namespace lbx
         __constant__ int kImg4[8][4] = {{0, 1, 2, 3}, {1, 0, 2, 3}, {0, 1, 3, 2}, {1, 0, 3, 2},
                                         {2, 3, 0, 1}, {3, 2, 0, 1}, {2, 3, 1, 0}, {3, 2, 1, 0}};
 
-        __global__ __launch_bounds__(64) void d1Kernel(const D1Args A)
+        // cw: ket columns (spherical cd indices) per chunk of the block
+        __global__ __launch_bounds__(64) void d1Kernel(const D1Args A, const int cw)
         {
-            __shared__ double sR[2][kD1MaxHerm];
-            __shared__ double sY[kD1MaxY];
-            __shared__ double sF[kD1MaxL1 + 1];
-            __shared__ int sTuv[kD1MaxHerm];
+            extern __shared__ double smem[];
             const int lane = threadIdx.x;
             const int la = A.la, lb = A.lb, lc = A.lc, ld = A.ld;
             // R to order L + 1 (first derivatives, spin-orbit) or L + 2 (second derivatives)
@@ -55,6 +53,9 @@ namespace lbx
             const int Hab = nHerm(la + lb), Hcd = nHerm(lc + ld), H1 = nHerm(L1);
             const int Sa = nSph(la), Sb = nSph(lb), Sc = nSph(lc), Sd = nSph(ld);
             const int Sab = Sa * Sb, Scd = Sc * Sd, nOut = Sab * Scd;
+            double *const sR0 = smem, *const sR1 = smem + H1, *const sY = smem + 2 * H1;
+            double *const sF = sY + Hab * cw;
+            int *const sTuv = reinterpret_cast<int *>(sF + L1 + 1);
             for (int h = lane; h < H1; h += 64)
                 sTuv[h] = hermTuv(h);
             __syncthreads();
@@ -73,8 +74,12 @@ namespace lbx
                 const int poff = A.bra.poff[p], pcnt = A.bra.pcnt[p];
                 const int qoff = A.ket.poff[q], qcnt = A.ket.pcnt[q];
                 const int ao[4] = {A.bra.ao_a[p], A.bra.ao_b[p], A.ket.ao_a[q], A.ket.ao_b[q]};
+                // per component, the block's ket columns in chunks of cw
                 for (int comp = 0; comp < ncomp; comp++)
+                    for (int c0 = 0; c0 < Scd; c0 += cw)
                 {
+                    // columns c0 .. c0 + cwc - 1 of the block: nO elements, lane + 64 r in acc[r]
+                    const int cwc = min(cw, Scd - c0), nO = Sab * cwc;
                     const int ci = comp / 3, x = comp % 3;
                     // internal centre: 0 A, 1 B, 2 C, 3 D; -1 for the integrals themselves
                     const int cent = (plain || extra) ? -1 : (A.kind == 2 ? (ci == 0 ? 0 : 2) : ci);
@@ -97,7 +102,7 @@ namespace lbx
                             if (lane == 0)
                                 boysRuntime(alpha * (X * X + Y * Y + Z * Z), A.boys, L1, sF);
                             __syncthreads();
-                            const double *R = hermiteRLds(sR[0], sR[1], sTuv, sF, L1, kTwoPi52 / (pq * sqrt(ps)),
+                            const double *R = hermiteRLds(sR0, sR1, sTuv, sF, L1, kTwoPi52 / (pq * sqrt(ps)),
                                                           alpha, X, Y, Z, lane, 64);
 
                             if (extra)
@@ -130,9 +135,9 @@ namespace lbx
                                         sh2 = (ii == 2) + (jj == 2);
                                         bm = A.bra.eb + static_cast<size_t>(i) * Hab * Sab;
                                     }
-                                    for (int e = lane; e < Hab * Scd; e += 64)
+                                    for (int e = lane; e < Hab * cwc; e += 64)
                                     {
-                                        const int h = e / Scd, s = e - h * Scd;
+                                        const int h = e / cwc, s = c0 + e - h * cwc;
                                         const int th = sTuv[h];
                                         const int t = th & 255, u = (th >> 8) & 255, v = th >> 16;
                                         double y = 0.0;
@@ -150,12 +155,12 @@ namespace lbx
                                     for (int r = 0; r < kD1Acc; r++)
                                     {
                                         const int o = lane + 64 * r;
-                                        if (o < nOut)
+                                        if (o < nO)
                                         {
-                                            const int sab = o / Scd, s = o - sab * Scd;
+                                            const int sab = o / cwc, s = o - sab * cwc;
                                             double a = 0.0;
                                             for (int h = 0; h < Hab; h++)
-                                                a = fma(bm[h * Sab + sab], sY[h * Scd + s], a);
+                                                a = fma(bm[h * Sab + sab], sY[h * cwc + s], a);
                                             acc[r] = fma(sg, a, acc[r]);
                                         }
                                     }
@@ -185,9 +190,9 @@ namespace lbx
                             const double *ek = A.ket.ek + static_cast<size_t>(k) * Hcd * Scd;
                             const double *e1k =
                                     a0 != 0.0 ? A.ket.e1k + (static_cast<size_t>(k) * 3 + x) * Hcd * Scd : nullptr;
-                            for (int e = lane; e < Hab * Scd; e += 64)
+                            for (int e = lane; e < Hab * cwc; e += 64)
                             {
-                                const int h = e / Scd, s = e - h * Scd;
+                                const int h = e / cwc, s = c0 + e - h * cwc;
                                 const int th = sTuv[h];
                                 const int t = th & 255, u = (th >> 8) & 255, v = th >> 16;
                                 double y = 0.0;
@@ -209,12 +214,12 @@ namespace lbx
                             for (int r = 0; r < kD1Acc; r++)
                             {
                                 const int o = lane + 64 * r;
-                                if (o < nOut)
+                                if (o < nO)
                                 {
-                                    const int sab = o / Scd, s = o - sab * Scd;
+                                    const int sab = o / cwc, s = o - sab * cwc;
                                     double a = acc[r];
                                     for (int h = 0; h < Hab; h++)
-                                        a = fma(eb[h * Sab + sab], sY[h * Scd + s], a);
+                                        a = fma(eb[h * Sab + sab], sY[h * cwc + s], a);
                                     acc[r] = a;
                                 }
                             }
@@ -222,9 +227,9 @@ namespace lbx
                             // the bra pair's own derivative: +- Eb'_x R_{h+g} Ek
                             if ((cent == 0 || cent == 1) && !A.bra_single)
                             {
-                                for (int e = lane; e < Hab * Scd; e += 64)
+                                for (int e = lane; e < Hab * cwc; e += 64)
                                 {
-                                    const int h = e / Scd, s = e - h * Scd;
+                                    const int h = e / cwc, s = c0 + e - h * cwc;
                                     const int th = sTuv[h];
                                     const int t = th & 255, u = (th >> 8) & 255, v = th >> 16;
                                     double y = 0.0;
@@ -243,12 +248,12 @@ namespace lbx
                                 for (int r = 0; r < kD1Acc; r++)
                                 {
                                     const int o = lane + 64 * r;
-                                    if (o < nOut)
+                                    if (o < nO)
                                     {
-                                        const int sab = o / Scd, s = o - sab * Scd;
+                                        const int sab = o / cwc, s = o - sab * cwc;
                                         double a = 0.0;
                                         for (int h = 0; h < Hab; h++)
-                                            a = fma(e1b[h * Sab + sab], sY[h * Scd + s], a);
+                                            a = fma(e1b[h * Sab + sab], sY[h * cwc + s], a);
                                         acc[r] = fma(sg, a, acc[r]);
                                     }
                                 }
@@ -299,9 +304,9 @@ namespace lbx
                             for (int r = 0; r < kD1Acc; r++)
                             {
                                 const int o = lane + 64 * r;
-                                if (o >= nOut)
+                                if (o >= nO)
                                     continue;
-                                const int sab = o / Scd, scd = o - sab * Scd;
+                                const int sab = o / cwc, scd = c0 + o - sab * cwc;
                                 const int i0 = sab / Sb, i1 = sab % Sb, i2 = scd / Sd, i3 = scd % Sd;
                                 const int m0 = swab ? i1 : i0, m1 = swab ? i0 : i1;
                                 const int m2 = swcd ? i3 : i2, m3 = swcd ? i2 : i3;
@@ -315,9 +320,9 @@ namespace lbx
                     for (int r = 0; r < kD1Acc; r++)
                     {
                         const int o = lane + 64 * r;
-                        if (o >= nOut)
+                        if (o >= nO)
                             continue;
-                        const int sab = o / Scd, scd = o - sab * Scd;
+                        const int sab = o / cwc, scd = c0 + o - sab * cwc;
                         const int idx[4] = {ao[0] + sab / Sb, ao[1] + sab % Sb, ao[2] + scd / Sd,
                                             ao[3] + scd % Sd};
                         const double val = acc[r];
@@ -360,11 +365,16 @@ namespace lbx
     {
         if (a.ntasks <= 0)
             return;
-        if (a.la + a.lb + a.lc + a.ld + (a.op == D1_OP_D2 ? 2 : 1) > kD1MaxL1 || nHerm(a.la + a.lb) * nSph(a.lc) * nSph(a.ld) > kD1MaxY ||
-            nSph(a.la) * nSph(a.lb) * nSph(a.lc) * nSph(a.ld) > kD1MaxOut)
+        // the block's columns in chunks: at most 64 kD1Acc elements per chunk, Y of a chunk and
+        // the fixed arrays within kD1Lds
+        const int L1 = a.la + a.lb + a.lc + a.ld + (a.op == D1_OP_D2 ? 2 : 1), H1 = nHerm(L1);
+        const int Hab = nHerm(a.la + a.lb), Sab = nSph(a.la) * nSph(a.lb), Scd = nSph(a.lc) * nSph(a.ld);
+        const int fixed = 2 * H1 + L1 + 1 + (H1 + 1) / 2;
+        const int cw = std::min({Scd, 64 * kD1Acc / Sab, (kD1Lds - fixed) / Hab});
+        if (L1 > kMaxLBoys || cw < 1)
             throw Error(LBX_ERR_INVALID, "derivative tensors: class beyond the kernel's limits");
         const int grid = static_cast<int>(a.ntasks < 16384 ? a.ntasks : 16384);
-        hipLaunchKernelGGL(d1Kernel, dim3(grid), dim3(64), 0, s, a);
+        hipLaunchKernelGGL(d1Kernel, dim3(grid), dim3(64), static_cast<size_t>(fixed + Hab * cw) * 8, s, a, cw);
         LBX_HIP(hipGetLastError());
     }
 }

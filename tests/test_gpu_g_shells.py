@@ -139,22 +139,32 @@ def test_g_one_electron(lbx, oracle):
         assert abs(gq[iq, x] - fd) <= 1e-7 * max(1.0, abs(fd)), (iq, x, gq[iq, x], fd)
 
 
-def test_g_blocks_and_limits(lbx, oracle):
-    """Per-quartet blocks with a g shell within the derivative kernel's sizes ((gs|ps), (gp|ss),
-    (sg|ss), integrals and first derivatives) against the oracle's dense tensors; a class beyond them
-    ((gg|gg) derivatives) and a main h shell report errors."""
-    from tests.test_gpu_parity import close_block, shell_offsets
+def test_g_derivatives_blocks_and_limits(lbx, oracle):
+    """Dense first-derivative tensor with g shells (every class pair up to (gg|gg), whose blocks
+    the derivative kernel takes in column chunks) and per-quartet blocks ((gs|ps), (gp|ss),
+    (sg|ss), (gg|gg); integrals and first derivatives) against the oracle; a main h shell is
+    rejected."""
+    from tests.test_gpu_parity import shell_offsets
+
+    def close_block(got, ref, scale):
+        # within 1e-12 of each element plus 1e-13 of the whole tensor's scale (the floor of the
+        # dense comparisons here; a block may be all but zero by symmetry)
+        err = np.abs(np.asarray(got) - ref)
+        assert (err <= 1e-12 * np.abs(ref) + 1e-13 * scale).all(), f"max err {err.max():.3e}"
+
     m = B.synthetic_molecule(401, 2, 3.0, RECIPE_G1)
     ctx = lbx.Context(m)
     s = oracle.Structure(m)
     e4, d4 = s.eri4(), s.eri4d1()
+    close(ctx.eri4d1().cpu().numpy(), d4, floor=1e-13)
     l, off = shell_offsets(m.shells)
     g, sh, p = np.flatnonzero(l == 4), np.flatnonzero(l == 0), np.flatnonzero(l == 1)
-    # every entry of one call has the same angular momenta per position; the last list has the
+    # every entry of one call has the same angular momenta per position; the third list has the
     # g shell second (the library stores the pair as (g, s) and swaps the block back)
     for q in (np.array([[g[0], sh[0], p[1], sh[1]], [g[1], sh[1], p[0], sh[0]]]),
               np.array([[g[1], p[0], sh[0], sh[1]], [g[0], p[1], sh[1], sh[1]]]),
-              np.array([[sh[0], g[0], sh[1], sh[1]], [sh[1], g[1], sh[0], sh[0]]])):
+              np.array([[sh[0], g[0], sh[1], sh[1]], [sh[1], g[1], sh[0], sh[0]]]),
+              np.array([[g[0], g[1], g[1], g[0]], [g[1], g[1], g[0], g[0]]])):
         got = ctx.blocks(q).cpu().numpy()
         gd = ctx.blocks(q, deriv=True).cpu().numpy()
         for t, (a, b, c, d) in enumerate(q):
@@ -162,8 +172,6 @@ def test_g_blocks_and_limits(lbx, oracle):
             sl = tuple(slice(off[x], off[x] + n) for x, n in zip((a, b, c, d), S))
             close_block(got[t, 0], e4[sl], np.abs(e4).max())
             close_block(gd[t], d4[(slice(None),) + sl], np.abs(d4).max())
-    with pytest.raises(lbx.LbxError, match="limits"):
-        ctx.blocks(np.array([[g[0], g[0], g[1], g[1]]]), deriv=True)
     mh = B.synthetic_molecule(406, 1, 1.0, [(0, 1, 1), (5, 1, 1)])
     with pytest.raises(lbx.LbxError):
         lbx.Context(mh)
