@@ -329,6 +329,48 @@ namespace
             lints::Structure s("def2-SVP", "def2-universal-jkfit", nrs_h2o, xyz_h2o);
             checkSum("eri2_diagonal", absSum(lints::eri2Diagonal(s)), 830.788976485982); // :149
         });
+        guarded("g_shells_def2qzvp", [] {
+            // a main basis with g shells (O in def2-QZVP, 7s4p3d2f1g: 57 functions): the
+            // per-class-pair functors' blocks add up to the dense tensor, and J/K equal its
+            // contraction
+            lints::Structure s("def2-qzvp", {8}, {{0.0, 0.0, 0.0}});
+            vec4d e = lints::eri4(s);
+            const size_t n = e.dim<0>();
+            auto sp = lints::shellPairData(false, s);
+            double sum = 0;
+            for (const auto &ab : sp)
+                for (const auto &cd : sp)
+                {
+                    lints::ERI4Kernel kernel(ab, cd);
+                    for (size_t p = 0; p < ab.n_pairs_; p++)
+                        for (size_t q = 0; q < cd.n_pairs_; q++)
+                            sum += absSum(kernel(p, q, ab, cd));
+                }
+            const double dense = absSum(e);
+            vec2d D(n, n);
+            for (size_t i = 0; i < n; i++)
+                for (size_t j = 0; j <= i; j++)
+                    D(i, j) = D(j, i) = std::cos(0.29 * i + 0.13 * j) / (1.0 + 0.1 * (i + j));
+            auto [J, K] = lints::coulombExchange(s, D, 0.0);
+            double err = 0, scale = 0;
+            for (size_t m = 0; m < n; m++)
+                for (size_t k = 0; k < n; k++)
+                {
+                    double j = 0, x = 0;
+                    for (size_t p = 0; p < n; p++)
+                        for (size_t q = 0; q < n; q++)
+                        {
+                            j += e(m, k, p, q) * D(p, q);
+                            x += e(m, p, k, q) * D(p, q);
+                        }
+                    err = std::max({err, std::fabs(J(m, k) - j), std::fabs(K(m, k) - x)});
+                    scale = std::max({scale, std::fabs(j), std::fabs(x)});
+                }
+            report("g_shells_def2qzvp", s.getMaxL() == 4 && n == 57 && std::fabs(sum - dense) <= 1e-11 * dense &&
+                                                err <= 1e-12 * scale,
+                   "functor sum " + std::to_string(sum) + ", dense " + std::to_string(dense) + ", J/K err " +
+                           std::to_string(err / scale) + " of scale");
+        });
         // the per-block kernels, driven as tests_ints.cpp:873-1053 drives them
         guarded("deploy_eri4_kernel", [] {
             lints::Structure structure("def2-svp", nrs_o3, xyz_o3);

@@ -57,4 +57,4 @@ def test_cpp_interface_known_answers():
             "eri2_diagonal", "ghost_atoms", "deploy_eri4_kernel", "deploy_eri3_kernel",
             "deploy_eri2_kernel", "deploy_eri4d1_kernel", "deploy_eri3d1_kernel",
             "deploy_eri2d1_kernel", "deploy_eri2d2_kernel", "deploy_eri4soc_kernel",
-            "deploy_eri3soc_kernel", "eri4_benchmark"} <= names
+            "deploy_eri3soc_kernel", "eri4_benchmark", "g_shells_def2qzvp"} <= names
