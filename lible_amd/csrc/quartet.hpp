@@ -96,10 +96,10 @@ namespace lbx
         // ---- one-lane-per-quartet plan: per-lane doubles of I (S_lane x NC) + X (H_lane x NC)
         // + R stay within kBudget (the region where the compiler keeps them in registers)
         static constexpr int kBudget = 160;
-        // Pinned: at 192 the plans of (21|11), (30|30), (22|10), (31|10) and four new forms
-        // change (static passes with uniform-side zero skipping, NC 2 with lane-side skipping),
-        // and one of them gave wrong J/K on the GPU (a few hundred elements off by up to 5%,
-        // cause not found).  Raise only with those forms checked element-wise against the oracle.
+        // Pinned: at 192 the ket-on-lanes (ff|ss) form (one static pass of all 49 columns
+        // instead of two of 35) gives a wrong lane-side J on the GPU (a few hundred elements off
+        // by up to 5%; its K and uniform-side J are right; cause not found).  Raise only with
+        // every changed form checked element-wise against the oracle.
         static_assert(kBudget == 160, "one-lane register budget: see the comment above");
         static constexpr int ncFor(int sl, int hl, int su)
         {
