@@ -47,6 +47,7 @@ namespace lbx
             int oDcd, oDbc, oDac;    // density blocks of the chunk
             int oY, oX, oW, oI;      // chunk work arrays
             int oRed;                // one double per wave: cross-wave maximum
+            int oSum;                // quartet kernel: R index of (h, g), Hab x Hcd ushorts, or -1
             int nt;                  // threads per workgroup: 64 or kGenThreads
             int doubles;
         };
@@ -91,7 +92,12 @@ namespace lbx
             g.oW = take(grad ? Sab * cw : 0);
             g.oI = take(grad ? 0 : Sab * cw);
             g.oRed = take(kGenThreads / 64);
+            // the R index of every (h, g) pair of the ket transform, when it takes at most 8 KiB
+            const int nsum = Hab * nHerm(lc + ld);
+            g.oSum = !grad && nsum <= 4096 ? take((nsum + 3) / 4) : -1;
             g.doubles = o;
+            if (o > kGenLds)
+                throw Error(LBX_ERR_INVALID, "class pair too large for the run-time-L kernel");
             // one wave when the widest loop per primitive quartet (Y, or the gradient's (h, g)
             // pairs) has little work, four otherwise
             const int widest = grad ? Hab * nHerm(lc + ld) : Hab * cw;
@@ -121,6 +127,13 @@ namespace lbx
             for (int h = lane; h < nHerm(G.L); h += NT)
                 sTuv[h] = hermTuv(h);
             __syncthreads();
+            unsigned short *const sSum = G.oSum >= 0 ? reinterpret_cast<unsigned short *>(smem + G.oSum) : nullptr;
+            if (sSum)
+            {
+                for (int e = lane; e < Hab * Hcd; e += NT)
+                    sSum[e] = static_cast<unsigned short>(hsum(sTuv[e / Hcd], sTuv[e % Hcd]));
+                __syncthreads();
+            }
 
             // contiguous task range per workgroup (the one-wave MFMA form's walk)
             const long long t0 = A.ntasks * blockIdx.x / gridDim.x;
@@ -193,15 +206,26 @@ namespace lbx
                             const double *R =
                                     hermiteRLds(sR0, sR1, sTuv, sF, G.L, kTwoPi52 / (pq * sqrt(ps)), alpha, X, Y, Z, lane, NT);
                             const double *ek = A.ket.ek + static_cast<size_t>(k) * Hcd * Scd + col0;
-                            for (int e = lane; e < Hab * cw; e += NT)
-                            {
-                                const int h = e / cw, s = e - h * cw;
-                                const int th = sTuv[h];
-                                double y = sY[e];
-                                for (int g = 0; g < Hcd; g++)
-                                    y = fma(R[hsum(th, sTuv[g])], ek[g * Scd + s], y);
-                                sY[e] = y;
-                            }
+                            if (sSum)
+                                for (int e = lane; e < Hab * cw; e += NT)
+                                {
+                                    const int h = e / cw, s = e - h * cw;
+                                    const unsigned short *row = sSum + h * Hcd;
+                                    double y = sY[e];
+                                    for (int g = 0; g < Hcd; g++)
+                                        y = fma(R[row[g]], ek[g * Scd + s], y);
+                                    sY[e] = y;
+                                }
+                            else
+                                for (int e = lane; e < Hab * cw; e += NT)
+                                {
+                                    const int h = e / cw, s = e - h * cw;
+                                    const int th = sTuv[h];
+                                    double y = sY[e];
+                                    for (int g = 0; g < Hcd; g++)
+                                        y = fma(R[hsum(th, sTuv[g])], ek[g * Scd + s], y);
+                                    sY[e] = y;
+                                }
                             __syncthreads();
                         }
                         const double *eb = A.bra.eb + static_cast<size_t>(i) * Hab * Sab;
