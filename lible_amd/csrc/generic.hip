@@ -48,6 +48,7 @@ namespace lbx
             int oY, oX, oW, oI;      // chunk work arrays
             int oRed;                // one double per wave: cross-wave maximum
             int oSum;                // quartet kernel: R index of (h, g), Hab x Hcd ushorts, or -1
+            int oE;                  // quartet kernel: the ket primitive's E columns of the chunk, or -1
             int nt;                  // threads per workgroup: 64 or kGenThreads
             int doubles;
         };
@@ -95,6 +96,10 @@ namespace lbx
             // the R index of every (h, g) pair of the ket transform, when it takes at most 8 KiB
             const int nsum = Hab * nHerm(lc + ld);
             g.oSum = !grad && nsum <= 4096 ? take((nsum + 3) / 4) : -1;
+            // the ket primitive's E for the chunk's columns, staged once for all h, when it
+            // takes at most 8 KiB
+            const int ne = nHerm(lc + ld) * cw;
+            g.oE = !grad && ne <= 1024 ? take(ne) : -1;
             g.doubles = o;
             if (o > kGenLds)
                 throw Error(LBX_ERR_INVALID, "class pair too large for the run-time-L kernel");
@@ -128,6 +133,7 @@ namespace lbx
                 sTuv[h] = hermTuv(h);
             __syncthreads();
             unsigned short *const sSum = G.oSum >= 0 ? reinterpret_cast<unsigned short *>(smem + G.oSum) : nullptr;
+            double *const sE = G.oE >= 0 ? smem + G.oE : nullptr;
             if (sSum)
             {
                 for (int e = lane; e < Hab * Hcd; e += NT)
@@ -202,10 +208,16 @@ namespace lbx
                             const double pq = P.x * Q.x, ps = P.x + Q.x, alpha = pq / ps;
                             const double X = P.y - Q.y, Y = P.z - Q.z, Z = P.w - Q.w;
                             boysLane(alpha * (X * X + Y * Y + Z * Z), A.boys, G.L, sF, lane);
+                            const double *ekg = A.ket.ek + static_cast<size_t>(k) * Hcd * Scd + col0;
+                            if (sE)
+                                for (int e = lane; e < Hcd * cw; e += NT)
+                                    sE[e] = ekg[(e / cw) * Scd + e % cw];
                             __syncthreads();
                             const double *R =
                                     hermiteRLds(sR0, sR1, sTuv, sF, G.L, kTwoPi52 / (pq * sqrt(ps)), alpha, X, Y, Z, lane, NT);
-                            const double *ek = A.ket.ek + static_cast<size_t>(k) * Hcd * Scd + col0;
+                            // E of column s, Hermite row g at ek[g * es + s]
+                            const double *ek = sE ? sE : ekg;
+                            const int es = sE ? cw : Scd;
                             if (sSum)
                                 for (int e = lane; e < Hab * cw; e += NT)
                                 {
@@ -213,7 +225,7 @@ namespace lbx
                                     const unsigned short *row = sSum + h * Hcd;
                                     double y = sY[e];
                                     for (int g = 0; g < Hcd; g++)
-                                        y = fma(R[row[g]], ek[g * Scd + s], y);
+                                        y = fma(R[row[g]], ek[g * es + s], y);
                                     sY[e] = y;
                                 }
                             else
@@ -223,7 +235,7 @@ namespace lbx
                                     const int th = sTuv[h];
                                     double y = sY[e];
                                     for (int g = 0; g < Hcd; g++)
-                                        y = fma(R[hsum(th, sTuv[g])], ek[g * Scd + s], y);
+                                        y = fma(R[hsum(th, sTuv[g])], ek[g * es + s], y);
                                     sY[e] = y;
                                 }
                             __syncthreads();
