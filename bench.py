@@ -344,7 +344,8 @@ def water_grid(n: int):
 
 def leg_qzvp() -> dict:
     """J/K of (H2O)8 in def2-QZVP (g shells on O: the run-time-L kernels): one build with the
-    default kernel choices, the autotuner, one tuned build, and the share of the g classes."""
+    default kernel choices, the autotuner, one tuned build, the share of the g classes, and one
+    J/K nuclear gradient."""
     import torch
     from lible_amd import basis as B
     from lible_amd import lbx
@@ -369,6 +370,8 @@ def leg_qzvp() -> dict:
     t_tune = time.perf_counter() - t0
     t_tuned, _ = timed(lambda: ctx.jk(D, schwarz=SCHWARZ), 1, "qzvp tuned")
     p_tuned = profiled()
+    ctx.jk_grad(D, schwarz=SCHWARZ)   # first call: derivative E matrices, task tables
+    t_grad, (grad, _) = timed(lambda: ctx.jk_grad(D, schwarz=SCHWARZ), 1, "qzvp gradient")
     nbf = ctx.nbf
     ctx.close()
 
@@ -380,7 +383,10 @@ def leg_qzvp() -> dict:
     return {"workload": "(H2O)8 def2-QZVP J/K", "nbf": nbf, "quartets": st["quartets"],
             "ms_default_kernels": t_default * 1e3, "ms_g_classes_default": g_share(p_default),
             "tune_s": t_tune, "ms_tuned": t_tuned * 1e3, "ms_g_classes_tuned": g_share(p_tuned),
-            "g_classes_tflops": g_flops / max(g_share(p_tuned), 1e-9) / 1e9}
+            "g_classes_tflops": g_flops / max(g_share(p_tuned), 1e-9) / 1e9,
+            "ms_gradient": t_grad * 1e3,
+            "max_abs_translation_sum": float(grad.sum(dim=0).abs().max()),
+            "max_abs_gradient": float(grad.abs().max())}
 
 
 LEGS = {"c1": leg_c1, "c3": leg_c3, "c4": leg_c4, "c5": leg_c5, "ri": leg_ri, "scf": leg_scf,
