@@ -246,3 +246,18 @@ def test_g_water_qzvp(lbx, oracle):
     print(f"H2O def2-QZVP RHF: GPU {res.energy:.10f}, oracle {E_ref:.10f} Eh", flush=True)
     assert abs(res.energy - E_ref) <= 1e-9
     assert -76.0676 < res.energy < -76.066
+
+
+def test_g_ri_sums_batch(lbx, oracle):
+    """Batched RI sums (one context over several molecules) with g main shells: every molecule's
+    sums equal the oracle's for that molecule alone, in FP64 and with the mixed split (the g
+    classes have no FP32 form and stay FP64)."""
+    mols = [B.synthetic_molecule(410 + k, 2 + k, 3.5, RECIPE_G1, RECIPE_AUX_H) for k in range(3)]
+    ctx = lbx.Context.batch(mols)
+    got, _ = ctx.ri_sums()
+    got = got.cpu().numpy()
+    for k, m in enumerate(mols):
+        close(got[k], oracle.Structure(m).ri_sums(), rel=1e-12, floor=1e-12)
+    mixed, sm = ctx.ri_sums(precision=lbx.LBX_MIXED, mixed_thresh=1e-5)
+    gm = mixed.cpu().numpy()
+    assert np.abs(gm[:, [0, 2]] - got[:, [0, 2]]).max() <= 1e-8 * np.abs(got[:, [0, 2]]).max()
